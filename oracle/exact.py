@@ -1,0 +1,247 @@
+"""Exact rational oracle -- TEST INFRASTRUCTURE ONLY.
+
+Pure-Python restatements (Fractions and Python ints, no floating point in
+any decision) used to pin the C oracle and the product on small cases:
+
+- `ellipsoid_points` / `kernel_reference`: the exact output set and DFS
+  order of the Fincke-Pohst kernel (reference tests/oracle.py:28-121,
+  contract pfvs/fp_kernel/fp_kernel.h:25-63).
+- `hnf`: canonical row HNF (flint's fmpz_mat_hnf form, which
+  pfvs/coni.py:172-202 uses).
+- `pfv_checks`: the 7 coniPFV constraints (reference README.md:38-45,
+  pfvs/pfv.py:651-715).
+- `is_singular`: exact rank test (pfvs/util.py:224-243).
+- `coni_points`: the exact per-p point set of the batched pipeline
+  (fp_kernel.pyx:480-622) computed from scratch with Python ints: the
+  lattice {M in Mbasis Z^h : (Z p).M = 0}, its ellipsoid, and the M0 / gcd
+  cuts, independent of the basis any implementation picks.
+"""
+
+from __future__ import annotations
+
+import math
+from fractions import Fraction
+
+
+def ldl_upper(mat):
+    """mat = R^T diag(d) R, R unit upper triangular, exactly."""
+    n = len(mat)
+    A = [[Fraction(int(mat[i][j])) for j in range(n)] for i in range(n)]
+    d = [Fraction(0)] * n
+    R = [[Fraction(int(i == j)) for j in range(n)] for i in range(n)]
+    for i in range(n):
+        d[i] = A[i][i] - sum(d[k] * R[k][i] ** 2 for k in range(i))
+        if d[i] <= 0:
+            raise ValueError("mat is not positive definite")
+        for j in range(i + 1, n):
+            R[i][j] = (A[i][j] - sum(d[k] * R[k][i] * R[k][j] for k in range(i))) / d[i]
+    return d, R
+
+
+def _isqrt_floor(x: Fraction) -> int:
+    return math.isqrt(x.numerator * x.denominator) // x.denominator
+
+
+def ellipsoid_points(mat, qmax):
+    """All integer c with c^T mat c <= qmax, last coordinate outermost and
+    each coordinate ascending (the kernels' DFS order)."""
+    n = len(mat)
+    d, R = ldl_upper(mat)
+    out, c = [], [0] * n
+
+    def rec(i, rem):
+        if i < 0:
+            out.append(tuple(c))
+            return
+        t = sum((R[i][j] * c[j] for j in range(i + 1, n)), Fraction(0))
+        s = _isqrt_floor(rem / d[i]) + 1
+        for v in range(math.floor(-t) - s, math.ceil(-t) + s + 1):
+            e = d[i] * (v + t) ** 2
+            if e <= rem:
+                c[i] = v
+                rec(i - 1, rem - e)
+        c[i] = 0
+
+    rec(n - 1, Fraction(qmax))
+    return out
+
+
+def qform(mat, c):
+    n = len(c)
+    return sum(int(mat[i][j]) * c[i] * c[j] for i in range(n) for j in range(n))
+
+
+def row_gcd(H, c):
+    g = 0
+    for row in H:
+        g = math.gcd(g, sum(int(h) * x for h, x in zip(row, c)))
+    return g
+
+
+def qmax_of(Q, dilation):
+    return math.floor(Fraction(dilation) * Q + Fraction(1, 10**7))
+
+
+def kernel_reference(mat, Q, dilation, H, linvec=None, linmin=None, strict=False, qmax=None):
+    """(points, qs): c kept iff q(c) <= qmax, linvec.c >= linmin and
+    (g == 0 or Q g >= q, strictly if `strict`), g = gcd(H c)."""
+    if qmax is None:
+        qmax = qmax_of(Q, dilation)
+    pts, qs = [], []
+    for c in ellipsoid_points(mat, qmax):
+        q = qform(mat, c)
+        if linvec is not None and sum(int(a) * b for a, b in zip(linvec, c)) < linmin:
+            continue
+        g = row_gcd(H, c)
+        if g != 0 and (Q * g <= q if strict else Q * g < q):
+            continue
+        pts.append(c)
+        qs.append(q)
+    return pts, qs
+
+
+def hnf(A):
+    """Canonical row HNF by repeated Euclidean column clearing."""
+    H = [[int(x) for x in r] for r in A]
+    r = len(H)
+    if r == 0:
+        return []
+    n = len(H[0])
+    top = 0
+    for col in range(n):
+        if top >= r:
+            break
+        # gcd-combine the column entries of rows top.. into row top
+        for i in range(top + 1, r):
+            a, b = H[top][col], H[i][col]
+            if b == 0:
+                continue
+            g, s, t = _xgcd(a, b)
+            ra, rb = H[top], H[i]
+            H[top] = [s * x + t * y for x, y in zip(ra, rb)]
+            H[i] = [(-b // g) * x + (a // g) * y for x, y in zip(ra, rb)]
+        if H[top][col] == 0:
+            continue
+        if H[top][col] < 0:
+            H[top] = [-x for x in H[top]]
+        p = H[top][col]
+        for i in range(top):
+            q = H[i][col] // p
+            if q:
+                H[i] = [x - q * y for x, y in zip(H[i], H[top])]
+        top += 1
+    return H
+
+
+def _xgcd(a, b):
+    r0, r1, s0, s1, t0, t1 = a, b, 1, 0, 0, 1
+    while r1:
+        q = r0 // r1
+        r0, r1 = r1, r0 - q * r1
+        s0, s1 = s1, s0 - q * s1
+        t0, t1 = t1, t0 - q * t1
+    if r0 < 0:
+        r0, s0, t0 = -r0, -s0, -t0
+    return r0, s0, t0
+
+
+def rank(A):
+    """Exact rank over Q (fraction-free elimination)."""
+    M = [[Fraction(int(x)) for x in r] for r in A]
+    if not M:
+        return 0
+    rk, n = 0, len(M[0])
+    for col in range(n):
+        piv = next((i for i in range(rk, len(M)) if M[i][col] != 0), None)
+        if piv is None:
+            continue
+        M[rk], M[piv] = M[piv], M[rk]
+        for i in range(rk + 1, len(M)):
+            if M[i][col] != 0:
+                f = M[i][col] / M[rk][col]
+                M[i] = [x - f * y for x, y in zip(M[i], M[rk])]
+        rk += 1
+    return rk
+
+
+def is_singular(N):
+    return rank(N) < len(N)
+
+
+def pfv_checks(kappa, c2, h11, h21, K, M, p):
+    """The 7 coniPFV constraints for (K, M) with p-vector p (p[0] = 0), all
+    exact.  Returns a dict name -> bool (pfvs/pfv.py:651-715 and README
+    constraints 1-7; a and b as pfvs/cydata.py:252-303)."""
+    h = h11
+    K = [int(x) for x in K]
+    M = [int(x) for x in M]
+    p = [int(x) for x in p]
+    a = [[int(kappa[x][x][y]) if x >= y else int(kappa[x][y][y]) for y in range(h)] for x in range(h)][1:]
+    b = [int(x) for x in c2]
+    b[0] += 2
+    N = [[sum(int(kappa[i][j][k]) * M[k] for k in range(h)) for j in range(h)] for i in range(h)]
+    Nr = [row[1:] for row in N[1:]]
+    KM = sum(x * y for x, y in zip(K, M))
+    out = {
+        "a_even": all(sum(ai * m for ai, m in zip(row, M)) % 2 == 0 for row in a),
+        "b_24": sum(bi * m for bi, m in zip(b, M)) % 24 == 0,
+        "tadpole": 0 <= -KM <= h11 + h21 + 4,
+        "K_nonzero": any(K),
+        "N_invertible": not is_singular(Nr),
+        "orthogonal": p[0] == 0 and sum(x * y for x, y in zip(p[1:], K[1:])) == 0,
+    }
+    if out["N_invertible"]:
+        Np = [sum(Nr[i][j] * p[1 + j] for j in range(h - 1)) for i in range(h - 1)]
+        out["NpK"] = Np == K[1:]
+    else:
+        out["NpK"] = False
+    return out
+
+
+def coni_points(kappa, Mbasis, p_full, Q, qmax, linmin):
+    """Exact point set of one p-vector as a set of (M, Kn, q) tuples, with no
+    floating point: the same lattice, ellipsoid and cuts as the pipeline,
+    but in a basis of our own (so only the set -- not the order -- is
+    comparable).  Small cases only."""
+    from itertools import product as _p  # noqa: F401
+    h = len(p_full)
+    p = [int(x) for x in p_full]
+    Mb = [[int(x) for x in r] for r in Mbasis]
+    Z = [[sum(int(kappa[i][j][k]) * p[k] for k in range(h)) for j in range(h)] for i in range(h)]
+    T = [sum(Z[i][j] * p[j] for j in range(h)) for i in range(h)]
+    v = [sum(T[i] * Mb[i][j] for i in range(h)) for j in range(h)]
+    O = _orth(v)                                  # rows: coefficient vectors
+    B = [[sum(Mb[i][l] * O[a][l] for l in range(h)) for i in range(h)] for a in range(len(O))]
+    d = len(B)
+    ZB = [[sum(Z[i][j] * B[a][j] for j in range(h)) for i in range(h)] for a in range(d)]
+    mat = [[-sum(B[a][i] * ZB[b][i] for i in range(h)) for b in range(d)] for a in range(d)]
+    out = set()
+    for c in ellipsoid_points(mat, qmax):
+        M = [sum(B[a][i] * c[a] for a in range(d)) for i in range(h)]
+        Kn = [sum(ZB[a][i] * c[a] for a in range(d)) for i in range(h)]
+        q = qform(mat, c)
+        if M[0] < linmin:
+            continue
+        g = 0
+        for x in Kn[1:]:
+            g = math.gcd(g, x)
+        if g != 0 and Q * g <= q:
+            continue
+        out.add((tuple(M), tuple(Kn), q))
+    return out
+
+
+def _orth(v):
+    n = len(v)
+    w = list(v)
+    U = [[int(i == j) for j in range(n)] for i in range(n)]
+    for k in range(1, n):
+        if w[k] == 0:
+            continue
+        g, s, t = _xgcd(w[0], w[k])
+        a, b = w[0], w[k]
+        w[0], w[k] = g, 0
+        r0, rk = U[0], U[k]
+        U[0] = [s * x + t * y for x, y in zip(r0, rk)]
+        U[k] = [(-b // g) * x + (a // g) * y for x, y in zip(r0, rk)]
+    return U[1:]

@@ -1,0 +1,218 @@
+"""The synthetic CY instance family G(h11, h21, seed) and its p-vector sets.
+
+G: coordinates c (index 0, the conifold), f, a, b, then k_1..k_{h-4}.
+eta is the integer symmetric form on {c, a, b, k} with eta_ab = 1,
+eta_cc = eta_kk = -2 and all else 0; kappa_{f,i,j} = eta_ij for every
+permutation of the indices and every other kappa entry is 0.  The Kahler
+cone hyperplanes are e_f, e_c (the conifold n-hat), e_k, e_a - e_c - sum e_k
+and e_b - e_c - sum e_k.  c2_f = 24 and the other c2 entries are seeded
+uniform even values in [0, 50].  Q = h11 + h21 + 2.  Optionally the data is
+conjugated by a seeded unimodular U made of 3 h elementary row operations
+(multipliers in -2..2) that never touch coordinate 0.
+
+p-vector rule.  With p_c = 0 the cone conditions read p_f >= 1, p_k >= 1,
+p_a > sum p_k and p_b > sum p_k, so a bound E on the raw entries leaves no
+admissible p at h11 >= 20.  The bound E is therefore applied to the cone
+coordinates (p_f, p_k, x_a, x_b) with p_a = sum p_k + x_a and
+p_b = sum p_k + x_b, all in [1, E]; non-primitive vectors are dropped.  The
+resulting p is mapped to the conjugated basis by p' = U^{-T} p.
+"""
+
+from __future__ import annotations
+
+import itertools
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .cydata import CYData
+
+
+@dataclass
+class Instance:
+    h11: int
+    h21: int
+    seed: int
+    Q: int
+    data: CYData
+    U: np.ndarray                 # conjugation (rows act on coordinates)
+    UinvT: np.ndarray             # maps cone-basis p to the data's basis
+    meta: dict = field(default_factory=dict)
+
+
+def _eta(h):
+    eta = np.zeros((h, h), dtype=np.int64)
+    c, a, b = 0, 2, 3
+    eta[a, b] = eta[b, a] = 1
+    eta[c, c] = -2
+    for k in range(4, h):
+        eta[k, k] = -2
+    return eta
+
+
+def _unconjugated(h, rng):
+    if h < 5:
+        raise ValueError("G needs h11 >= 5")
+    f = 1
+    eta = _eta(h)
+    kappa = np.zeros((h, h, h), dtype=np.int64)
+    rest = [i for i in range(h) if i != f]
+    for i in rest:
+        for j in rest:
+            v = eta[i, j]
+            if v:
+                for (x, y, z) in set(itertools.permutations((f, i, j))):
+                    kappa[x, y, z] = v
+    e = np.eye(h, dtype=np.int64)
+    ks = list(range(4, h))
+    sk = e[ks].sum(axis=0)
+    H = np.vstack([e[f], e[0]] + [e[k] for k in ks]
+                  + [e[2] - e[0] - sk, e[3] - e[0] - sk])
+    c2 = 2 * rng.integers(0, 26, h)
+    c2[f] = 24
+    return kappa, c2.astype(np.int64), H
+
+
+def _conjugator(h, rng):
+    U = np.eye(h, dtype=np.int64)
+    for _ in range(3 * h):
+        i, j = rng.choice(np.arange(1, h), 2, replace=False)
+        m = int(rng.integers(-2, 3))
+        U[i] = U[i] + m * U[j]
+    return U
+
+
+def _inv_unimodular(U):
+    h = U.shape[0]
+    A = [[int(x) for x in row] + [int(i == j) for j in range(h)] for i, row in enumerate(U)]
+    # Gauss-Jordan over Z works for a unimodular U with exact Fractions
+    from fractions import Fraction
+    M = [[Fraction(x) for x in r] for r in A]
+    for col in range(h):
+        piv = next(r for r in range(col, h) if M[r][col] != 0)
+        M[col], M[piv] = M[piv], M[col]
+        pv = M[col][col]
+        M[col] = [x / pv for x in M[col]]
+        for r in range(h):
+            if r != col and M[r][col] != 0:
+                fct = M[r][col]
+                M[r] = [x - fct * y for x, y in zip(M[r], M[col])]
+    inv = np.array([[int(x) for x in r[h:]] for r in M], dtype=np.int64)
+    assert np.array_equal(inv @ U, np.eye(h, dtype=np.int64))
+    return inv
+
+
+def _transform3(kappa, U):
+    """kappa'_{abc} = U_ai U_bj U_ck kappa_ijk, one index at a time."""
+    t = np.tensordot(U, kappa, axes=([1], [0]))                     # a j k
+    t = np.tensordot(t, U, axes=([1], [1])).transpose(0, 2, 1)      # a b k
+    return np.tensordot(t, U, axes=([2], [1]))                      # a b c
+
+
+def make_G(h11: int, h21: int, seed: int, conjugate: bool = True) -> Instance:
+    """The instance G(h11, h21, seed) as coni CYData (conifold at index 0,
+    identity change of basis: the data is already in the coni basis)."""
+    rng = np.random.default_rng(seed)
+    kappa, c2, H = _unconjugated(h11, rng)
+    if conjugate:
+        U = _conjugator(h11, rng)
+    else:
+        U = np.eye(h11, dtype=np.int64)
+    kap = _transform3(kappa, U)
+    c2u = U @ c2
+    Hu = H @ U.T
+    e0 = np.zeros(h11, dtype=np.int64)
+    e0[0] = 1
+    data = CYData(h21, kap, c2u, Hu, coni_curve=e0, coni_cob=np.eye(h11, dtype=np.int64))
+    UinvT = _inv_unimodular(U).T
+    return Instance(h11, h21, seed, h11 + h21 + 2, data, U, UinvT)
+
+
+def _cone_to_p(h, pf, pk, xa, xb):
+    p = np.zeros(h, dtype=np.int64)
+    p[1] = pf
+    sk = int(np.sum(pk))
+    p[2] = sk + xa
+    p[3] = sk + xb
+    p[4:] = pk
+    return p
+
+
+def all_pvecs(inst: Instance, E: int) -> np.ndarray:
+    """Every primitive admissible p (without the coni entry, in the data's
+    basis) whose cone coordinates all lie in [1, E]."""
+    h = inst.h11
+    nk = h - 4
+    out = []
+    for t in itertools.product(range(1, E + 1), repeat=nk + 3):
+        pf, xa, xb = t[0], t[1], t[2]
+        pk = np.array(t[3:], dtype=np.int64)
+        p = _cone_to_p(h, pf, pk, xa, xb)
+        if math.gcd(*[int(x) for x in p]) != 1:
+            continue
+        out.append(inst.UinvT @ p)
+    P = np.array(out, dtype=np.int64).reshape(-1, h)
+    return P[:, 1:]
+
+
+def random_pvecs(inst: Instance, n: int, E: int, seed: int) -> np.ndarray:
+    """n distinct primitive admissible p with cone coordinates uniform in
+    [1, E] (seeded), in the data's basis without the coni entry."""
+    h = inst.h11
+    rng = np.random.default_rng(seed)
+    seen, out = set(), []
+    tries = 0
+    total = E ** (h - 1)
+    if n > total:
+        raise ValueError("fewer admissible p-vectors than requested")
+    while len(out) < n:
+        tries += 1
+        if tries > 50 * n + 1000:
+            raise RuntimeError("could not draw enough distinct p-vectors")
+        t = rng.integers(1, E + 1, h - 1)
+        key = tuple(int(x) for x in t)
+        if key in seen:
+            continue
+        seen.add(key)
+        pk = t[3:].astype(np.int64)
+        p = _cone_to_p(h, int(t[0]), pk, int(t[1]), int(t[2]))
+        if math.gcd(*[int(x) for x in p]) != 1:
+            continue
+        out.append(inst.UinvT @ p)
+    return np.array(out, dtype=np.int64)[:, 1:]
+
+
+# The workloads (Q = h11 + h21 + 2, M0min = 13 throughout)
+# -------------------------------------------------------
+def config_cpu_plain():
+    """G(5, 70, 0): all admissible p with cone coordinates <= 4, dilation 1."""
+    inst = make_G(5, 70, 0)
+    return inst, all_pvecs(inst, 4), 1
+
+
+def config_plain_gpu():
+    """G(8, 120, 1): all admissible p with cone coordinates <= 6, dilation 1."""
+    inst = make_G(8, 120, 1)
+    return inst, all_pvecs(inst, 6), 1
+
+
+BATCH_MEMBERS = [(6, 90, 2), (10, 150, 3), (14, 200, 4), (20, 250, 5)]
+
+
+def config_batch(n_per=4096, E=12, members=BATCH_MEMBERS, seed_offset=0):
+    """The mixed batch: 4 instances x n_per p-vectors (cone coordinates <= E),
+    per-p dilation uniform in 1..4 (seeded by the instance seed)."""
+    out = []
+    for (h, h21, s) in members:
+        inst = make_G(h, h21, s)
+        ps = random_pvecs(inst, n_per, E, seed=1000 + s + seed_offset)
+        dil = np.random.default_rng(2000 + s + seed_offset).integers(1, 5, len(ps))
+        out.append((inst, ps, dil))
+    return out
+
+
+def config_large_tree(n=2048, E=20):
+    """G(30, 300, 6): n p-vectors with cone coordinates <= E, dilation 3."""
+    inst = make_G(30, 300, 6)
+    return inst, random_pvecs(inst, n, E, seed=1006), 3
