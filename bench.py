@@ -1,0 +1,131 @@
+"""Benchmark: FP nodes/s of the batched persistent coniZpM search against the
+plain one-thread-per-p form, on the h11 = 20 member of the mixed batch
+(G(20, 250, 5), 4096 seeded p-vectors, cone coordinates <= 12, per-p
+dilation uniform in 1..4), with the oracle's CPU path timed on a bounded
+sample of the same p-vectors.
+
+One step = one pass of the device pipeline (preparation, frontier roots,
+search, post-processing with the exact filter) over the whole batch, inputs
+resident in HBM.  Prints one JSON line.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+P_VALU = None   # lane-ops/s, from the device's CU count and clock
+
+
+def workload(n_per=4096):
+    from pfvs_amd.instances import BATCH_MEMBERS, config_batch
+    members = [m for m in BATCH_MEMBERS if m[0] == 20]
+    (inst, ps, dil), = config_batch(n_per=n_per, members=members)
+    geoms, pgeo = [], np.zeros(len(ps), np.int32)
+    kap, Mb = inst.data.kappa_cob, inst.data.M_lattice()
+    for k, D in enumerate([1, 2, 3, 4]):
+        geoms.append(dict(kappa=kap, Mbasis=Mb, Q=inst.Q, dilation=D, M0min=13))
+        pgeo[dil == D] = k
+    pf = np.hstack([np.zeros((len(ps), 1), np.int64), ps])
+    return inst, geoms, pf, pgeo, dil
+
+
+def time_mode(gpu, geoms, pf, pgeo, mode, steps, warmup, **kw):
+    import torch
+    for _ in range(warmup):
+        gpu.run(geoms, pf, pgeo, mode=mode, keep_points=False, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = []
+    for _ in range(steps):
+        r = gpu.run(geoms, pf, pgeo, mode=mode, keep_points=False, **kw)
+        stats.append(r["stats"])
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    return wall, stats, r
+
+
+def cpu_baseline(inst, pf, dil, budget_s=15.0):
+    """The oracle (single thread) over p-vectors of the same batch, in a
+    fixed seeded order, until budget_s seconds have elapsed."""
+    from oracle import cpu
+    kap, Mb = inst.data.kappa_cob, inst.data.M_lattice()
+    order = np.random.default_rng(7).permutation(len(pf))
+    nodes = done = 0
+    t0 = time.perf_counter()
+    for i in order:
+        M, K, q, pidx, st, cnt = cpu.coni_batch(kap, Mb, pf[i:i + 1], inst.Q,
+                                                cpu.qmax_of(inst.Q, int(dil[i])))
+        nodes += cnt[0]
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    t = time.perf_counter() - t0
+    return nodes / t, done, t
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n-per", type=int, default=4096)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    from pfvs_amd import gpu
+    if not gpu.available():
+        raise SystemExit("HIP backend unavailable: " + gpu.unavailable_reason())
+    inst, geoms, pf, pgeo, dil = workload(args.n_per)
+    print(f"# workload: G(20,250,5) {len(pf)} p-vectors", file=sys.stderr, flush=True)
+    wall_b, st_b, r_b = time_mode(gpu, geoms, pf, pgeo, "batched", args.steps, args.warmup)
+    nodes = st_b[-1]["nodes"]
+    ms_dev = np.mean([s["ms_total"] for s in st_b])
+    ms_search = np.mean([s["ms_search"] for s in st_b])
+    print(f"# batched: {nodes:.0f} nodes, device {ms_dev:.2f} ms, search {ms_search:.2f} ms, "
+          f"wall {wall_b*1e3:.1f} ms, passes {st_b[-1]['passes']:.0f}", file=sys.stderr, flush=True)
+    out = {
+        "metric": "FP nodes/s (batched persistent coniZpM, h11=20)",
+        "value": nodes / (ms_dev * 1e-3),
+        "unit": "nodes/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_dev,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int64+f64", "data": "synthetic",
+        "config": {"workload": "G(20,250,5), 4096 p-vectors (cone coords <= 12, p_f >= 3), "
+                               "dilation uniform 1..4, M0min 13", "parallelism": "none"},
+        "detail": {"nodes": nodes, "points": st_b[-1]["points"], "kept": st_b[-1]["kept"],
+                   "pfvs": st_b[-1]["expansions"], "search_ms": ms_search,
+                   "prep_ms": float(np.mean([s["ms_prep"] for s in st_b])),
+                   "post_ms": float(np.mean([s["ms_post"] for s in st_b])),
+                   "passes": st_b[-1]["passes"], "host_p": st_b[-1]["host_p"],
+                   "coniPFVs_per_s": st_b[-1]["expansions"] / (ms_dev * 1e-3),
+                   "wall_ms": wall_b * 1e3},
+    }
+    if not args.no_plain:
+        wall_p, st_p, _ = time_mode(gpu, geoms, pf, pgeo, "plain", 1, 0)
+        ms_p = st_p[-1]["ms_total"]
+        out["detail"]["plain_ms"] = ms_p
+        out["detail"]["plain_nodes_per_s"] = st_p[-1]["nodes"] / (ms_p * 1e-3)
+        out["detail"]["batched_over_plain"] = out["value"] / out["detail"]["plain_nodes_per_s"]
+        assert st_p[-1]["nodes"] == nodes, "plain and batched forms visited different nodes"
+    if not args.no_cpu:
+        v, done, t = cpu_baseline(inst, pf, dil, args.cpu_seconds)
+        out["cpu_baseline"] = {"value": v, "unit": "nodes/s", "cores": 1, "kind": "port",
+                               "sample": f"{done} p-vectors of the same batch in {t:.1f} s "
+                                         "(seeded order)"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,162 @@
+/*
+ * pfvs_amd.h -- C ABI of libpfvs_amd.so, the MI355X (gfx950) coniZpM search.
+ *
+ * Plain C types only (pointers and sizes); no torch or HIP types.  All
+ * arrays are host memory owned by the caller, except the output arrays,
+ * which the library allocates with malloc and the caller releases with the
+ * matching *_free call.
+ *
+ * Part 1 replaces the reference's GPU library interface one for one (the
+ * ctypes binding in pfvs/gpu.py:37-82 loads these names):
+ *   pfg_coni_batch   <- pfvs/fp_kernel/cuda/pfvs_gpu.cu:64-97, 489-751
+ *   pfg_output_free  <- pfvs_gpu.cu:64-97
+ *   pfg_max_h11      <- pfvs_gpu.cu:46-48, 519-522
+ *   pfg_backend      <- pfvs_gpu.cu:64-97 (returns "hip")
+ *   pfg_device_count <- pfvs_gpu.cu:64-97
+ *   pfg_release      <- pfvs_gpu.cu:753-758
+ * Part 2 adds entry points of this library: the run call with a choice of
+ * search form and counters, the single-problem kernel batch, and the
+ * serving session.
+ */
+#ifndef PFVS_AMD_H
+#define PFVS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ part 1 */
+
+/* Same field order and meaning as the reference's pfg_input (gpu.py:37-43):
+ * per geometry h11, Q, qmax = floor(dilation Q + 1e-7), linmin =
+ * ceil(M0min - 1e-9), kappa (h^3, concatenated) and Mbasis (h^2, columns
+ * are basis vectors); p-vectors padded to pfg_max_h11() entries with the
+ * leading coni 0; pgeo the geometry of each p-vector. */
+typedef struct {
+    int n_geo;
+    const int32_t *h;
+    const int64_t *Q, *qmax, *linmin;
+    const int64_t *kappa;
+    const int64_t *Mbasis;
+    int64_t n_p;
+    const int64_t *ps;
+    const int32_t *pgeo;
+    int device;
+    int64_t batch;
+    int verbose;
+} pfg_input;
+
+/* Points grouped by p-vector (ascending), sorted by (M, Kn) within one;
+ * rows padded to pfg_max_h11() with zeros; pstat[i] 0 done, 1 redo on the
+ * exact host path (none of its points are returned). */
+typedef struct {
+    int64_t n;
+    int64_t *M, *Kn;
+    int64_t *q, *pidx;
+    int8_t *pstat;
+    double seconds_gpu;
+    char err[256];
+} pfg_output;
+
+/* 0 on success; nonzero with a message in out->err. */
+int pfg_coni_batch(const pfg_input *in, pfg_output *out);
+void pfg_output_free(pfg_output *out);
+int pfg_max_h11(void);
+const char *pfg_backend(void);
+int pfg_device_count(void);
+void pfg_release(int device);
+
+/* ------------------------------------------------------------ part 2 */
+
+enum { PFA_MODE_PLAIN = 0, PFA_MODE_BATCHED = 1, PFA_MODE_HOST = 2 };
+
+/* Geometry/p-vector layout as pfg_input.  mode: PFA_MODE_PLAIN (one
+ * launch per p-vector batch, one thread per p-vector), PFA_MODE_BATCHED
+ * (persistent wavefronts, LIFO frontier stacks, global frontier queue),
+ * PFA_MODE_HOST (the same search on host threads).  Tunables <= 0 take
+ * defaults.  filter != 0 also evaluates the exact per-point filter
+ * (K-integrality count and det != 0). */
+typedef struct {
+    int n_geo;
+    const int32_t *h;
+    const int64_t *Q, *qmax, *linmin;
+    const int64_t *kappa;
+    const int64_t *Mbasis;
+    int64_t n_p;
+    const int64_t *ps;
+    const int32_t *pgeo;
+    int device;
+    int mode;
+    int waves_per_cu;     /* batched: resident waves per CU */
+    int budget;           /* batched: steps per wave and pass */
+    int stack_cap;        /* batched: entries per wave stack */
+    int host_threads;     /* host mode */
+    int filter;
+    int sort;             /* 1: group by p and sort by (M, Kn) */
+    int keep_points;      /* 0: return counters only (no point arrays) */
+    int verbose;
+} pfa_run_input;
+
+/* stats[]: see PFA_ST_* */
+enum {
+    PFA_ST_NODES = 0, PFA_ST_CAND, PFA_ST_LEAF, PFA_ST_POINTS, PFA_ST_PASSES,
+    PFA_ST_MS_PREP, PFA_ST_MS_ROOTS, PFA_ST_MS_SEARCH, PFA_ST_MS_POST, PFA_ST_MS_TOTAL,
+    PFA_ST_SEARCH_LAUNCHES, PFA_ST_HOST_P, PFA_ST_QUEUE_PEAK, PFA_ST_MS_WALL,
+    PFA_ST_KEPT, PFA_ST_EXPANSIONS, PFA_ST_NONSING, PFA_ST_UNDECIDED, PFA_ST_N
+};
+
+typedef struct {
+    int64_t n;            /* points */
+    int64_t *M, *Kn;      /* n x pfg_max_h11() */
+    int64_t *q, *pidx;
+    int8_t *pstat;        /* n_p: 0 done, 1 exact host path, < 0 kernel status */
+    int32_t *pcode;       /* n_p: detailed code (preparation stage codes) */
+    int64_t *nexp;        /* n: (g, K0) pairs of the point (filter), -1 unknown */
+    int8_t *nonsing;      /* n: 1 det != 0, 0 det == 0, 2 undecided */
+    double stats[PFA_ST_N];
+    char err[256];
+} pfa_run_output;
+
+int pfa_coni_run(const pfa_run_input *in, pfa_run_output *out);
+void pfa_run_output_free(pfa_run_output *out);
+
+/* Single-problem kernel batch (the contract of conipfv_kernel,
+ * pfvs/fp_kernel/fp_kernel.pyx:258-318 / fp_kernel.h:25-63): for problem
+ * k of dimension dims[k], mat (d*d), H (nh[k] rows of d, echelon form,
+ * int64), leaf rows L (nl[k] rows, int64; used when some H rows were
+ * dropped for not fitting int64), linvec (d, or has_lin[k] = 0), Q, qmax,
+ * linmin, strict.  Arrays of all problems are concatenated.  Outputs the
+ * points in depth-first order per problem (pts int32 (n, dmax), qs, pidx)
+ * and per-problem status and node counts. */
+typedef struct {
+    int n_prob;
+    const int32_t *dims, *nh, *nl, *has_lin, *strict;
+    const int64_t *mat, *H, *L, *linvec, *Q, *qmax, *linmin;
+    int device;
+    int mode;
+} pfa_kernel_input;
+
+typedef struct {
+    int64_t n;
+    int32_t *pts;         /* n x dmax */
+    int64_t *qs, *pidx;
+    int32_t *status;      /* n_prob */
+    int64_t *nodes;       /* n_prob */
+    int dmax;
+    char err[256];
+} pfa_kernel_output;
+
+int pfa_kernel_batch(const pfa_kernel_input *in, pfa_kernel_output *out);
+void pfa_kernel_output_free(pfa_kernel_output *out);
+
+/* Library facts: number of compute units of `device` and the build's
+ * architecture string. */
+int pfa_device_cus(int device);
+const char *pfa_arch(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

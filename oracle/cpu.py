@@ -16,6 +16,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 
+# GMP 6 (headers and library) ships with the image's conda prefix
+GMP_PREFIX = os.environ.get("PFVS_GMP_PREFIX", "/opt/conda")
+
 _i64p = ctypes.POINTER(ctypes.c_int64)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _lib = None
@@ -27,8 +30,9 @@ def build(force: bool = False) -> str:
     os.makedirs(os.path.dirname(LIB_PATH), exist_ok=True)
     src = os.path.join(HERE, "pfvs_oracle.c")
     if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
-        subprocess.run(["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-o", LIB_PATH, src, "-lm"],
-                       check=True)
+        subprocess.run(["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-o", LIB_PATH, src,
+                        "-I" + GMP_PREFIX + "/include", "-L" + GMP_PREFIX + "/lib",
+                        "-Wl,-rpath," + GMP_PREFIX + "/lib", "-lgmp", "-lm"], check=True)
     return LIB_PATH
 
 

@@ -19,6 +19,7 @@
  * H rows must fit int64 here (the reference's fast path,
  * fp_kernel.h:510-519); otherwise the p-vector gets status 1.
  */
+#include <gmp.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -529,6 +530,80 @@ int or_factor(const int64_t *mat, int dim, const int64_t *linvec,
     return 0;
 }
 
+/* The same factor in exact GMP arithmetic, for when int128 is too narrow
+ * (fp_kernel.h:193-306): fraction-free elimination, U_kk = sqrt(p_k/p_{k-1}),
+ * U_kj = U_kk A_kj / p_k; M0 data from z = R^{-T} linvec; m0Va = |m0V|. */
+static int or_factor_gmp(const int64_t *mat, int dim, const int64_t *linvec,
+                         double *U, double *m0N, double *m0V, double *m0Va)
+{
+    const int n2 = dim * dim;
+    mpz_t *A = malloc(sizeof(mpz_t) * n2);
+    mpz_t prev, t;
+    mpq_t r;
+    for (int k = 0; k < n2; ++k) mpz_init_set_si(A[k], mat[k]);
+    mpz_init_set_ui(prev, 1);
+    mpz_init(t);
+    mpq_init(r);
+    int st = 0;
+    for (int k = 0; k < dim && !st; ++k) {
+        mpz_t *pk = &A[k * dim + k];
+        if (mpz_sgn(*pk) <= 0) { st = -9; break; }
+        mpq_set_num(r, *pk); mpq_set_den(r, prev); mpq_canonicalize(r);
+        double ukk = sqrt(mpq_get_d(r));
+        U[k * dim + k] = ukk;
+        for (int j = 0; j < k; ++j) U[k * dim + j] = 0.0;
+        for (int j = k + 1; j < dim; ++j) {
+            mpq_set_num(r, A[k * dim + j]); mpq_set_den(r, *pk); mpq_canonicalize(r);
+            U[k * dim + j] = ukk * mpq_get_d(r);
+        }
+        for (int i = k + 1; i < dim; ++i)
+            for (int j = i; j < dim; ++j) {
+                mpz_mul(t, *pk, A[i * dim + j]);
+                mpz_submul(t, A[i * dim + k], A[k * dim + j]);
+                mpz_divexact(A[i * dim + j], t, prev);
+                if (j != i) mpz_set(A[j * dim + i], A[i * dim + j]);
+            }
+        mpz_set(prev, *pk);
+    }
+    if (!st && linvec) {
+        mpq_t *z = malloc(sizeof(mpq_t) * dim), *V = malloc(sizeof(mpq_t) * dim);
+        mpq_t R, acc, N;
+        mpq_inits(R, acc, N, NULL);
+        for (int k = 0; k < dim; ++k) { mpq_init(z[k]); mpq_init(V[k]); }
+        for (int k = 0; k < dim; ++k) {
+            mpq_set_si(z[k], linvec[k], 1);
+            for (int m = 0; m < k; ++m) {
+                mpq_set_num(R, A[m * dim + k]); mpq_set_den(R, A[m * dim + m]); mpq_canonicalize(R);
+                mpq_mul(acc, R, z[m]);
+                mpq_sub(z[k], z[k], acc);
+            }
+        }
+        mpz_set_ui(prev, 1);
+        for (int i = 0; i < dim; ++i) {
+            mpq_mul(acc, z[i], z[i]);
+            mpq_set_num(R, prev); mpq_set_den(R, A[i * dim + i]); mpq_canonicalize(R);
+            mpq_mul(acc, acc, R);
+            mpq_add(N, N, acc);
+            m0N[i] = mpq_get_d(N);
+            for (int j = i + 1; j < dim; ++j) {
+                mpq_set_num(R, A[i * dim + j]); mpq_set_den(R, A[i * dim + i]); mpq_canonicalize(R);
+                mpq_mul(acc, R, z[i]);
+                mpq_add(V[j], V[j], acc);
+                m0V[i * dim + j] = mpq_get_d(V[j]);
+                m0Va[i * dim + j] = fabs(m0V[i * dim + j]);
+            }
+            mpz_set(prev, A[i * dim + i]);
+        }
+        for (int k = 0; k < dim; ++k) { mpq_clear(z[k]); mpq_clear(V[k]); }
+        mpq_clears(R, acc, N, NULL);
+        free(z); free(V);
+    }
+    for (int k = 0; k < n2; ++k) mpz_clear(A[k]);
+    free(A);
+    mpz_clear(prev); mpz_clear(t); mpq_clear(r);
+    return st;
+}
+
 /* ------------------------------------------------------------ search */
 typedef struct {
     int dim, strict, nrows, m0_level;
@@ -620,7 +695,11 @@ int or_search(const or_problem *P, or_out *out)
         memcpy(m0Va, P->fac + 2 * d2 + dim, sizeof(double) * d2);
     } else {
         int st = or_factor(P->mat, dim, P->linvec, U, m0N, m0V, m0Va);
-        if (st) return st == -9 ? -9 : 1;
+        if (st == -1) {
+            memset(m0V, 0, sizeof m0V); memset(m0Va, 0, sizeof m0Va);
+            st = or_factor_gmp(P->mat, dim, P->linvec, U, m0N, m0V, m0Va);
+        }
+        if (st) return st;
     }
     for (int i = 0; i < dim; ++i) Uinv[i] = 1.0 / U[i * dim + i];
     int m0_level = -1;

@@ -1,0 +1,1459 @@
+// pfa_lib.hip -- libpfvs_amd.so: the coniZpM search on MI355X (gfx950).
+//
+// Pipeline per call (one HIP stream, inputs uploaded once):
+//   k_prep      one wavefront (64 lanes, LDS working set) per p-vector:
+//               lattice, HNF, FP64 Cholesky with its error margin, record.
+//   k_roots     one lane per p-vector: top-level node -> frontier queue.
+//   search      PLAIN: k_plain, one lane per p-vector runs the whole
+//               depth-first search (the one-launch-per-batch form).
+//               BATCHED: k_batched, persistent wavefronts each holding a
+//               LIFO stack of nodes; every step expands up to 64 sibling
+//               candidates of the top nodes at once (prefix sum over their
+//               ranges), pushes surviving children with a ballot, writes
+//               leaves with one atomic per wavefront; after `budget` steps
+//               a wavefront donates its stack to the global frontier ring,
+//               which the next pass redistributes over all wavefronts.
+//   k_post      one lane per point: M = Binter c, Kn = ZB c (checked),
+//               the K-integrality count; k_det: det((kappa M)[1:,1:]) != 0.
+// The host path runs the same preparation and search code on host threads.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+#include "pfa_int.h"
+#include "pfa_post.h"
+#include "pfa_prep.h"
+#include "pfa_search.h"
+#include "../../include/pfvs_amd.h"
+
+#define PFA_MAX_H11 (PFA_DMAX + 1)
+
+// ------------------------------------------------------------ record
+// Per-problem record in 8-byte words; offsets depend on the batch's max d.
+struct RecL {
+    int D;
+    int U, Uinv, mat, lin, H, L, Bi, ZB, words;
+};
+
+static RecL rec_layout(int D)
+{
+    RecL r;
+    r.D = D;
+    int o = 16;
+    r.U = o; o += D * D;
+    r.Uinv = o; o += D;
+    r.mat = o; o += D * D;
+    r.lin = o; o += D;
+    r.H = o; o += D * D;
+    r.L = o; o += 2 * (D + 1) * D;
+    r.Bi = o; o += (D + 1) * D;
+    r.ZB = o; o += (D + 1) * D;
+    r.words = (o + 1) & ~1;
+    return r;
+}
+
+// header (int32 view): [0] status [1] d [20] m0_level [21] leafg [22] nL
+// [23] strict [24] h [25] code [26] geo; (int64 view) [1] qmax [2] Q
+// [3] linmin [4..8] Qe QGmul phi slack Kerr (double) [9] hmask
+PFA_HD static inline PfaProb rec_view(const int64_t *r, const RecL &L)
+{
+    const int32_t *ri = (const int32_t *)r;
+    const double *rd = (const double *)r;
+    PfaProb P;
+    P.d = ri[1];
+    P.m0_level = ri[20];
+    P.leafg = ri[21];
+    P.nL = ri[22];
+    P.strict = ri[23];
+    P.qmax = r[1];
+    P.Q = r[2];
+    P.linmin = r[3];
+    P.Qe = rd[4];
+    P.QGmul = rd[5];
+    P.phi = rd[6];
+    P.slack = rd[7];
+    P.Kerr = rd[8];
+    P.hmask = (uint64_t)r[9];
+    P.U = (const double *)(r + L.U);
+    P.Uinv = (const double *)(r + L.Uinv);
+    P.mat = r + L.mat;
+    P.linvec = r + L.lin;
+    P.H = r + L.H;
+    P.L = r + L.L;
+    return P;
+}
+
+PFA_HD static inline int32_t &rec_status(int64_t *r) { return ((int32_t *)r)[0]; }
+
+// scalar constants of the search for a problem of dimension d with factor
+// margin phi (slack and Kerr as fpk_search_consts, fpk_search_impl.h:30-39)
+PFA_HD static inline void rec_set_consts(int64_t *r, int d, int64_t qmax, double phi)
+{
+    double *rd = (double *)r;
+    const double u = 0x1p-53;
+    double inv = 1.0 / (1.0 - phi);
+    double Qe = (double)qmax * inv * (1.0 + 8 * u);
+    rd[4] = Qe;
+    rd[5] = inv * (1.0 + 8 * u);
+    rd[6] = phi;
+    rd[7] = fmax(1e-4, 1e-9 * (Qe + 1.0));
+    rd[8] = 2.0 * (double)(d + 8) * u;
+}
+
+struct GeoArgs {
+    const int32_t *h;
+    const int64_t *Q, *qmax, *linmin, *kappa, *Mbasis, *kofs, *mofs;
+};
+
+// --------------------------------------------------- preparation (team)
+PFA_HD static inline void prep_one(const PfaTeam &T, char *wsmem, const GeoArgs &G,
+                                   const int64_t *ps, int MH, const int32_t *pgeo, int64_t ip,
+                                   int64_t *rec, RecL L)
+{
+    const int geo = pgeo[ip];
+    const int h = G.h[geo], d = h - 1;
+    int32_t *ri = (int32_t *)rec;
+    const int64_t *p = ps + ip * MH;
+    if (T.lane == 0) {
+        for (int k = 0; k < 16; ++k) rec[k] = 0;
+        ri[0] = PFA_HOST;
+        ri[1] = d;
+        rec[1] = G.qmax[geo];
+        rec[2] = G.Q[geo];
+        rec[3] = G.linmin[geo];
+        ri[23] = 1;
+        ri[24] = h;
+        ri[26] = geo;
+    }
+    PfaWS W;
+    pfa_ws_carve(W, wsmem, h);
+    int st = pfa_build_lattice(T, W, G.kappa + G.kofs[geo], G.Mbasis + G.mofs[geo], p);
+    if (st) {
+        PFA_SYNC();
+        if (T.lane == 0) ri[25] = st;
+        return;
+    }
+    // H = HNF(ZB[1:]) in the GG region; v, w, pivots in W1..W3
+    pfa_i128 *H = W.GG;
+    int hst = pfa_hnf(T, W, W.ZB + d, d, d, H, (pfa_i128 *)W.W1, (pfa_i128 *)W.W2, (int *)W.W3);
+    double *U = (double *)W.W1, *X = (double *)W.W2, phi = 0.0;
+    PFA_SYNC();
+    int cst = pfa_cholesky(T, W.mat, d, U, X, &phi, W.flag);
+    if (cst) {
+        PFA_SYNC();
+        if (T.lane == 0) ri[25] = cst == PFA_NOT_PD ? -60 : -70;
+        return;
+    }
+    // row levels of H (pivot columns) and whether each row fits int64
+    int *lvl = (int *)W.W3;             // d levels + d fit flags
+    if (T.lane == 0) {
+        for (int r = 0; r < d; ++r) {
+            int lv = -1, fit = 1;
+            if (!hst)
+                for (int j = 0; j < d; ++j) {
+                    pfa_i128 x = H[r * d + j];
+                    if (lv < 0 && x != 0) lv = j;
+                    if (!pfa_fits64(x)) fit = 0;
+                }
+            lvl[r] = lv;
+            lvl[d + r] = fit;
+        }
+    }
+    PFA_SYNC();
+    int64_t *rU = rec + L.U, *rM = rec + L.mat, *rH = rec + L.H, *rL = rec + L.L;
+    for (int e = T.lane; e < d * d; e += T.n) {
+        ((double *)rU)[e] = U[e];
+        rM[e] = W.mat[e];
+        rH[e] = 0;
+    }
+    for (int e = T.lane; e < d; e += T.n) {
+        ((double *)(rec + L.Uinv))[e] = 1.0 / U[e * d + e];
+        rec[L.lin + e] = W.Bi[e];
+    }
+    for (int e = T.lane; e < h * d; e += T.n) {
+        rec[L.Bi + e] = W.Bi[e];
+        rec[L.ZB + e] = W.ZB[e];
+    }
+    PFA_SYNC();
+    for (int e = T.lane; e < d * d; e += T.n) {
+        int r = e / d, j = e % d;
+        if (lvl[r] >= 0 && lvl[d + r]) rH[lvl[r] * d + j] = (int64_t)H[r * d + j];
+    }
+    uint64_t hmask = 0;
+    int leafg = hst != 0;
+    for (int r = 0; r < d; ++r) {
+        if (lvl[r] < 0) continue;
+        if (lvl[d + r]) hmask |= 1ULL << lvl[r];
+        else leafg = 1;
+    }
+    if (leafg) {
+        // exact leaf gcd from Kn[1:] = ZB[1:] c (int64 rows)
+        for (int e = T.lane; e < d * d; e += T.n) {
+            rL[2 * e] = W.ZB[d + e];
+            rL[2 * e + 1] = 0;
+        }
+    }
+    int m0 = -1;
+    for (int j = 0; j < d; ++j)
+        if (W.Bi[j] != 0) { m0 = j; break; }
+    PFA_SYNC();
+    if (T.lane == 0) {
+        rec[9] = (int64_t)hmask;
+        ri[20] = m0;
+        ri[21] = leafg;
+        ri[22] = leafg ? d : 0;
+        ri[25] = hst ? -15 : 0;
+        rec_set_consts(rec, d, rec[1], phi);
+        ri[0] = PFA_OK;
+    }
+    PFA_SYNC();
+}
+
+__global__ __launch_bounds__(64) void k_prep(GeoArgs G, const int64_t *ps, int MH,
+                                             const int32_t *pgeo, int64_t n, int64_t *recs, RecL L)
+{
+    extern __shared__ __align__(16) char smem[];
+    PfaTeam T{(int)threadIdx.x, 64};
+    for (int64_t ip = blockIdx.x; ip < n; ip += gridDim.x) {
+        prep_one(T, smem, G, ps, MH, pgeo, ip, recs + ip * L.words, L);
+        __syncthreads();
+    }
+}
+
+// Factor-only preparation for the single-problem kernel API: the record
+// already holds mat, lin, H, L and the header; computes U, Uinv, phi.
+PFA_HD static inline void factor_one(const PfaTeam &T, char *wsmem, int64_t *rec, RecL L)
+{
+    int32_t *ri = (int32_t *)rec;
+    const int d = ri[1];
+    double *U = (double *)wsmem, *X = U + d * d;
+    int *flag = (int *)(X + d * d);
+    double phi = 0.0;
+    int cst = pfa_cholesky(T, rec + L.mat, d, U, X, &phi, flag);
+    if (cst) {
+        PFA_SYNC();
+        if (T.lane == 0) { ri[0] = PFA_HOST; ri[25] = cst == PFA_NOT_PD ? -60 : -70; }
+        return;
+    }
+    for (int e = T.lane; e < d * d; e += T.n) ((double *)(rec + L.U))[e] = U[e];
+    for (int e = T.lane; e < d; e += T.n) ((double *)(rec + L.Uinv))[e] = 1.0 / U[e * d + e];
+    PFA_SYNC();
+    if (T.lane == 0) {
+        rec_set_consts(rec, d, rec[1], phi);
+        ri[0] = PFA_OK;
+    }
+    PFA_SYNC();
+}
+
+__global__ __launch_bounds__(64) void k_factor(int64_t n, int64_t *recs, RecL L)
+{
+    extern __shared__ __align__(16) char smem[];
+    PfaTeam T{(int)threadIdx.x, 64};
+    for (int64_t ip = blockIdx.x; ip < n; ip += gridDim.x) {
+        factor_one(T, smem, recs + ip * L.words, L);
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------- frontier entries
+// Entry: PfaNode (128 B) | pidx | pad | c[D] (int32)
+struct EntL {
+    int D, bytes;
+};
+static EntL ent_layout(int D) { return EntL{D, ((144 + 4 * D) + 15) & ~15}; }
+
+struct QueueArgs {
+    char *ring;
+    unsigned long long cap;
+    unsigned long long *ctl;      // [0] head [1] tail [2] lim [3] head at pass start [4] overflow
+};
+
+struct PointArgs {
+    int32_t *pidx;
+    int64_t *q;
+    int32_t *c;                   // cap x D
+    unsigned long long cap;
+    unsigned long long *count;
+    int *overflow;
+};
+
+struct CountArgs {
+    unsigned long long *c;        // nodes, cand, leaf
+};
+
+__device__ static inline void entry_copy(char *dst, const char *src, int bytes)
+{
+    const int4 *s = (const int4 *)src;
+    int4 *t = (int4 *)dst;
+    for (int k = 0; k < bytes / 16; ++k) t[k] = s[k];
+}
+
+__global__ void k_roots(int64_t n, int64_t *recs, RecL L, QueueArgs Qa, EntL E, CountArgs C)
+{
+    int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= n) return;
+    int64_t *rec = recs + ip * L.words;
+    if (rec_status(rec) != PFA_OK) return;
+    PfaProb P = rec_view(rec, L);
+    if (pfa_trivially_empty(P)) return;
+    PfaNode nd;
+    pfa_root(P, nd);
+    int32_t c[PFA_DMAX];
+    for (int j = 0; j < PFA_DMAX; ++j) c[j] = 0;
+    if (pfa_enter(P, c, nd)) { rec_status(rec) = PFA_I32; return; }
+    atomicAdd(&C.c[0], 1ULL);
+    unsigned long long slot = atomicAdd(&Qa.ctl[1], 1ULL);
+    char *e = Qa.ring + (slot % Qa.cap) * E.bytes;
+    *(PfaNode *)e = nd;
+    ((int32_t *)(e + 128))[0] = (int32_t)ip;
+    int32_t *ec = (int32_t *)(e + 144);
+    for (int j = 0; j < E.D; ++j) ec[j] = 0;
+}
+
+// snapshot the ring window of the next pass
+__global__ void k_snapshot(QueueArgs Qa)
+{
+    Qa.ctl[2] = Qa.ctl[1];
+    Qa.ctl[3] = Qa.ctl[0];
+}
+
+// ----------------------------------------------------------- plain form
+__global__ void k_plain(int64_t n, int64_t *recs, RecL L, PointArgs Pt, int D, CountArgs C)
+{
+    int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= n) return;
+    int64_t *rec = recs + ip * L.words;
+    if (rec_status(rec) != PFA_OK) return;
+    PfaProb P = rec_view(rec, L);
+    PfaNode st[PFA_DMAX];
+    int32_t c[PFA_DMAX];
+    PfaCounts cnt{0, 0, 0};
+    auto emit = [&](const int32_t *cc, int64_t q) -> int {
+        unsigned long long k = atomicAdd(Pt.count, 1ULL);
+        if (k >= Pt.cap) { *Pt.overflow = 1; return 0; }
+        Pt.pidx[k] = (int32_t)ip;
+        Pt.q[k] = q;
+        for (int j = 0; j < D; ++j) Pt.c[k * D + j] = j < P.d ? cc[j] : 0;
+        return 0;
+    };
+    int r = pfa_dfs(P, st, c, cnt, emit);
+    if (r) rec_status(rec) = r;
+    atomicAdd(&C.c[0], cnt.nodes);
+    atomicAdd(&C.c[1], cnt.cand);
+    atomicAdd(&C.c[2], cnt.leaf);
+}
+
+// --------------------------------------------------------- batched form
+struct BatchArgs {
+    int64_t *recs;
+    RecL L;
+    EntL E;
+    QueueArgs Q;
+    char *stacks;
+    int stack_cap, budget;
+    PointArgs Pt;
+    CountArgs C;
+};
+
+__device__ static inline int wave_scan_incl(int x, int lane)
+{
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// spill entries [0, top) of the stack into the ring; returns 0 or 1 overflow
+__device__ static inline void stack_spill(const BatchArgs &a, char *stk, int top, int lane,
+                                          unsigned long long *s_base)
+{
+    if (lane == 0) *s_base = atomicAdd(&a.Q.ctl[1], (unsigned long long)top);
+    __syncthreads();
+    unsigned long long base = *s_base;
+    if (base + top > a.Q.ctl[3] + a.Q.cap) {
+        if (lane == 0) atomicExch((unsigned long long *)&a.Q.ctl[4], 1ULL);
+    } else {
+        for (int k = lane; k < top; k += 64)
+            entry_copy(a.Q.ring + ((base + k) % a.Q.cap) * a.E.bytes, stk + (size_t)k * a.E.bytes,
+                       a.E.bytes);
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(64) void k_batched(BatchArgs a)
+{
+    const int lane = threadIdx.x;
+    const int ES = a.E.bytes, D = a.E.D;
+    char *stk = a.stacks + (size_t)blockIdx.x * a.stack_cap * ES;
+    __shared__ int s_incl[64];
+    __shared__ unsigned long long s_base;
+    unsigned long long n_nodes = 0, n_cand = 0, n_leaf = 0;
+    const unsigned long long lim = a.Q.ctl[2];
+    int top = 0, steps = 0;
+    for (;;) {
+        if (top == 0) {
+            if (steps >= a.budget) break;
+            if (lane == 0) s_base = atomicAdd(&a.Q.ctl[0], 64ULL);
+            __syncthreads();
+            unsigned long long base = s_base;
+            __syncthreads();
+            long long avail = (long long)lim - (long long)base;
+            int n = avail <= 0 ? 0 : (avail > 64 ? 64 : (int)avail);
+            if (n == 0) break;
+            if (lane < n) entry_copy(stk + (size_t)lane * ES, a.Q.ring + ((base + lane) % a.Q.cap) * ES, ES);
+            top = n;
+            __syncthreads();
+        }
+        steps++;
+        // gather up to 64 candidates from the top nodes (lane l: node top-1-l)
+        const int ntop = top < 64 ? top : 64;
+        int cur = 0, hi = -1;
+        if (lane < ntop) {
+            const PfaNode *nd = (const PfaNode *)(stk + (size_t)(top - 1 - lane) * ES);
+            cur = nd->cur;
+            hi = nd->hi;
+        }
+        long long cl = lane < ntop ? (long long)hi - cur + 1 : 0;
+        int cnt = cl <= 0 ? 0 : (cl > 64 ? 65 : (int)cl);     // 65 = "more than a step"
+        int incl = wave_scan_incl(cnt, lane);
+        s_incl[lane] = incl;
+        __syncthreads();
+        const int total = s_incl[63];
+        const int take = total < 64 ? total : 64;
+        unsigned long long full = __ballot(lane < ntop && incl <= 64);
+        const int F = __popcll(full);
+        // my candidate: slot s = lane -> node j = first lane with incl > s
+        int j = -1;
+        if (lane < take) {
+            int lo = 0, hi2 = ntop - 1;
+            while (lo < hi2) {
+                int mid = (lo + hi2) >> 1;
+                if (s_incl[mid] > lane) hi2 = mid; else lo = mid + 1;
+            }
+            j = lo;
+        }
+        PfaNode nd;
+        int32_t c[PFA_DMAX];
+        int32_t pidx = 0;
+        int32_t v = 0;
+        if (j >= 0) {
+            const char *e = stk + (size_t)(top - 1 - j) * ES;
+            nd = *(const PfaNode *)e;
+            pidx = ((const int32_t *)(e + 128))[0];
+            const int32_t *ec = (const int32_t *)(e + 144);
+            for (int k = 0; k < D; ++k) c[k] = ec[k];
+            int excl = (j > 0 ? s_incl[j - 1] : 0);
+            v = nd.cur + (lane - excl);
+        }
+        // partially consumed node (lane F): its new cur
+        int part_new_cur = 0;
+        if (F < ntop && lane == 0) {
+            const PfaNode *pn = (const PfaNode *)(stk + (size_t)(top - 1 - F) * ES);
+            int exclF = F > 0 ? s_incl[F - 1] : 0;
+            part_new_cur = pn->cur + (64 - exclF);
+        }
+        __syncthreads();                     // every parent read before any write
+        int newtop = top - F;
+        if (F < ntop && lane == 0)
+            ((PfaNode *)(stk + (size_t)(top - 1 - F) * ES))->cur = part_new_cur;
+        // test the candidate; children and leaves
+        int has_child = 0, emit = 0;
+        int64_t q = 0;
+        PfaNode ch;
+        if (j >= 0) {
+            const int64_t *rec = a.recs + (int64_t)pidx * a.L.words;
+            PfaProb P = rec_view(rec, a.L);
+            n_cand++;
+            int r = pfa_cand(P, nd, v, ch);
+            if (r == PFA_CAND_OVF) {
+                rec_status((int64_t *)rec) = PFA_HOST;
+            } else if (r == 1) {
+                c[nd.i] = v;
+                if (nd.i > 0) {
+                    n_nodes++;
+                    if (pfa_enter(P, c, ch)) rec_status((int64_t *)rec) = PFA_I32;
+                    else has_child = ch.hi >= ch.cur;
+                } else {
+                    n_leaf++;
+                    int lr = pfa_leaf(P, ch, c, &q);
+                    if (lr == PFA_CAND_OVF) rec_status((int64_t *)rec) = PFA_HOST;
+                    else emit = lr == 1;
+                }
+            }
+        }
+        // leaves: one atomic per wavefront
+        unsigned long long em = __ballot(emit);
+        if (em) {
+            if (lane == 0) s_base = atomicAdd(a.Pt.count, (unsigned long long)__popcll(em));
+            __syncthreads();
+            if (emit) {
+                unsigned long long k = s_base + __popcll(em & ((1ULL << lane) - 1));
+                if (k < a.Pt.cap) {
+                    a.Pt.pidx[k] = pidx;
+                    a.Pt.q[k] = q;
+                    for (int t = 0; t < D; ++t) a.Pt.c[k * D + t] = c[t];
+                } else {
+                    *a.Pt.overflow = 1;
+                }
+            }
+            __syncthreads();
+        }
+        // children: make room, then push with a ballot
+        unsigned long long cm = __ballot(has_child);
+        const int nch = __popcll(cm);
+        if (newtop + nch > a.stack_cap) {
+            __syncthreads();
+            stack_spill(a, stk, newtop, lane, &s_base);
+            newtop = 0;
+        }
+        if (has_child) {
+            int slot = newtop + __popcll(cm & ((1ULL << lane) - 1));
+            char *e = stk + (size_t)slot * ES;
+            *(PfaNode *)e = ch;
+            ((int32_t *)(e + 128))[0] = pidx;
+            int32_t *ec = (int32_t *)(e + 144);
+            for (int k = 0; k < D; ++k) ec[k] = k < ch.i + 1 ? 0 : c[k];
+        }
+        top = newtop + nch;
+        __syncthreads();                     // writes visible to the next step
+        if (steps >= a.budget && top > 0) {
+            stack_spill(a, stk, top, lane, &s_base);
+            top = 0;
+            break;
+        }
+    }
+    if (n_nodes) atomicAdd(&a.C.c[0], n_nodes);
+    if (n_cand) atomicAdd(&a.C.c[1], n_cand);
+    if (n_leaf) atomicAdd(&a.C.c[2], n_leaf);
+}
+
+// ------------------------------------------------------------ post
+struct PostArgs {
+    int64_t n;
+    const int32_t *pidx;
+    const int64_t *q;
+    const int32_t *c;
+    int D, MH;
+    const int64_t *recs;
+    RecL L;
+    int64_t *M, *Kn;
+    int64_t *nexp;
+    int8_t *bad;                  // per problem: M/Kn overflow
+    int filter;
+};
+
+PFA_HD static inline void post_one(const PostArgs &a, int64_t k)
+{
+    const int32_t ip = a.pidx[k];
+    const int64_t *rec = a.recs + (int64_t)ip * a.L.words;
+    const int32_t *ri = (const int32_t *)rec;
+    const int d = ri[1], h = d + 1;
+    const int32_t *c = a.c + k * a.D;
+    const int64_t *Bi = rec + a.L.Bi, *ZB = rec + a.L.ZB;
+    int64_t *M = a.M + k * a.MH, *Kn = a.Kn + k * a.MH;
+    int bad = 0;
+    for (int i = 0; i < a.MH; ++i) {
+        if (i >= h) { M[i] = 0; Kn[i] = 0; continue; }
+        pfa_i128 sm = 0, sk = 0;
+        for (int t = 0; t < d; ++t) {
+            sm += (pfa_i128)Bi[i * d + t] * c[t];
+            sk += (pfa_i128)ZB[i * d + t] * c[t];
+        }
+        if (!pfa_fits64(sm) || !pfa_fits64(sk)) bad = 1;
+        M[i] = (int64_t)sm;
+        Kn[i] = (int64_t)sk;
+    }
+    if (bad) a.bad[ip] = 1;
+    if (a.filter) a.nexp[k] = bad ? -1 : pfa_num_expansions(M, Kn, h, a.q[k], rec[2]);
+}
+
+__global__ void k_post(PostArgs a)
+{
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n;
+         k += (int64_t)gridDim.x * blockDim.x)
+        post_one(a, k);
+}
+
+struct DetArgs {
+    int64_t n;
+    const int32_t *pidx;
+    const int64_t *M;
+    const int64_t *nexp;
+    int MH;
+    const int64_t *recs;
+    RecL L;
+    GeoArgs G;
+    const uint32_t *primes;
+    int np;
+    int64_t *Nw;                  // per thread (MH-1)^2 int64
+    uint32_t *Ww;                 // per thread (MH-1)^2 uint32
+    int8_t *nonsing;
+};
+
+// N = (kappa M)[1:, 1:] for a point; 0 or 1 if an entry leaves int64
+PFA_HD static inline int kappa_M(const int64_t *kappa, int h, const int64_t *M, int64_t *N)
+{
+    const int n = h - 1;
+    for (int i = 1; i < h; ++i)
+        for (int j = 1; j < h; ++j) {
+            pfa_i128 s = 0;
+            for (int k = 0; k < h; ++k) s += (pfa_i128)kappa[((size_t)i * h + j) * h + k] * M[k];
+            if (!pfa_fits64(s)) return 1;
+            N[(i - 1) * n + (j - 1)] = (int64_t)s;
+        }
+    return 0;
+}
+
+__global__ void k_det(DetArgs a)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nt = (int64_t)gridDim.x * blockDim.x;
+    const int S = (a.MH - 1) * (a.MH - 1);
+    int64_t *N = a.Nw + tid * S;
+    uint32_t *w = a.Ww + tid * S;
+    for (int64_t k = tid; k < a.n; k += nt) {
+        if (a.nexp[k] <= 0) { a.nonsing[k] = 0; continue; }
+        const int64_t *rec = a.recs + (int64_t)a.pidx[k] * a.L.words;
+        const int32_t *ri = (const int32_t *)rec;
+        const int h = ri[24], geo = ri[26];
+        if (kappa_M(a.G.kappa + a.G.kofs[geo], h, a.M + k * a.MH, N)) { a.nonsing[k] = 2; continue; }
+        a.nonsing[k] = (int8_t)pfa_nonsingular(N, h - 1, a.primes, a.np, w);
+    }
+}
+
+// ============================================================ host side
+#define HIPCHK(x)                                                                      \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess) {                                                        \
+            snprintf(errbuf, 256, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, \
+                     __LINE__);                                                        \
+            throw 1;                                                                   \
+        }                                                                              \
+    } while (0)
+
+// grow-only device buffers kept per device until pfg_release
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+struct DevCtx {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    std::vector<DevBuf> bufs;
+    int cus = 0;
+    size_t lds_max = 0;
+};
+
+static std::mutex g_ctx_mu;
+static std::vector<DevCtx *> g_ctx;
+
+static DevCtx *ctx_for(int device)
+{
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    if ((int)g_ctx.size() <= device) g_ctx.resize(device + 1, nullptr);
+    if (!g_ctx[device]) g_ctx[device] = new DevCtx();
+    return g_ctx[device];
+}
+
+static void *dev_get(DevCtx *C, int slot, size_t bytes, char *errbuf)
+{
+    if ((int)C->bufs.size() <= slot) C->bufs.resize(slot + 1);
+    DevBuf &b = C->bufs[slot];
+    if (b.n < bytes) {
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.n = 0;
+        size_t want = bytes + bytes / 4 + 4096;
+        HIPCHK(hipMalloc(&b.p, want));
+        b.n = want;
+    }
+    return b.p;
+}
+
+enum {
+    B_GEO_H, B_GEO_Q, B_GEO_QMAX, B_GEO_LIN, B_KAPPA, B_MBASIS, B_KOFS, B_MOFS, B_PS, B_PGEO,
+    B_RECS, B_RING, B_STACKS, B_CTL, B_PT_PIDX, B_PT_Q, B_PT_C, B_CNT, B_M, B_KN, B_NEXP, B_BAD,
+    B_PRIMES, B_NW, B_WW, B_NONSING, B_NBUF
+};
+
+static std::vector<uint32_t> primes31(int k)
+{
+    std::vector<uint32_t> out;
+    for (uint32_t x = 2147483647u; (int)out.size() < k; x -= 2) {
+        bool pr = true;
+        for (uint32_t f = 3; (uint64_t)f * f <= x; f += 2)
+            if (x % f == 0) { pr = false; break; }
+        if (pr) out.push_back(x);
+    }
+    return out;
+}
+
+static double ms_between(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return ms;
+}
+
+struct HostPts {
+    std::vector<int32_t> pidx, c;
+    std::vector<int64_t> q;
+};
+
+static int batch_dmax(const pfa_run_input *in)
+{
+    int D = 1;
+    for (int g = 0; g < in->n_geo; ++g) D = std::max(D, (int)in->h[g] - 1);
+    return D;
+}
+
+// canonical order: by p-vector, then (M, Kn) lexicographically
+static void canonical_order(std::vector<int64_t> &idx, const int64_t *pidx, const int64_t *M,
+                            const int64_t *Kn, int MH)
+{
+    std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) {
+        if (pidx[a] != pidx[b]) return pidx[a] < pidx[b];
+        const int64_t *ma = M + a * MH, *mb = M + b * MH;
+        for (int i = 0; i < MH; ++i) if (ma[i] != mb[i]) return ma[i] < mb[i];
+        const int64_t *ka = Kn + a * MH, *kb = Kn + b * MH;
+        for (int i = 0; i < MH; ++i) if (ka[i] != kb[i]) return ka[i] < kb[i];
+        return false;
+    });
+}
+
+static void fill_geo_offsets(const pfa_run_input *in, std::vector<int64_t> &kofs,
+                             std::vector<int64_t> &mofs, int64_t &ktot, int64_t &mtot)
+{
+    kofs.resize(in->n_geo);
+    mofs.resize(in->n_geo);
+    ktot = mtot = 0;
+    for (int g = 0; g < in->n_geo; ++g) {
+        int64_t h = in->h[g];
+        kofs[g] = ktot;
+        mofs[g] = mtot;
+        ktot += h * h * h;
+        mtot += h * h;
+    }
+}
+
+// Final assembly shared by the device and host runs: statuses, sort, copy.
+static int finish_run(const pfa_run_input *in, pfa_run_output *out, int64_t npts, int HS,
+                      const int32_t *pidx32, const int64_t *q, const int64_t *M, const int64_t *Kn,
+                      const int64_t *nexp, const int8_t *nonsing, const int32_t *status,
+                      const int32_t *code, const int8_t *bad, char *errbuf)
+{
+    const int MH = PFA_MAX_H11;
+    const int64_t n_p = in->n_p;
+    out->pstat = (int8_t *)malloc(std::max<int64_t>(n_p, 1));
+    out->pcode = (int32_t *)malloc(sizeof(int32_t) * std::max<int64_t>(n_p, 1));
+    for (int64_t i = 0; i < n_p; ++i) {
+        int s = status[i];
+        if (s == PFA_OK && bad[i]) s = PFA_HOST;
+        out->pstat[i] = (int8_t)(s == PFA_OK ? 0 : (s == PFA_HOST ? 1 : s));
+        out->pcode[i] = code[i];
+    }
+    std::vector<int64_t> idx;
+    idx.reserve(npts);
+    for (int64_t k = 0; k < npts; ++k)
+        if (out->pstat[pidx32[k]] == 0) idx.push_back(k);
+    std::vector<int64_t> pidx64(npts);
+    for (int64_t k = 0; k < npts; ++k) pidx64[k] = pidx32[k];
+    if (in->sort) canonical_order(idx, pidx64.data(), M, Kn, HS);
+    const int64_t n = (int64_t)idx.size();
+    out->n = n;
+    size_t nb = std::max<int64_t>(n, 1);
+    out->M = (int64_t *)malloc(sizeof(int64_t) * nb * MH);
+    out->Kn = (int64_t *)malloc(sizeof(int64_t) * nb * MH);
+    out->q = (int64_t *)malloc(sizeof(int64_t) * nb);
+    out->pidx = (int64_t *)malloc(sizeof(int64_t) * nb);
+    out->nexp = (int64_t *)malloc(sizeof(int64_t) * nb);
+    out->nonsing = (int8_t *)malloc(nb);
+    int64_t kept = 0, nex = 0, nns = 0, und = 0, nhost = 0;
+    for (int64_t t = 0; t < n; ++t) {
+        int64_t k = idx[t];
+        for (int i = 0; i < MH; ++i) {
+            out->M[t * MH + i] = i < HS ? M[k * HS + i] : 0;
+            out->Kn[t * MH + i] = i < HS ? Kn[k * HS + i] : 0;
+        }
+        out->q[t] = q[k];
+        out->pidx[t] = pidx64[k];
+        out->nexp[t] = nexp ? nexp[k] : -1;
+        out->nonsing[t] = nonsing ? nonsing[k] : 2;
+        if (nexp && nexp[k] > 0) {
+            kept++;
+            if (nonsing && nonsing[k] == 1) { nex += nexp[k]; nns++; }
+            if (nonsing && nonsing[k] == 2) und++;
+        }
+    }
+    for (int64_t i = 0; i < n_p; ++i) nhost += out->pstat[i] == 1;
+    out->stats[PFA_ST_POINTS] = (double)n;
+    out->stats[PFA_ST_KEPT] = (double)kept;
+    out->stats[PFA_ST_EXPANSIONS] = (double)nex;
+    out->stats[PFA_ST_NONSING] = (double)nns;
+    out->stats[PFA_ST_UNDECIDED] = (double)und;
+    out->stats[PFA_ST_HOST_P] = (double)nhost;
+    (void)errbuf;
+    return 0;
+}
+
+// --------------------------------------------------------- device run
+static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
+{
+    auto t_wall = std::chrono::steady_clock::now();
+    DevCtx *C = ctx_for(in->device);
+    std::lock_guard<std::mutex> lk(C->mu);
+    HIPCHK(hipSetDevice(in->device));
+    if (!C->stream) {
+        HIPCHK(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, in->device));
+        C->cus = prop.multiProcessorCount;
+        C->lds_max = prop.sharedMemPerBlock;
+    }
+    hipStream_t s = C->stream;
+    const int MH = PFA_MAX_H11, D = batch_dmax(in);
+    const int64_t n_p = in->n_p;
+    const RecL L = rec_layout(D);
+    const EntL E = ent_layout(D);
+    std::vector<int64_t> kofs, mofs;
+    int64_t ktot, mtot;
+    fill_geo_offsets(in, kofs, mofs, ktot, mtot);
+    int hmax = D + 1;
+    size_t lds = pfa_ws_bytes(hmax);
+    if (lds > C->lds_max) {
+        snprintf(errbuf, 256, "h11 = %d needs %zu B of LDS (device has %zu)", hmax, lds, C->lds_max);
+        return 1;
+    }
+    auto up = [&](int slot, const void *src, size_t bytes) -> void * {
+        void *d = dev_get(C, slot, std::max<size_t>(bytes, 16), errbuf);
+        if (bytes) HIPCHK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, s));
+        return d;
+    };
+    GeoArgs G;
+    G.h = (const int32_t *)up(B_GEO_H, in->h, 4 * in->n_geo);
+    G.Q = (const int64_t *)up(B_GEO_Q, in->Q, 8 * in->n_geo);
+    G.qmax = (const int64_t *)up(B_GEO_QMAX, in->qmax, 8 * in->n_geo);
+    G.linmin = (const int64_t *)up(B_GEO_LIN, in->linmin, 8 * in->n_geo);
+    G.kappa = (const int64_t *)up(B_KAPPA, in->kappa, 8 * ktot);
+    G.Mbasis = (const int64_t *)up(B_MBASIS, in->Mbasis, 8 * mtot);
+    G.kofs = (const int64_t *)up(B_KOFS, kofs.data(), 8 * in->n_geo);
+    G.mofs = (const int64_t *)up(B_MOFS, mofs.data(), 8 * in->n_geo);
+    const int64_t *d_ps = (const int64_t *)up(B_PS, in->ps, 8 * n_p * MH);
+    const int32_t *d_pgeo = (const int32_t *)up(B_PGEO, in->pgeo, 4 * n_p);
+    int64_t *d_recs = (int64_t *)dev_get(C, B_RECS, 8 * (size_t)L.words * std::max<int64_t>(n_p, 1), errbuf);
+    unsigned long long *d_cnt = (unsigned long long *)dev_get(C, B_CNT, 64, errbuf);
+
+    hipEvent_t ev[8];
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipEventRecord(ev[0], s));
+    if (n_p > 0) {
+        int grid = (int)std::min<int64_t>(n_p, 1 << 20);
+        HIPCHK(hipFuncSetAttribute((const void *)k_prep, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+        hipLaunchKernelGGL(k_prep, dim3(grid), dim3(64), lds, s, G, d_ps, MH, d_pgeo, n_p, d_recs, L);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(ev[1], s));
+
+    // points buffer: grow and rerun the search on overflow
+    unsigned long long pcap = std::max<int64_t>(1 << 16, 16 * n_p);
+    const int waves_per_cu = in->waves_per_cu > 0 ? in->waves_per_cu : 8;
+    const int budget = in->budget > 0 ? in->budget : 4096;
+    const int stack_cap = in->stack_cap > 0 ? in->stack_cap : 64 * (D + 2);
+    const int nwaves = std::max(1, C->cus * waves_per_cu);
+    unsigned long long ring_cap = (unsigned long long)std::max<int64_t>(2 * n_p + 64,
+                                  (int64_t)nwaves * stack_cap * 2);
+    double ms_roots = 0, ms_search = 0;
+    int passes = 0, launches = 0;
+    unsigned long long npts = 0, queue_peak = 0;
+    int32_t *d_pidx = nullptr;
+    int64_t *d_q = nullptr;
+    int32_t *d_c = nullptr;
+    int *d_ovf = nullptr;
+    std::vector<int64_t> hdr0;
+    for (int attempt = 0;; ++attempt) {
+        d_pidx = (int32_t *)dev_get(C, B_PT_PIDX, 4 * pcap, errbuf);
+        d_q = (int64_t *)dev_get(C, B_PT_Q, 8 * pcap, errbuf);
+        d_c = (int32_t *)dev_get(C, B_PT_C, 4 * pcap * D, errbuf);
+        unsigned long long *d_ctl = (unsigned long long *)dev_get(C, B_CTL, 128, errbuf);
+        d_ovf = (int *)(d_ctl + 8);
+        HIPCHK(hipMemsetAsync(d_ctl, 0, 128, s));
+        HIPCHK(hipMemsetAsync(d_cnt, 0, 64, s));
+        if (n_p > 0) {
+            // the search may change statuses: keep (first attempt) or restore
+            // (later attempts) the headers as the preparation left them
+            if (attempt == 0) {
+                hdr0.resize((size_t)n_p * 16);
+                HIPCHK(hipMemcpy2DAsync(hdr0.data(), 16 * 8, d_recs, (size_t)L.words * 8, 16 * 8,
+                                        n_p, hipMemcpyDeviceToHost, s));
+            } else {
+                HIPCHK(hipMemcpy2DAsync(d_recs, (size_t)L.words * 8, hdr0.data(), 16 * 8, 16 * 8,
+                                        n_p, hipMemcpyHostToDevice, s));
+            }
+        }
+        PointArgs Pt{d_pidx, d_q, d_c, pcap, d_ctl + 6, d_ovf};
+        CountArgs Cn{d_cnt};
+        passes = 0;
+        launches = 0;
+        if (in->mode == PFA_MODE_PLAIN) {
+            HIPCHK(hipEventRecord(ev[2], s));
+            HIPCHK(hipEventRecord(ev[3], s));
+            if (n_p > 0) {
+                hipLaunchKernelGGL(k_plain, dim3((n_p + 63) / 64), dim3(64), 0, s, n_p, d_recs, L, Pt, D, Cn);
+                HIPCHK(hipGetLastError());
+                launches = 1;
+            }
+            HIPCHK(hipEventRecord(ev[4], s));
+            passes = 1;
+        } else {
+            char *ring = (char *)dev_get(C, B_RING, (size_t)ring_cap * E.bytes, errbuf);
+            char *stacks = (char *)dev_get(C, B_STACKS, (size_t)nwaves * stack_cap * E.bytes, errbuf);
+            QueueArgs Qa{ring, ring_cap, d_ctl};
+            HIPCHK(hipEventRecord(ev[2], s));
+            if (n_p > 0) {
+                hipLaunchKernelGGL(k_roots, dim3((n_p + 255) / 256), dim3(256), 0, s, n_p, d_recs, L, Qa, E, Cn);
+                HIPCHK(hipGetLastError());
+            }
+            HIPCHK(hipEventRecord(ev[3], s));
+            BatchArgs A{d_recs, L, E, Qa, stacks, stack_cap, budget, Pt, Cn};
+            unsigned long long hctl[8];
+            for (;;) {
+                hipLaunchKernelGGL(k_snapshot, dim3(1), dim3(1), 0, s, Qa);
+                hipLaunchKernelGGL(k_batched, dim3(nwaves), dim3(64), 0, s, A);
+                HIPCHK(hipGetLastError());
+                launches++;
+                passes++;
+                HIPCHK(hipMemcpyAsync(hctl, d_ctl, 64, hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                unsigned long long head = std::min(hctl[0], hctl[2]);
+                unsigned long long live = hctl[1] - head;
+                queue_peak = std::max(queue_peak, live);
+                if (hctl[4]) {
+                    snprintf(errbuf, 256, "frontier queue overflow (capacity %llu)", ring_cap);
+                    return 1;
+                }
+                if (live == 0) break;
+                // consumed head may overshoot lim by a grab: clamp it
+                if (hctl[0] > hctl[2]) {
+                    HIPCHK(hipMemcpyAsync(d_ctl, &hctl[2], 8, hipMemcpyHostToDevice, s));
+                }
+                if (passes > 100000) {
+                    snprintf(errbuf, 256, "batched search did not finish");
+                    return 1;
+                }
+            }
+            HIPCHK(hipEventRecord(ev[4], s));
+        }
+        HIPCHK(hipMemcpyAsync(&npts, d_ctl + 6, 8, hipMemcpyDeviceToHost, s));
+        int ovf = 0;
+        HIPCHK(hipMemcpyAsync(&ovf, d_ovf, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        ms_roots = ms_between(ev[2], ev[3]);
+        ms_search = ms_between(ev[3], ev[4]);
+        if (!ovf && npts <= pcap) break;
+        pcap = std::max<unsigned long long>(2 * pcap, npts + npts / 4 + 1024);
+        if (attempt > 6) {
+            snprintf(errbuf, 256, "point buffer overflow");
+            return 1;
+        }
+    }
+    // post: M, Kn, filter
+    const int64_t n = (int64_t)npts;
+    const int HS = D + 1;                    // row stride of M, Kn on the device
+    int64_t *d_M = (int64_t *)dev_get(C, B_M, 8 * std::max<int64_t>(n, 1) * HS, errbuf);
+    int64_t *d_Kn = (int64_t *)dev_get(C, B_KN, 8 * std::max<int64_t>(n, 1) * HS, errbuf);
+    int64_t *d_nexp = (int64_t *)dev_get(C, B_NEXP, 8 * std::max<int64_t>(n, 1), errbuf);
+    int8_t *d_bad = (int8_t *)dev_get(C, B_BAD, std::max<int64_t>(n_p, 1), errbuf);
+    int8_t *d_ns = (int8_t *)dev_get(C, B_NONSING, std::max<int64_t>(n, 1), errbuf);
+    HIPCHK(hipMemsetAsync(d_bad, 0, std::max<int64_t>(n_p, 1), s));
+    HIPCHK(hipEventRecord(ev[5], s));
+    if (n > 0) {
+        PostArgs Pa{n, d_pidx, d_q, d_c, D, HS, d_recs, L, d_M, d_Kn, d_nexp, d_bad, in->filter};
+        int grid = (int)std::min<int64_t>((n + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_post, dim3(grid), dim3(256), 0, s, Pa);
+        HIPCHK(hipGetLastError());
+        if (in->filter) {
+            static std::vector<uint32_t> pr = primes31(96);
+            uint32_t *d_pr = (uint32_t *)up(B_PRIMES, pr.data(), 4 * pr.size());
+            const int dgrid = (int)std::min<int64_t>((n + 255) / 256, 256);
+            const int64_t nthr = (int64_t)dgrid * 256;
+            const int64_t S = (int64_t)(HS - 1) * (HS - 1);
+            int64_t *d_Nw = (int64_t *)dev_get(C, B_NW, 8 * nthr * S, errbuf);
+            uint32_t *d_Ww = (uint32_t *)dev_get(C, B_WW, 4 * nthr * S, errbuf);
+            DetArgs Da{n, d_pidx, d_M, d_nexp, HS, d_recs, L, G, d_pr, (int)pr.size(), d_Nw, d_Ww, d_ns};
+            hipLaunchKernelGGL(k_det, dim3(dgrid), dim3(256), 0, s, Da);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    HIPCHK(hipEventRecord(ev[6], s));
+    // results back
+    const int64_t nh = in->keep_points ? n : 0;     // points copied back
+    std::vector<int32_t> h_pidx(nh), h_status(n_p), h_code(n_p);
+    std::vector<int64_t> h_q(nh), h_M(nh * HS), h_Kn(nh * HS), h_nexp(in->filter ? nh : 0);
+    std::vector<int8_t> h_ns(in->filter ? nh : 0), h_bad(n_p);
+    std::vector<int64_t> hdr((size_t)n_p * 16);
+    if (in->filter && n > 0 && !in->keep_points) {
+        // counts only: reduce the filter results on the device side copy
+        std::vector<int64_t> tmp(n);
+        std::vector<int8_t> tns(n);
+        HIPCHK(hipMemcpyAsync(tmp.data(), d_nexp, 8 * n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(tns.data(), d_ns, n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        int64_t kept = 0, nex = 0, nns = 0, und = 0;
+        for (int64_t k = 0; k < n; ++k)
+            if (tmp[k] > 0) {
+                kept++;
+                if (tns[k] == 1) { nex += tmp[k]; nns++; }
+                if (tns[k] == 2) und++;
+            }
+        out->stats[PFA_ST_KEPT] = (double)kept;
+        out->stats[PFA_ST_EXPANSIONS] = (double)nex;
+        out->stats[PFA_ST_NONSING] = (double)nns;
+        out->stats[PFA_ST_UNDECIDED] = (double)und;
+    }
+    if (nh > 0) {
+        HIPCHK(hipMemcpyAsync(h_pidx.data(), d_pidx, 4 * n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h_q.data(), d_q, 8 * n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h_M.data(), d_M, 8 * n * HS, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h_Kn.data(), d_Kn, 8 * n * HS, hipMemcpyDeviceToHost, s));
+        if (in->filter) {
+            HIPCHK(hipMemcpyAsync(h_nexp.data(), d_nexp, 8 * n, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(h_ns.data(), d_ns, n, hipMemcpyDeviceToHost, s));
+        }
+    }
+    if (n_p > 0) {
+        HIPCHK(hipMemcpy2DAsync(hdr.data(), 16 * 8, d_recs, (size_t)L.words * 8, 16 * 8, n_p,
+                                hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h_bad.data(), d_bad, n_p, hipMemcpyDeviceToHost, s));
+    }
+    unsigned long long hc[8] = {0};
+    HIPCHK(hipMemcpyAsync(hc, d_cnt, 64, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int64_t ip = 0; ip < n_p; ++ip) {
+        const int32_t *ri = (const int32_t *)&hdr[ip * 16];
+        h_status[ip] = ri[0];
+        h_code[ip] = ri[25];
+    }
+    out->stats[PFA_ST_NODES] = (double)hc[0];
+    out->stats[PFA_ST_CAND] = (double)hc[1];
+    out->stats[PFA_ST_LEAF] = (double)hc[2];
+    out->stats[PFA_ST_PASSES] = passes;
+    out->stats[PFA_ST_SEARCH_LAUNCHES] = launches;
+    out->stats[PFA_ST_MS_PREP] = ms_between(ev[0], ev[1]);
+    out->stats[PFA_ST_MS_ROOTS] = ms_roots;
+    out->stats[PFA_ST_MS_SEARCH] = ms_search;
+    out->stats[PFA_ST_MS_POST] = ms_between(ev[5], ev[6]);
+    out->stats[PFA_ST_MS_TOTAL] = out->stats[PFA_ST_MS_PREP] + ms_roots + ms_search +
+                                  out->stats[PFA_ST_MS_POST];
+    out->stats[PFA_ST_QUEUE_PEAK] = (double)queue_peak;
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    double st_keep[4] = {out->stats[PFA_ST_KEPT], out->stats[PFA_ST_EXPANSIONS],
+                         out->stats[PFA_ST_NONSING], out->stats[PFA_ST_UNDECIDED]};
+    int rc = finish_run(in, out, nh, HS, h_pidx.data(), h_q.data(), h_M.data(), h_Kn.data(),
+                        in->filter ? h_nexp.data() : nullptr, in->filter ? h_ns.data() : nullptr,
+                        h_status.data(), h_code.data(), h_bad.data(), errbuf);
+    if (!in->keep_points) {
+        out->stats[PFA_ST_POINTS] = (double)n;
+        out->stats[PFA_ST_KEPT] = st_keep[0];
+        out->stats[PFA_ST_EXPANSIONS] = st_keep[1];
+        out->stats[PFA_ST_NONSING] = st_keep[2];
+        out->stats[PFA_ST_UNDECIDED] = st_keep[3];
+    }
+    out->stats[PFA_ST_MS_WALL] =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
+    return rc;
+}
+
+// ------------------------------------------------------------ host run
+static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    const int MH = PFA_MAX_H11, D = batch_dmax(in);
+    const int64_t n_p = in->n_p;
+    const RecL L = rec_layout(D);
+    std::vector<int64_t> kofs, mofs;
+    int64_t ktot, mtot;
+    fill_geo_offsets(in, kofs, mofs, ktot, mtot);
+    GeoArgs G{in->h, in->Q, in->qmax, in->linmin, in->kappa, in->Mbasis, kofs.data(), mofs.data()};
+    std::vector<int64_t> recs((size_t)L.words * std::max<int64_t>(n_p, 1));
+    int nth = in->host_threads > 0 ? in->host_threads : (int)std::thread::hardware_concurrency();
+    nth = std::max(1, std::min<int>(nth, (int)std::max<int64_t>(n_p, 1)));
+    std::vector<HostPts> tp(nth);
+    std::vector<PfaCounts> tc(nth, PfaCounts{0, 0, 0});
+    std::atomic<int64_t> next{0};
+    auto worker = [&](int t) {
+        std::vector<char> ws(pfa_ws_bytes(D + 1) + 64);
+        std::vector<PfaNode> st(PFA_DMAX);
+        int32_t c[PFA_DMAX];
+        PfaTeam T{0, 1};
+        for (;;) {
+            int64_t ip = next.fetch_add(1);
+            if (ip >= n_p) break;
+            int64_t *rec = recs.data() + ip * L.words;
+            prep_one(T, ws.data(), G, in->ps, MH, in->pgeo, ip, rec, L);
+            if (rec_status(rec) != PFA_OK) continue;
+            PfaProb P = rec_view(rec, L);
+            HostPts &hp = tp[t];
+            size_t mark = hp.q.size();
+            auto emit = [&](const int32_t *cc, int64_t q) -> int {
+                hp.pidx.push_back((int32_t)ip);
+                hp.q.push_back(q);
+                for (int j = 0; j < D; ++j) hp.c.push_back(j < P.d ? cc[j] : 0);
+                return 0;
+            };
+            int r = pfa_dfs(P, st.data(), c, tc[t], emit);
+            if (r) {
+                rec_status(rec) = r;
+                hp.pidx.resize(mark);
+                hp.q.resize(mark);
+                hp.c.resize(mark * D);
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; ++t) th.emplace_back(worker, t);
+    for (auto &x : th) x.join();
+    auto t1 = std::chrono::steady_clock::now();
+    // gather
+    HostPts all;
+    for (auto &hp : tp) {
+        all.pidx.insert(all.pidx.end(), hp.pidx.begin(), hp.pidx.end());
+        all.q.insert(all.q.end(), hp.q.begin(), hp.q.end());
+        all.c.insert(all.c.end(), hp.c.begin(), hp.c.end());
+    }
+    const int64_t n = (int64_t)all.q.size();
+    const int HS = D + 1;
+    std::vector<int64_t> M((size_t)n * HS), Kn((size_t)n * HS), nexp(n);
+    std::vector<int8_t> ns(n, 2), bad(std::max<int64_t>(n_p, 1), 0);
+    PostArgs Pa{n, all.pidx.data(), all.q.data(), all.c.data(), D, HS, recs.data(), L, M.data(),
+                Kn.data(), nexp.data(), bad.data(), in->filter};
+    for (int64_t k = 0; k < n; ++k) post_one(Pa, k);
+    if (in->filter) {
+        static std::vector<uint32_t> pr = primes31(96);
+        std::vector<int64_t> N((HS - 1) * (HS - 1));
+        std::vector<uint32_t> w((HS - 1) * (HS - 1));
+        for (int64_t k = 0; k < n; ++k) {
+            if (nexp[k] <= 0) { ns[k] = 0; continue; }
+            const int64_t *rec = recs.data() + (int64_t)all.pidx[k] * L.words;
+            const int32_t *ri = (const int32_t *)rec;
+            const int h = ri[24], geo = ri[26];
+            if (kappa_M(in->kappa + kofs[geo], h, M.data() + k * HS, N.data())) { ns[k] = 2; continue; }
+            ns[k] = (int8_t)pfa_nonsingular(N.data(), h - 1, pr.data(), (int)pr.size(), w.data());
+        }
+    }
+    std::vector<int32_t> status(n_p), code(n_p);
+    for (int64_t ip = 0; ip < n_p; ++ip) {
+        const int32_t *ri = (const int32_t *)(recs.data() + ip * L.words);
+        status[ip] = ri[0];
+        code[ip] = ri[25];
+    }
+    PfaCounts tot{0, 0, 0};
+    for (auto &x : tc) { tot.nodes += x.nodes; tot.cand += x.cand; tot.leaf += x.leaf; }
+    out->stats[PFA_ST_NODES] = (double)tot.nodes;
+    out->stats[PFA_ST_CAND] = (double)tot.cand;
+    out->stats[PFA_ST_LEAF] = (double)tot.leaf;
+    out->stats[PFA_ST_PASSES] = 1;
+    out->stats[PFA_ST_MS_SEARCH] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    int rc = finish_run(in, out, n, HS, all.pidx.data(), all.q.data(), M.data(), Kn.data(),
+                        in->filter ? nexp.data() : nullptr, in->filter ? ns.data() : nullptr,
+                        status.data(), code.data(), bad.data(), errbuf);
+    out->stats[PFA_ST_MS_WALL] =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    out->stats[PFA_ST_MS_TOTAL] = out->stats[PFA_ST_MS_WALL];
+    return rc;
+}
+
+static int validate_run(const pfa_run_input *in, char *errbuf)
+{
+    if (in->n_geo < 0 || in->n_p < 0) { snprintf(errbuf, 256, "negative sizes"); return 1; }
+    for (int g = 0; g < in->n_geo; ++g)
+        if (in->h[g] < 3 || in->h[g] > PFA_MAX_H11) {
+            snprintf(errbuf, 256, "h11 = %d outside 3..%d", in->h[g], PFA_MAX_H11);
+            return 1;
+        }
+    for (int64_t i = 0; i < in->n_p; ++i)
+        if (in->pgeo[i] < 0 || in->pgeo[i] >= in->n_geo) {
+            snprintf(errbuf, 256, "pgeo: geometry index out of range");
+            return 1;
+        }
+    return 0;
+}
+
+extern "C" int pfa_coni_run(const pfa_run_input *in, pfa_run_output *out)
+{
+    memset(out, 0, sizeof(*out));
+    char *errbuf = out->err;
+    if (validate_run(in, errbuf)) return 1;
+    try {
+        if (in->mode == PFA_MODE_HOST) return run_host(in, out, errbuf);
+        return run_device(in, out, errbuf);
+    } catch (int) {
+        return 1;
+    }
+}
+
+extern "C" void pfa_run_output_free(pfa_run_output *out)
+{
+    free(out->M); free(out->Kn); free(out->q); free(out->pidx);
+    free(out->pstat); free(out->pcode); free(out->nexp); free(out->nonsing);
+    out->M = out->Kn = out->q = out->pidx = out->nexp = nullptr;
+    out->pstat = nullptr; out->nonsing = nullptr; out->pcode = nullptr;
+    out->n = 0;
+}
+
+// ----------------------------------------------- reference-compatible ABI
+extern "C" int pfg_coni_batch(const pfg_input *in, pfg_output *out)
+{
+    memset(out, 0, sizeof(*out));
+    pfa_run_input r;
+    memset(&r, 0, sizeof r);
+    r.n_geo = in->n_geo; r.h = in->h; r.Q = in->Q; r.qmax = in->qmax; r.linmin = in->linmin;
+    r.kappa = in->kappa; r.Mbasis = in->Mbasis; r.n_p = in->n_p; r.ps = in->ps; r.pgeo = in->pgeo;
+    r.device = in->device; r.mode = PFA_MODE_BATCHED; r.filter = 0; r.sort = 1; r.keep_points = 1;
+    r.verbose = in->verbose;
+    pfa_run_output o;
+    int rc = pfa_coni_run(&r, &o);
+    if (rc) {
+        memcpy(out->err, o.err, 256);
+        pfa_run_output_free(&o);
+        return rc;
+    }
+    out->n = o.n;
+    out->M = o.M; out->Kn = o.Kn; out->q = o.q; out->pidx = o.pidx;
+    // pstat 0/1 only: a kernel status (< 0) is redone by the host path too
+    out->pstat = o.pstat;
+    for (int64_t i = 0; i < in->n_p; ++i) if (out->pstat[i] != 0) out->pstat[i] = 1;
+    out->seconds_gpu = o.stats[PFA_ST_MS_TOTAL] * 1e-3;
+    free(o.pcode); free(o.nexp); free(o.nonsing);
+    return 0;
+}
+
+extern "C" void pfg_output_free(pfg_output *out)
+{
+    free(out->M); free(out->Kn); free(out->q); free(out->pidx); free(out->pstat);
+    out->M = out->Kn = out->q = out->pidx = nullptr;
+    out->pstat = nullptr;
+    out->n = 0;
+}
+
+extern "C" int pfg_max_h11(void) { return PFA_MAX_H11; }
+extern "C" const char *pfg_backend(void) { return "hip"; }
+
+extern "C" int pfg_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" void pfg_release(int device)
+{
+    if (device < 0) return;
+    DevCtx *C;
+    {
+        std::lock_guard<std::mutex> lk(g_ctx_mu);
+        if (device >= (int)g_ctx.size() || !g_ctx[device]) return;
+        C = g_ctx[device];
+    }
+    std::lock_guard<std::mutex> lk(C->mu);
+    if (hipSetDevice(device) != hipSuccess) return;
+    for (auto &b : C->bufs) {
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.n = 0;
+    }
+}
+
+extern "C" int pfa_device_cus(int device)
+{
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
+    return prop.multiProcessorCount;
+}
+
+extern "C" const char *pfa_arch(void) { return "gfx950"; }
+
+// ------------------------------------------------- single-problem batch
+extern "C" int pfa_kernel_batch(const pfa_kernel_input *in, pfa_kernel_output *out)
+{
+    memset(out, 0, sizeof(*out));
+    char *errbuf = out->err;
+    try {
+        int D = 1;
+        for (int k = 0; k < in->n_prob; ++k) {
+            if (in->dims[k] < 1 || in->dims[k] > PFA_DMAX) {
+                snprintf(errbuf, 256, "problem dimension outside 1..%d", PFA_DMAX);
+                return 1;
+            }
+            if (in->nl[k] > in->dims[k] + 1) {
+                snprintf(errbuf, 256, "too many leaf rows");
+                return 1;
+            }
+            D = std::max(D, (int)in->dims[k]);
+        }
+        const RecL L = rec_layout(D);
+        const int64_t n = in->n_prob;
+        std::vector<int64_t> recs((size_t)L.words * std::max<int64_t>(n, 1), 0);
+        int64_t om = 0, oh = 0, ol = 0, olin = 0;
+        for (int64_t k = 0; k < n; ++k) {
+            int64_t *r = recs.data() + k * L.words;
+            int32_t *ri = (int32_t *)r;
+            const int d = in->dims[k];
+            ri[0] = PFA_HOST;
+            ri[1] = d;
+            r[1] = in->qmax[k];
+            r[2] = in->Q[k];
+            r[3] = in->has_lin[k] ? in->linmin[k] : INT64_MIN;
+            ri[23] = in->strict[k];
+            ri[24] = d + 1;
+            for (int e = 0; e < d * d; ++e) r[L.mat + e] = in->mat[om + e];
+            om += (int64_t)d * d;
+            int m0 = -1;
+            for (int j = 0; j < d; ++j) {
+                r[L.lin + j] = in->has_lin[k] ? in->linvec[olin + j] : 0;
+                if (m0 < 0 && r[L.lin + j] != 0) m0 = j;
+            }
+            if (in->has_lin[k]) olin += d;
+            ri[20] = m0;
+            uint64_t hm = 0;
+            for (int rr = 0; rr < in->nh[k]; ++rr) {
+                const int64_t *hr = in->H + oh + (int64_t)rr * d;
+                int lv = -1;
+                for (int j = 0; j < d; ++j) if (hr[j]) { lv = j; break; }
+                if (lv < 0) continue;
+                if ((hm >> lv) & 1) { snprintf(errbuf, 256, "H is not in echelon form"); return 1; }
+                hm |= 1ULL << lv;
+                for (int j = 0; j < d; ++j) r[L.H + (int64_t)lv * d + j] = hr[j];
+            }
+            oh += (int64_t)in->nh[k] * d;
+            r[9] = (int64_t)hm;
+            ri[21] = in->nl[k] ? 2 : 0;
+            ri[22] = in->nl[k];
+            for (int e = 0; e < in->nl[k] * d * 2; ++e) r[L.L + e] = in->L[ol + e];
+            ol += (int64_t)in->nl[k] * d * 2;
+        }
+        // factor and search
+        std::vector<int32_t> pidx;
+        std::vector<int64_t> qs;
+        std::vector<int32_t> cs;
+        std::vector<int64_t> nodes(n, 0);
+        if (in->mode == PFA_MODE_HOST) {
+            std::vector<char> ws((size_t)2 * D * D * 8 + 64);
+            PfaTeam T{0, 1};
+            std::vector<PfaNode> st(PFA_DMAX);
+            int32_t c[PFA_DMAX];
+            for (int64_t k = 0; k < n; ++k) {
+                int64_t *r = recs.data() + k * L.words;
+                factor_one(T, ws.data(), r, L);
+                if (rec_status(r) != PFA_OK) continue;
+                PfaProb P = rec_view(r, L);
+                PfaCounts cnt{0, 0, 0};
+                size_t mark = qs.size();
+                auto emit = [&](const int32_t *cc, int64_t q) -> int {
+                    pidx.push_back((int32_t)k);
+                    qs.push_back(q);
+                    for (int j = 0; j < D; ++j) cs.push_back(j < P.d ? cc[j] : 0);
+                    return 0;
+                };
+                int rr = pfa_dfs(P, st.data(), c, cnt, emit);
+                nodes[k] = (int64_t)cnt.nodes;
+                if (rr) {
+                    rec_status(r) = rr;
+                    pidx.resize(mark); qs.resize(mark); cs.resize(mark * D);
+                }
+            }
+        } else {
+            DevCtx *C = ctx_for(in->device);
+            std::lock_guard<std::mutex> lk(C->mu);
+            HIPCHK(hipSetDevice(in->device));
+            if (!C->stream) HIPCHK(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+            hipStream_t s = C->stream;
+            int64_t *d_recs = (int64_t *)dev_get(C, B_RECS, 8 * recs.size(), errbuf);
+            HIPCHK(hipMemcpyAsync(d_recs, recs.data(), 8 * recs.size(), hipMemcpyHostToDevice, s));
+            if (n > 0)
+                hipLaunchKernelGGL(k_factor, dim3((unsigned)std::min<int64_t>(n, 65536)), dim3(64),
+                                   (size_t)2 * D * D * 8 + 64, s, n, d_recs, L);
+            unsigned long long pcap = std::max<int64_t>(1 << 14, 64 * n);
+            unsigned long long *d_cnt = (unsigned long long *)dev_get(C, B_CNT, 64, errbuf);
+            unsigned long long *d_ctl = (unsigned long long *)dev_get(C, B_CTL, 128, errbuf);
+            int32_t *d_pidx = (int32_t *)dev_get(C, B_PT_PIDX, 4 * pcap, errbuf);
+            int64_t *d_q = (int64_t *)dev_get(C, B_PT_Q, 8 * pcap, errbuf);
+            int32_t *d_c = (int32_t *)dev_get(C, B_PT_C, 4 * pcap * D, errbuf);
+            HIPCHK(hipMemsetAsync(d_ctl, 0, 128, s));
+            HIPCHK(hipMemsetAsync(d_cnt, 0, 64, s));
+            PointArgs Pt{d_pidx, d_q, d_c, pcap, d_ctl + 6, (int *)(d_ctl + 8)};
+            CountArgs Cn{d_cnt};
+            if (n > 0) hipLaunchKernelGGL(k_plain, dim3((n + 63) / 64), dim3(64), 0, s, n, d_recs, L, Pt, D, Cn);
+            HIPCHK(hipGetLastError());
+            unsigned long long npts = 0;
+            int ovf = 0;
+            HIPCHK(hipMemcpyAsync(&npts, d_ctl + 6, 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(&ovf, d_ctl + 8, 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(recs.data(), d_recs, 8 * recs.size(), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            if (ovf || npts > pcap) { snprintf(errbuf, 256, "point buffer overflow"); return 1; }
+            pidx.resize(npts); qs.resize(npts); cs.resize(npts * D);
+            if (npts) {
+                HIPCHK(hipMemcpy(pidx.data(), d_pidx, 4 * npts, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(qs.data(), d_q, 8 * npts, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(cs.data(), d_c, 4 * npts * D, hipMemcpyDeviceToHost));
+            }
+        }
+        // per-problem depth-first order: lexicographic in (c[d-1], ..., c[0])
+        const int64_t np = (int64_t)qs.size();
+        std::vector<int64_t> idx(np);
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) {
+            if (pidx[a] != pidx[b]) return pidx[a] < pidx[b];
+            for (int j = D - 1; j >= 0; --j)
+                if (cs[a * D + j] != cs[b * D + j]) return cs[a * D + j] < cs[b * D + j];
+            return false;
+        });
+        out->dmax = D;
+        out->status = (int32_t *)malloc(4 * std::max<int64_t>(n, 1));
+        out->nodes = (int64_t *)malloc(8 * std::max<int64_t>(n, 1));
+        for (int64_t k = 0; k < n; ++k) {
+            out->status[k] = ((int32_t *)(recs.data() + k * L.words))[0];
+            out->nodes[k] = nodes[k];
+        }
+        int64_t keep = 0;
+        for (int64_t t = 0; t < np; ++t) keep += out->status[pidx[idx[t]]] == PFA_OK;
+        out->n = keep;
+        out->pts = (int32_t *)malloc(4 * std::max<int64_t>(keep, 1) * D);
+        out->qs = (int64_t *)malloc(8 * std::max<int64_t>(keep, 1));
+        out->pidx = (int64_t *)malloc(8 * std::max<int64_t>(keep, 1));
+        int64_t o = 0;
+        for (int64_t t = 0; t < np; ++t) {
+            int64_t k = idx[t];
+            if (out->status[pidx[k]] != PFA_OK) continue;
+            memcpy(out->pts + o * D, &cs[k * D], 4 * D);
+            out->qs[o] = qs[k];
+            out->pidx[o] = pidx[k];
+            o++;
+        }
+        return 0;
+    } catch (int) {
+        return 1;
+    }
+}
+
+extern "C" void pfa_kernel_output_free(pfa_kernel_output *out)
+{
+    free(out->pts); free(out->qs); free(out->pidx); free(out->status); free(out->nodes);
+    out->pts = nullptr; out->qs = nullptr; out->pidx = nullptr; out->status = nullptr;
+    out->nodes = nullptr;
+    out->n = 0;
+}

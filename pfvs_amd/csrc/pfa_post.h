@@ -1,0 +1,138 @@
+// pfa_post.h -- the exact per-point filter of coniZpM's post-processing
+// (pfvs/coni.py:275-412), evaluated where the points are produced.
+//
+// For a lattice point c of p-vector p: M = Binter c, Kn = Z Binter c
+// (int128, must fit int64: fp_kernel.pyx:591-609), q = c^T mat c.
+//   K_gcd = gcd(Kn[1:]) (0 -> 1); the point is kept only if q < Q K_gcd
+//   (coni.py:313-320).  Each g = 1 .. (Q K_gcd - 1) // q admits the K0 with
+//   M0 K0 = g rawQperp - Q, rawQperp = (q + M0 Kn0) / K_gcd, capped by
+//   K' > 0: K0 <= ceil(Kn0 g / K_gcd) - 1 (coni.py:327-365).  `n_exp`
+//   counts those (g, K0); 0 means the K-integrality filter drops the point.
+//   det((kappa M)[1:, 1:]) != 0 (coni.py:397-407): decided exactly by
+//   determinants modulo 31-bit primes until their product exceeds the
+//   Hadamard bound (then every residue 0 proves det = 0).
+#pragma once
+#include <math.h>
+#include "pfa_int.h"
+
+// ceil / floor of a / b for b > 0 (int128)
+PFA_HD static inline pfa_i128 pfa_cdiv_pos(pfa_i128 a, pfa_i128 b)
+{
+    return -pfa_fdiv128(-a, b);
+}
+
+// Number of (g, K0) pairs of one point, or -1 if some value leaves int128.
+PFA_HD static inline int64_t pfa_num_expansions(const int64_t *M, const int64_t *Kn, int h,
+                                               int64_t q, int64_t Q)
+{
+    pfa_u128 kg = 0;
+    for (int i = 1; i < h; ++i) kg = pfa_gcd128(kg, pfa_abs128((pfa_i128)Kn[i]));
+    if (kg == 0) kg = 1;
+    if ((kg >> 62) != 0) return -1;
+    pfa_i128 Kgcd = (pfa_i128)kg;
+    pfa_i128 QK = (pfa_i128)Q * Kgcd;
+    if (!((pfa_i128)q < QK)) return 0;
+    if (q <= 0) return -1;                    // never: M0 >= linmin > 0 forces q > 0
+    pfa_i128 M0 = M[0];
+    if (M0 <= 0) return -1;
+    pfa_i128 num = (pfa_i128)q + M0 * (pfa_i128)Kn[0];
+    if (num % Kgcd != 0) return -1;          // exact by construction
+    pfa_i128 raw = num / Kgcd;
+    pfa_i128 gmax = (QK - 1) / q;
+    if (gmax > 1000000) return -1;
+    int64_t cnt = 0;
+    for (pfa_i128 g = 1; g <= gmax; ++g) {
+        pfa_i128 Qp = g * raw;
+        pfa_i128 lo = pfa_cdiv_pos(Qp - Q, M0);
+        pfa_i128 up = pfa_fdiv128(Qp - Q, M0);
+        pfa_i128 cap = pfa_cdiv_pos((pfa_i128)Kn[0] * g, Kgcd) - 1;
+        if (cap < up) up = cap;
+        if (up >= lo) cnt += (int64_t)(up - lo + 1);
+    }
+    return cnt;
+}
+
+// (a * b) mod p for a, b < p < 2^31
+PFA_HD static inline uint32_t pfa_mulmod(uint32_t a, uint32_t b, uint32_t p)
+{
+    return (uint32_t)(((uint64_t)a * b) % p);
+}
+
+PFA_HD static inline uint32_t pfa_powmod(uint32_t a, uint32_t e, uint32_t p)
+{
+    uint32_t r = 1;
+    while (e) {
+        if (e & 1) r = pfa_mulmod(r, a, p);
+        a = pfa_mulmod(a, a, p);
+        e >>= 1;
+    }
+    return r;
+}
+
+PFA_HD static inline uint32_t pfa_mod(int64_t x, uint32_t p)
+{
+    int64_t r = x % (int64_t)p;
+    return (uint32_t)(r < 0 ? r + p : r);
+}
+
+// det(A) mod p of the n x n int64 matrix A (row stride n); work: n*n uint32.
+// Sequential (one lane per matrix).
+PFA_HD static inline uint32_t pfa_det_mod(const int64_t *A, int n, uint32_t p, uint32_t *w)
+{
+    for (int e = 0; e < n * n; ++e) w[e] = pfa_mod(A[e], p);
+    uint32_t det = 1;
+    for (int k = 0; k < n; ++k) {
+        int piv = -1;
+        for (int i = k; i < n; ++i) if (w[i * n + k]) { piv = i; break; }
+        if (piv < 0) return 0;
+        if (piv != k) {
+            for (int j = k; j < n; ++j) { uint32_t t = w[k * n + j]; w[k * n + j] = w[piv * n + j]; w[piv * n + j] = t; }
+            det = det ? p - det : 0;
+        }
+        uint32_t pv = w[k * n + k];
+        det = pfa_mulmod(det, pv, p);
+        uint32_t inv = pfa_powmod(pv, p - 2, p);
+        for (int i = k + 1; i < n; ++i) {
+            uint32_t f = pfa_mulmod(w[i * n + k], inv, p);
+            if (!f) continue;
+            for (int j = k + 1; j < n; ++j) {
+                uint32_t s = pfa_mulmod(f, w[k * n + j], p);
+                uint32_t x = w[i * n + j];
+                w[i * n + j] = x >= s ? x - s : x + p - s;
+            }
+        }
+    }
+    return det;
+}
+
+// log2 of the Hadamard bound prod_i |row_i|_2 of the n x n matrix, rounded
+// up generously (each row norm computed in double with 1e-12 slack).
+PFA_HD static inline double pfa_hadamard_log2(const int64_t *A, int n)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double r = 0.0;
+        for (int j = 0; j < n; ++j) {
+            double x = (double)A[i * n + j];
+            r += x * x;
+        }
+        if (r == 0.0) return -1e300;            // a zero row: det = 0
+        s += 0.5 * log2(r) + 1e-9;
+    }
+    return s + 1.0;
+}
+
+// 1 nonsingular, 0 singular, 2 undecided (more primes needed)
+PFA_HD static inline int pfa_nonsingular(const int64_t *A, int n, const uint32_t *primes, int np,
+                                         uint32_t *w)
+{
+    double need = pfa_hadamard_log2(A, n);
+    if (need < -1e299) return 0;
+    double have = 0.0;
+    for (int k = 0; k < np; ++k) {
+        if (pfa_det_mod(A, n, primes[k], w) != 0) return 1;
+        have += log2((double)primes[k]) - 1e-9;
+        if (have > need) return 0;
+    }
+    return 2;
+}

@@ -1,0 +1,655 @@
+// pfa_prep.h -- per-p preparation of the coniZpM search, written once for a
+// "team" of lanes: on the device a team is one wavefront (64 lanes, one
+// workgroup) with the working matrices in LDS; on the host a team is one
+// thread.  Scalar control flow is evaluated identically by every lane from
+// LDS data; loops over columns / matrix entries are split across lanes, and
+// every reduction that feeds a floating-point decision is evaluated by each
+// lane in the same order as the host, so both produce the same basis.
+//
+// Steps (pfvs/fp_kernel/pfv_lattice.h:604-680, restated):
+//   Z = kappa.p; v = (Z p)^T Mbasis; O = basis of v^perp (Bezout
+//   elimination, 151-188); LLL(O) with an exact int128 Gram matrix
+//   (393-521); Binter = Mbasis O^T with the Binter[0] == 0 columns first;
+//   ZB = Z Binter; mat = -Binter^T ZB; cut-aware basis T = [K | w]
+//   (554-602) so that Binter[0] = (0, ..., 0, g); H = row HNF of ZB[1:]
+//   (250-313).  Every intermediate is checked: overflow returns -1 (the
+//   problem then goes to the exact host path).
+// Then the FP64 Cholesky factor with its proven backward-error margin phi
+// (replacing the reference's exact Bareiss factor, fpk_common.h:232-309).
+#pragma once
+#include <math.h>
+#include <string.h>
+#include "pfa_int.h"
+#include "pfa_search.h"
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#define PFA_SYNC() __syncthreads()
+#else
+#define PFA_SYNC() ((void)0)
+#endif
+
+struct PfaTeam {
+    int lane, n;
+};
+
+// Working storage of one preparation (LDS on the device).  Sizes for h:
+//   Z, W1, W2, W3: h*h int64; Binter, ZB: h*(h-1); mat: (h-1)^2;
+//   GG: (h+1)^2 int128 (later the HNF); mu: h*h double; bb, mw, vec: h;
+//   flag: one int.
+struct PfaWS {
+    int h, d;
+    int64_t *Z, *Bi, *ZB, *mat, *W1, *W2, *W3, *vec;
+    pfa_i128 *GG;
+    double *mu, *bb, *mw;
+    int *flag;
+};
+
+// Bytes of a PfaWS for h (everything 16-byte aligned).
+PFA_HD static inline size_t pfa_ws_bytes(int h)
+{
+    size_t d = h - 1, w = 0;
+    w += (size_t)h * h * 8 * 4;          // Z, W1, W2, W3
+    w += (size_t)h * d * 8 * 2;          // Bi, ZB
+    w += d * d * 8;                      // mat
+    w += (size_t)(h + 1) * (h + 1) * 16; // GG / H
+    w += (size_t)h * h * 8;              // mu
+    w += (size_t)h * 8 * 3;              // bb, mw, vec
+    w += 64;                             // flag + padding
+    return (w + 15) & ~(size_t)15;
+}
+
+PFA_HD static inline void pfa_ws_carve(PfaWS &W, char *base, int h)
+{
+    size_t d = h - 1;
+    char *p = base;
+    auto take = [&](size_t bytes) { char *r = p; p += (bytes + 15) & ~(size_t)15; return r; };
+    W.h = h;
+    W.d = (int)d;
+    W.GG = (pfa_i128 *)take((size_t)(h + 1) * (h + 1) * 16);
+    W.Z = (int64_t *)take((size_t)h * h * 8);
+    W.W1 = (int64_t *)take((size_t)h * h * 8);
+    W.W2 = (int64_t *)take((size_t)h * h * 8);
+    W.W3 = (int64_t *)take((size_t)h * h * 8);
+    W.Bi = (int64_t *)take((size_t)h * d * 8);
+    W.ZB = (int64_t *)take((size_t)h * d * 8);
+    W.mat = (int64_t *)take(d * d * 8);
+    W.mu = (double *)take((size_t)h * h * 8);
+    W.bb = (double *)take((size_t)h * 8);
+    W.mw = (double *)take((size_t)h * 8);
+    W.vec = (int64_t *)take((size_t)h * 8);
+    W.flag = (int *)take(16);
+}
+
+// team-wide "any lane failed": lanes set *flag, then all read it
+PFA_HD static inline int pfa_team_any(const PfaTeam &T, int *flag, int mine)
+{
+    if (mine) *flag = 1;
+    PFA_SYNC();
+    int r = *flag;
+    PFA_SYNC();
+    return r;
+}
+
+PFA_HD static inline void pfa_team_clear(const PfaTeam &T, int *flag)
+{
+    if (T.lane == 0) *flag = 0;
+    PFA_SYNC();
+}
+
+// Extended Euclid with floor division (signs as the reference's Bezout
+// steps, pfv_lattice.h:116-130).
+PFA_HD static inline int64_t pfa_xgcd(int64_t a, int64_t b, int64_t *s, int64_t *t)
+{
+    int64_t r0 = a, r1 = b, s0 = 1, s1 = 0, t0 = 0, t1 = 1;
+    while (r1 != 0) {
+        int64_t q = pfa_fdiv64(r0, r1), x;
+        x = r0 - q * r1; r0 = r1; r1 = x;
+        x = s0 - q * s1; s0 = s1; s1 = x;
+        x = t0 - q * t1; t0 = t1; t1 = x;
+    }
+    *s = s0;
+    *t = t0;
+    return r0;
+}
+
+// U (n x n, stride n) unimodular with U v = (gcd, 0, ..., 0); v is read
+// from shared memory and not modified.  Returns 0, -1 overflow, -2 degenerate.
+PFA_HD static inline int pfa_unimodular(const PfaTeam &T, const int64_t *v, int n, int64_t *U, int *flag)
+{
+    for (int e = T.lane; e < n * n; e += T.n) U[e] = (e / n) == (e % n);
+    pfa_team_clear(T, flag);
+    int64_t w0 = v[0];
+    int bad = 0;
+    for (int k = 1; k < n; ++k) {
+        int64_t wk = v[k];
+        if (wk == 0) continue;
+        int64_t s, t, g = pfa_xgcd(w0, wk, &s, &t);
+        if (g == 0) return -2;
+        int64_t m10 = -pfa_fdiv64(wk, g), m11 = pfa_fdiv64(w0, g);
+        w0 = g;
+        for (int r = T.lane; r < n; r += T.n) {      // each lane owns columns r
+            pfa_i128 a = (pfa_i128)s * U[r] + (pfa_i128)t * U[k * n + r];
+            pfa_i128 b = (pfa_i128)m10 * U[r] + (pfa_i128)m11 * U[k * n + r];
+            if (!pfa_fits64(a) || !pfa_fits64(b)) bad = 1;
+            U[r] = (int64_t)a;
+            U[k * n + r] = (int64_t)b;
+        }
+    }
+    return pfa_team_any(T, flag, bad) ? -1 : 0;
+}
+
+// Gram-Schmidt row kk from the exact Gram matrix (every lane, same order)
+PFA_HD static inline void pfa_gs_row(const PfaWS &W, int kk, int L, const pfa_i128 *GG)
+{
+    double *mu = W.mu, *bb = W.bb;
+    const int S = W.h;
+    for (int j = 0; j < kk; ++j) {
+        double dd = (double)GG[kk * L + j];
+        for (int l = 0; l < j; ++l) dd -= mu[j * S + l] * mu[kk * S + l] * bb[l];
+        mu[kk * S + j] = dd / bb[j];
+    }
+    double dd = (double)GG[kk * L + kk];
+    for (int l = 0; l < kk; ++l) dd -= mu[kk * S + l] * mu[kk * S + l] * bb[l];
+    bb[kk] = dd;
+}
+
+// b_k -= q b_j on the rows (columns split over lanes) and on GG (exact).
+// Row index m means `last`.
+PFA_HD static inline int pfa_lll_sub(const PfaTeam &T, const PfaWS &W, int64_t *B, int64_t *last,
+                                     int m, int n, pfa_i128 *GG, int L, int M, int kk, int jj,
+                                     int64_t q)
+{
+    int64_t *bk = kk == m ? last : &B[kk * n];
+    const int64_t *bj = jj == m ? last : &B[jj * n];
+    int bad = 0;
+    for (int c = T.lane; c < n; c += T.n) {
+        pfa_i128 x = (pfa_i128)bk[c] - (pfa_i128)q * bj[c];
+        if (!pfa_fits64(x)) bad = 1;
+        bk[c] = (int64_t)x;
+    }
+    // new G_kk = G_kk - 2 q G_kj + q^2 G_jj, computed before row kk changes
+    pfa_i128 kj = GG[kk * L + jj], jv = GG[jj * L + jj], t1, t2;
+    int badk = pfa_mul_ovf(jv, (pfa_i128)q, &t1) || pfa_sub_ovf(t1, 2 * kj, &t1) ||
+               pfa_mul_ovf(t1, (pfa_i128)q, &t2) || pfa_add_ovf(GG[kk * L + kk], t2, &t2);
+    PFA_SYNC();
+    for (int l = T.lane; l < M; l += T.n) {
+        if (l == kk) continue;
+        pfa_i128 t3;
+        if (pfa_mul_ovf(GG[jj * L + l], (pfa_i128)q, &t3) || pfa_sub_ovf(GG[kk * L + l], t3, &t3))
+            bad = 1;
+        GG[kk * L + l] = t3;
+        GG[l * L + kk] = t3;
+    }
+    if (T.lane == 0) GG[kk * L + kk] = t2;
+    return pfa_team_any(T, W.flag, bad || badk);
+}
+
+// LLL (delta 0.99) of the m rows of B (length n, stride n) given their exact
+// Gram matrix GG (stride L = m + 1); `last` (row m of GG) is size-reduced
+// afterwards.  0, -1 overflow, -2 rank loss, -3 no convergence.
+PFA_HD static inline int pfa_lll_core(const PfaTeam &T, const PfaWS &W, int64_t *B, int m, int n,
+                                      pfa_i128 *GG, int64_t *last)
+{
+    const int L = m + 1, M = m + (last != nullptr), S = W.h;
+    double *mu = W.mu, *bb = W.bb, *mw = W.mw;
+    int iters = 0;
+    pfa_team_clear(T, W.flag);
+    if (m >= 1) {
+        pfa_gs_row(W, 0, L, GG);
+        if (!(bb[0] > 0)) return -2;
+    }
+    if (m >= 2) pfa_gs_row(W, 1, L, GG);
+    PFA_SYNC();
+    int k = 1;
+    while (k < m) {
+        if (++iters > 100000) return -3;
+        for (int pass = 0; pass < 4; ++pass) {
+            int changed = 0;
+            for (int j = k - 1; j >= 0; --j) {
+                double q = nearbyint(mu[k * S + j]);
+                if (q == 0.0) continue;
+                if (!(fabs(q) <= 9.0e18)) return -1;
+                if (pfa_lll_sub(T, W, B, last, m, n, GG, L, M, k, j, (int64_t)q)) return -1;
+                for (int l = 0; l < j; ++l) mu[k * S + l] -= q * mu[j * S + l];
+                mu[k * S + j] -= q;
+                changed = 1;
+                PFA_SYNC();
+            }
+            if (!changed) break;
+            pfa_gs_row(W, k, L, GG);
+            PFA_SYNC();
+        }
+        if (!(bb[k] > 0)) return -2;
+        if (bb[k] < (0.99 - mu[k * S + k - 1] * mu[k * S + k - 1]) * bb[k - 1]) {
+            for (int c = T.lane; c < n; c += T.n) {
+                int64_t t = B[k * n + c];
+                B[k * n + c] = B[(k - 1) * n + c];
+                B[(k - 1) * n + c] = t;
+            }
+            for (int l = T.lane; l < M; l += T.n) {
+                pfa_i128 t = GG[k * L + l];
+                GG[k * L + l] = GG[(k - 1) * L + l];
+                GG[(k - 1) * L + l] = t;
+            }
+            PFA_SYNC();
+            for (int l = T.lane; l < M; l += T.n) {
+                pfa_i128 t = GG[l * L + k];
+                GG[l * L + k] = GG[l * L + k - 1];
+                GG[l * L + k - 1] = t;
+            }
+            PFA_SYNC();
+            k = k > 1 ? k - 1 : 1;
+            if (k == 1) pfa_gs_row(W, 0, L, GG);
+            pfa_gs_row(W, k - 1, L, GG);
+            pfa_gs_row(W, k, L, GG);
+            PFA_SYNC();
+        } else {
+            k++;
+            if (k < m) pfa_gs_row(W, k, L, GG);
+            PFA_SYNC();
+        }
+    }
+    if (last && m > 0) {
+        for (int pass = 0; pass < 4; ++pass) {
+            for (int j = 0; j < m; ++j) {
+                double dd = (double)GG[m * L + j];
+                for (int l = 0; l < j; ++l) dd -= mu[j * S + l] * mw[l] * bb[l];
+                mw[j] = dd / bb[j];
+            }
+            PFA_SYNC();
+            int changed = 0;
+            for (int j = m - 1; j >= 0; --j) {
+                double q = nearbyint(mw[j]);
+                if (q == 0.0) continue;
+                if (!(fabs(q) <= 9.0e18)) return -1;
+                if (pfa_lll_sub(T, W, B, last, m, n, GG, L, M, m, j, (int64_t)q)) return -1;
+                for (int l = 0; l < j; ++l) mw[l] -= q * mu[j * S + l];
+                mw[j] -= q;
+                changed = 1;
+                PFA_SYNC();
+            }
+            if (!changed) break;
+        }
+    }
+    PFA_SYNC();
+    return 0;
+}
+
+// Euclidean LLL of the m rows of B
+PFA_HD static inline int pfa_lll_euclid(const PfaTeam &T, const PfaWS &W, int64_t *B, int m, int n)
+{
+    if (m < 2) return 0;
+    const int L = m + 1;
+    int bad = 0;
+    for (int e = T.lane; e < m * m; e += T.n) {
+        int i = e / m, j = e % m;
+        if (j > i) continue;
+        pfa_i128 s = 0;
+        for (int c = 0; c < n; ++c)
+            if (pfa_add_ovf(s, (pfa_i128)B[i * n + c] * B[j * n + c], &s)) bad = 1;
+        W.GG[i * L + j] = s;
+        W.GG[j * L + i] = s;
+    }
+    pfa_team_clear(T, W.flag);
+    if (pfa_team_any(T, W.flag, bad)) return -1;
+    return pfa_lll_core(T, W, B, m, n, W.GG, nullptr);
+}
+
+// LLL of the m rows of B w.r.t. the form G (n x n); `last` size-reduced.
+// Gy (n entries) uses W.vec as scratch.
+PFA_HD static inline int pfa_lll_form(const PfaTeam &T, const PfaWS &W, int64_t *B, int m, int n,
+                                      const int64_t *G, int64_t *last)
+{
+    const int L = m + 1, M = m + (last != nullptr);
+    pfa_i128 *Gy = (pfa_i128 *)W.mu;        // mu is free until pfa_lll_core
+    int bad = 0;
+    pfa_team_clear(T, W.flag);
+    for (int j = 0; j < M; ++j) {
+        const int64_t *y = j == m ? last : &B[j * n];
+        for (int i = T.lane; i < n; i += T.n) {
+            pfa_i128 r = 0;
+            for (int c = 0; c < n; ++c)
+                if (pfa_add_ovf(r, (pfa_i128)G[i * n + c] * y[c], &r)) bad = 1;
+            Gy[i] = r;
+        }
+        PFA_SYNC();
+        for (int i = T.lane; i <= j; i += T.n) {
+            const int64_t *x = i == m ? last : &B[i * n];
+            pfa_i128 s = 0, t;
+            for (int c = 0; c < n; ++c) {
+                if (!x[c]) continue;
+                if (pfa_mul_ovf(Gy[c], (pfa_i128)x[c], &t) || pfa_add_ovf(s, t, &s)) bad = 1;
+            }
+            W.GG[i * L + j] = s;
+            W.GG[j * L + i] = s;
+        }
+        PFA_SYNC();
+    }
+    if (pfa_team_any(T, W.flag, bad)) return -1;
+    return pfa_lll_core(T, W, B, m, n, W.GG, last);
+}
+
+// C = A B (int64, exact checks); entries split over lanes.  C must not alias.
+PFA_HD static inline int pfa_matmul(const PfaTeam &T, const PfaWS &W, const int64_t *A,
+                                    const int64_t *B, int64_t *C, int m, int k, int n)
+{
+    int bad = 0;
+    for (int e = T.lane; e < m * n; e += T.n) {
+        int i = e / n, j = e % n;
+        pfa_i128 s = 0;
+        for (int l = 0; l < k; ++l)
+            if (pfa_add_ovf(s, (pfa_i128)A[i * k + l] * B[l * n + j], &s)) bad = 1;
+        if (!pfa_fits64(s)) bad = 1;
+        C[e] = (int64_t)s;
+    }
+    pfa_team_clear(T, W.flag);
+    return pfa_team_any(T, W.flag, bad) ? -1 : 0;
+}
+
+PFA_HD static inline void pfa_copy(const PfaTeam &T, int64_t *dst, const int64_t *src, int n)
+{
+    PFA_SYNC();
+    for (int e = T.lane; e < n; e += T.n) dst[e] = src[e];
+    PFA_SYNC();
+}
+
+// ---------------------------------------------------------------- HNF
+// Row HNF (upper echelon, positive pivots, entries above a pivot reduced
+// into [0, pivot)) of the r x n int64 matrix A, rows inserted one at a time
+// into an already reduced HNF (pfv_lattice.h:205-313).  Entries are kept
+// below 2^125; -1 on overflow.  H: r x n int128 (zero rows last).
+// Row ops are split over lanes; pivots/quotients are read by every lane.
+#define PFA_HLIM ((pfa_i128)1 << 125)
+
+PFA_HD static inline int pfa_hnf_muladd(pfa_i128 a, pfa_i128 b, pfa_i128 c, pfa_i128 *r)
+{
+    pfa_i128 x;
+    if (pfa_mul_ovf(a, b, &x) || pfa_add_ovf(x, c, &x) || !(x < PFA_HLIM && x > -PFA_HLIM)) return 1;
+    *r = x;
+    return 0;
+}
+
+// row[c0:] -= q prow[c0:] (team); returns the failure flag of this lane
+PFA_HD static inline int pfa_hnf_rowsub(const PfaTeam &T, pfa_i128 *row, const pfa_i128 *prow,
+                                        pfa_i128 q, int c0, int n)
+{
+    int bad = 0;
+    if (q == 0) return 0;
+    for (int c = c0 + T.lane; c < n; c += T.n)
+        if (pfa_hnf_muladd(-q, prow[c], row[c], &row[c])) bad = 1;
+    return bad;
+}
+
+// reduce row by rows t0..m-1 of H at their pivots (in order)
+PFA_HD static inline int pfa_hnf_reduce(const PfaTeam &T, int *flag, pfa_i128 *row,
+                                        const pfa_i128 *H, const int *piv, int t0, int m, int n)
+{
+    int bad = 0;
+    for (int l = t0; l < m; ++l) {
+        int j = piv[l];
+        pfa_i128 q = pfa_fdiv128(row[j], H[l * n + j]);
+        PFA_SYNC();
+        bad |= pfa_hnf_rowsub(T, row, &H[l * n], q, j, n);
+        PFA_SYNC();
+    }
+    return pfa_team_any(T, flag, bad);
+}
+
+PFA_HD static inline int pfa_hnf(const PfaTeam &T, const PfaWS &W, const int64_t *A, int r, int n,
+                                 pfa_i128 *H, pfa_i128 *v, pfa_i128 *w, int *piv)
+{
+    int m = 0;
+    pfa_team_clear(T, W.flag);
+    for (int k = 0; k < r; ++k) {
+        for (int c = T.lane; c < n; c += T.n) v[c] = A[k * n + c];
+        PFA_SYNC();
+        int t = 0;
+        for (;;) {
+            int j = 0;
+            while (j < n && v[j] == 0) ++j;
+            if (j == n) break;
+            while (t < m && piv[t] < j) ++t;
+            if (t == m || piv[t] > j) {
+                // new pivot j: insert v as row t
+                PFA_SYNC();
+                for (int l = m; l > t; --l)
+                    for (int c = T.lane; c < n; c += T.n) H[l * n + c] = H[(l - 1) * n + c];
+                PFA_SYNC();
+                if (T.lane == 0)
+                    for (int l = m; l > t; --l) piv[l] = piv[l - 1];
+                int neg = v[j] < 0;
+                for (int c = T.lane; c < n; c += T.n) H[t * n + c] = neg ? -v[c] : v[c];
+                if (T.lane == 0) piv[t] = j;
+                PFA_SYNC();
+                m++;
+                if (pfa_hnf_reduce(T, W.flag, &H[t * n], H, piv, t + 1, m, n)) return -1;
+                break;
+            }
+            pfa_i128 *hr = &H[t * n];
+            pfa_i128 a = hr[j], b = v[j];
+            int bad = 0;
+            if (b % a == 0) {
+                pfa_i128 q = b / a;
+                PFA_SYNC();
+                bad = pfa_hnf_rowsub(T, v, hr, q, j, n);
+                if (pfa_team_any(T, W.flag, bad)) return -1;
+            } else {
+                pfa_i128 r0 = a, r1 = b, s0 = 1, s1 = 0, x0 = 0, x1 = 1;
+                while (r1) {
+                    pfa_i128 q = r0 / r1, tt;
+                    tt = r0 - q * r1; r0 = r1; r1 = tt;
+                    tt = s0 - q * s1; s0 = s1; s1 = tt;
+                    tt = x0 - q * x1; x0 = x1; x1 = tt;
+                }
+                if (r0 < 0) { r0 = -r0; s0 = -s0; x0 = -x0; }
+                pfa_i128 ag = a / r0, bg = b / r0;
+                PFA_SYNC();
+                for (int c = j + T.lane; c < n; c += T.n) {
+                    pfa_i128 hn = 0, vn = 0, t1 = 0;
+                    if (pfa_mul_ovf(s0, hr[c], &t1) || pfa_hnf_muladd(x0, v[c], t1, &hn)) bad = 1;
+                    if (pfa_mul_ovf(-bg, hr[c], &t1) || pfa_hnf_muladd(ag, v[c], t1, &vn)) bad = 1;
+                    w[c] = hn;
+                    v[c] = vn;
+                }
+                PFA_SYNC();
+                for (int c = j + T.lane; c < n; c += T.n) hr[c] = w[c];
+                if (pfa_team_any(T, W.flag, bad)) return -1;
+                if (pfa_hnf_reduce(T, W.flag, hr, H, piv, t + 1, m, n)) return -1;
+            }
+            if (pfa_hnf_reduce(T, W.flag, v, H, piv, t + 1, m, n)) return -1;
+            for (int l = 0; l < t; ++l)
+                if (pfa_hnf_reduce(T, W.flag, &H[l * n], H, piv, t, m, n)) return -1;
+            t++;
+        }
+        for (int l = 0; l < t && l < m; ++l)
+            if (pfa_hnf_reduce(T, W.flag, &H[l * n], H, piv, t < m ? t : m, m, n)) return -1;
+    }
+    for (int e = m * n + T.lane; e < r * n; e += T.n) H[e] = 0;
+    PFA_SYNC();
+    return 0;
+}
+
+// --------------------------------------------------- lattice build
+// Builds Z, Binter (W.Bi), ZB, mat for p (length h, p[0] = 0).  Returns 0
+// or 10 * rc - stage (stage 1 orthogonal, 2 LLL, 4 cut-aware basis, 0 a
+// product overflow), as pfl_build's status (pfv_lattice.h:604-680).
+PFA_HD static inline int pfa_build_lattice(const PfaTeam &T, const PfaWS &W, const int64_t *kappa,
+                                           const int64_t *Mbasis, const int64_t *p)
+{
+    const int h = W.h, d = W.d;
+    int bad = 0, rc;
+    pfa_team_clear(T, W.flag);
+    for (int e = T.lane; e < h * h; e += T.n) {
+        pfa_i128 s = 0;
+        for (int k = 0; k < h; ++k)
+            if (pfa_add_ovf(s, (pfa_i128)kappa[(size_t)e * h + k] * p[k], &s)) bad = 1;
+        if (!pfa_fits64(s)) bad = 1;
+        W.Z[e] = (int64_t)s;
+    }
+    if (pfa_team_any(T, W.flag, bad)) return -1;
+    // T = Z p into W.W2[0:h]; v = T^T Mbasis into W.vec
+    int64_t *Tv = W.W2;
+    for (int i = T.lane; i < h; i += T.n) {
+        pfa_i128 s = 0;
+        for (int k = 0; k < h; ++k)
+            if (pfa_add_ovf(s, (pfa_i128)W.Z[i * h + k] * p[k], &s)) bad = 1;
+        if (!pfa_fits64(s)) bad = 1;
+        Tv[i] = (int64_t)s;
+    }
+    if (pfa_team_any(T, W.flag, bad)) return -1;
+    for (int j = T.lane; j < h; j += T.n) {
+        pfa_i128 s = 0;
+        for (int l = 0; l < h; ++l)
+            if (pfa_add_ovf(s, (pfa_i128)Tv[l] * Mbasis[l * h + j], &s)) bad = 1;
+        if (!pfa_fits64(s)) bad = 1;
+        W.vec[j] = (int64_t)s;
+    }
+    if (pfa_team_any(T, W.flag, bad)) return -1;
+    // orthogonal lattice: rows 1.. of the unimodular U (W1)
+    if ((rc = pfa_unimodular(T, W.vec, h, W.W1, W.flag))) return rc * 10 - 1;
+    int64_t *O = W.W1 + h;                            // (d, h) rows
+    PFA_SYNC();
+    if ((rc = pfa_lll_euclid(T, W, O, d, h))) return rc * 10 - 2;
+    // BT[a][i] = sum_l Mbasis[i][l] O[a][l]  -> W2 (d, h)
+    int64_t *BT = W.W2;
+    for (int e = T.lane; e < d * h; e += T.n) {
+        int a = e / h, i = e % h;
+        pfa_i128 s = 0;
+        for (int l = 0; l < h; ++l)
+            if (pfa_add_ovf(s, (pfa_i128)Mbasis[i * h + l] * O[a * h + l], &s)) bad = 1;
+        if (!pfa_fits64(s)) bad = 1;
+        BT[e] = (int64_t)s;
+    }
+    if (pfa_team_any(T, W.flag, bad)) return -1;
+    // columns with Binter[0] == 0 first (stable); order in W3
+    int64_t *order = W.W3;
+    if (T.lane == 0) {
+        int no = 0;
+        for (int a = 0; a < d; ++a) if (BT[a * h] == 0) order[no++] = a;
+        for (int a = 0; a < d; ++a) if (BT[a * h] != 0) order[no++] = a;
+    }
+    PFA_SYNC();
+    for (int e = T.lane; e < h * d; e += T.n) {
+        int i = e / d, a = e % d;
+        W.Bi[e] = BT[order[a] * h + i];
+    }
+    PFA_SYNC();
+    if (pfa_matmul(T, W, W.Z, W.Bi, W.ZB, h, h, d)) return -1;
+    for (int e = T.lane; e < d * d; e += T.n) {
+        int a = e / d, b = e % d;
+        pfa_i128 s = 0;
+        for (int i = 0; i < h; ++i)
+            if (pfa_add_ovf(s, (pfa_i128)W.Bi[i * d + a] * W.ZB[i * d + b], &s)) bad = 1;
+        if (!pfa_fits64(s)) bad = 1;
+        W.mat[e] = (int64_t)(-s);
+    }
+    if (pfa_team_any(T, W.flag, bad)) return -1;
+
+    // cut-aware basis: V l = (g, 0, ...) for l = Binter[0, :]
+    int nz = 0;
+    for (int a = 0; a < d; ++a) nz += W.Bi[a] != 0;
+    if (nz == 0) return 0;
+    int64_t *l = W.vec;
+    PFA_SYNC();
+    for (int a = T.lane; a < d; a += T.n) l[a] = W.Bi[a];
+    PFA_SYNC();
+    int64_t *V = W.W1;                                 // (d, d); row 0 = w, rows 1.. = K
+    if ((rc = pfa_unimodular(T, l, d, V, W.flag))) return rc * 10 - 4;
+    PFA_SYNC();
+    if ((rc = pfa_lll_form(T, W, V + d, d - 1, d, W.mat, V))) return rc * 10 - 4;
+    int64_t *Tm = W.W2;                                // T[i][a]: column a = K_a, last = w
+    for (int e = T.lane; e < d * d; e += T.n) {
+        int i = e / d, a = e % d;
+        Tm[e] = a < d - 1 ? V[d + a * d + i] : V[i];
+    }
+    PFA_SYNC();
+    if (pfa_matmul(T, W, W.Bi, Tm, W.W3, h, d, d)) return -14;
+    pfa_copy(T, W.Bi, W.W3, h * d);
+    if (pfa_matmul(T, W, W.ZB, Tm, W.W3, h, d, d)) return -14;
+    pfa_copy(T, W.ZB, W.W3, h * d);
+    if (pfa_matmul(T, W, W.mat, Tm, W.W3, d, d, d)) return -14;   // MT = mat T
+    for (int e = T.lane; e < d * d; e += T.n) {
+        int a = e / d, b = e % d;
+        pfa_i128 s = 0;
+        for (int i = 0; i < d; ++i)
+            if (pfa_add_ovf(s, (pfa_i128)Tm[i * d + a] * W.W3[i * d + b], &s)) bad = 1;
+        if (!pfa_fits64(s)) bad = 1;
+        W.W1[e] = (int64_t)s;
+    }
+    if (pfa_team_any(T, W.flag, bad)) return -14;
+    pfa_copy(T, W.mat, W.W1, d * d);
+    return 0;
+}
+
+// -------------------------------------------------- FP64 Cholesky + phi
+// U (d x d, upper) with U^T U = mat + E in FP64.  By Higham, Accuracy and
+// Stability of Numerical Algorithms, Thm 10.3, |E| <= gamma_{d+1} |U|^T |U|
+// for any summation order.  Then for integer c,
+//   |c^T E c| <= gamma_{d+1} | |U| |c| |^2 <= gamma_{d+1} ||U||_F^2 ||U^-1||_2^2 |U c|^2,
+// and ||U^-1||_2 <= ||X||_F / (1 - rho) for X ~ U^-1 with
+// rho >= ||I - U X||_F (Neumann series), rho from the computed residual
+// plus its own rounding bound gamma_{d+1} || |U| |X| ||_F.  So
+//   (1 - phi) |U c|^2 <= q(c) <= (1 + phi) |U c|^2,
+//   phi = gamma_{d+1} ||U||_F^2 ||X||_F^2 / (1 - rho)^2   (with 1% slack).
+// Norm sums run on one lane in a fixed order (all lanes compute them), so
+// the host and device get the same phi.  Returns 0, PFA_NOT_PD if the
+// factorization broke down (the exact host path then decides), or
+// PFA_HOST if phi is too large to be useful or mat is not exact in FP64.
+PFA_HD static inline int pfa_cholesky(const PfaTeam &T, const int64_t *mat, int d, double *U,
+                                      double *X, double *phi_out, int *flag)
+{
+    int bad = 0;
+    for (int e = T.lane; e < d * d; e += T.n) {
+        int64_t m = mat[e];
+        if (m > (INT64_C(1) << 53) || m < -(INT64_C(1) << 53)) bad = 1;
+        U[e] = 0.0;
+    }
+    pfa_team_clear(T, flag);
+    if (pfa_team_any(T, flag, bad)) return PFA_HOST;
+    for (int k = 0; k < d; ++k) {
+        double s = (double)mat[k * d + k];
+        for (int l = 0; l < k; ++l) s -= U[l * d + k] * U[l * d + k];
+        if (!(s > 0.0)) return PFA_NOT_PD;
+        double ukk = sqrt(s);
+        PFA_SYNC();
+        if (T.lane == 0) U[k * d + k] = ukk;
+        for (int j = k + 1 + T.lane; j < d; j += T.n) {
+            double t = (double)mat[k * d + j];
+            for (int l = 0; l < k; ++l) t -= U[l * d + k] * U[l * d + j];
+            U[k * d + j] = t / ukk;
+        }
+        PFA_SYNC();
+    }
+    // X = U^-1 (upper), column j by back substitution
+    for (int j = T.lane; j < d; j += T.n) {
+        for (int i = d - 1; i >= 0; --i) {
+            if (i > j) { X[i * d + j] = 0.0; continue; }
+            double t = i == j ? 1.0 : 0.0;
+            for (int l = i + 1; l <= j; ++l) t -= U[i * d + l] * X[l * d + j];
+            X[i * d + j] = t / U[i * d + i];
+        }
+    }
+    PFA_SYNC();
+    const double u = 0x1p-53;
+    const double g = (d + 1) * u / (1.0 - (d + 1) * u);
+    double nU = 0.0, nX = 0.0, nR = 0.0, nA = 0.0;
+    for (int i = 0; i < d; ++i)
+        for (int j = i; j < d; ++j) {
+            nU += U[i * d + j] * U[i * d + j];
+            nX += X[i * d + j] * X[i * d + j];
+            double r = (i == j) ? 1.0 : 0.0, a = 0.0;
+            for (int l = i; l <= j; ++l) {
+                r -= U[i * d + l] * X[l * d + j];
+                a += fabs(U[i * d + l] * X[l * d + j]);
+            }
+            nR += r * r;
+            nA += a * a;
+        }
+    // strictly-lower entries of U X are exactly 0 (both upper triangular)
+    double rho = (sqrt(nR) + g * sqrt(nA)) * 1.01 + 1e-300;
+    if (!(rho < 0.5)) return PFA_HOST;
+    double phi = g * nU * nX / ((1.0 - rho) * (1.0 - rho)) * 1.01;
+    if (!(phi < 0.25)) return PFA_HOST;
+    *phi_out = phi;
+    return 0;
+}

@@ -16,6 +16,16 @@ admissible p at h11 >= 20.  The bound E is therefore applied to the cone
 coordinates (p_f, p_k, x_a, x_b) with p_a = sum p_k + x_a and
 p_b = sum p_k + x_b, all in [1, E]; non-primitive vectors are dropped.  The
 resulting p is mapped to the conjugated basis by p' = U^{-T} p.
+
+Tree sizes.  On the lattice of the search, q(M) >= 2 p_f M_c^2 with equality
+reachable, so the cut M_0 = M_c >= 13 empties a tree unless
+338 p_f <= floor(D Q) (D the dilation), and the tree grows roughly like
+((D Q - 338 p_f) / (2 p_f))^((h11 - 2) / 2) beyond it: with p_f = 1 at
+h11 = 20 a single direction holds more than 10^7 nodes.  The workloads
+therefore draw p_f from [p_f_min, E] with p_f_min = max(1, floor(D_max Q /
+338)), D_max the largest dilation of the workload, which keeps every tree
+finite (empty, or one leftover band below 338) while the mix of empty and
+heavy trees stays uneven.
 """
 
 from __future__ import annotations
@@ -139,14 +149,16 @@ def _cone_to_p(h, pf, pk, xa, xb):
     return p
 
 
-def all_pvecs(inst: Instance, E: int) -> np.ndarray:
+def all_pvecs(inst: Instance, E: int, pf_min: int = 1) -> np.ndarray:
     """Every primitive admissible p (without the coni entry, in the data's
-    basis) whose cone coordinates all lie in [1, E]."""
+    basis) whose cone coordinates all lie in [1, E], p_f >= pf_min."""
     h = inst.h11
     nk = h - 4
     out = []
     for t in itertools.product(range(1, E + 1), repeat=nk + 3):
         pf, xa, xb = t[0], t[1], t[2]
+        if pf < pf_min:
+            continue
         pk = np.array(t[3:], dtype=np.int64)
         p = _cone_to_p(h, pf, pk, xa, xb)
         if math.gcd(*[int(x) for x in p]) != 1:
@@ -156,14 +168,20 @@ def all_pvecs(inst: Instance, E: int) -> np.ndarray:
     return P[:, 1:]
 
 
-def random_pvecs(inst: Instance, n: int, E: int, seed: int) -> np.ndarray:
+def pf_floor(Q: int, dmax: float) -> int:
+    """Smallest p_f kept by the workloads: floor(dmax Q / (2 * 13^2))."""
+    return max(1, int(math.floor(dmax * Q + 1e-7)) // 338)
+
+
+def random_pvecs(inst: Instance, n: int, E: int, seed: int, pf_min: int = 1) -> np.ndarray:
     """n distinct primitive admissible p with cone coordinates uniform in
-    [1, E] (seeded), in the data's basis without the coni entry."""
+    [1, E] and p_f in [pf_min, E] (seeded), in the data's basis without the
+    coni entry."""
     h = inst.h11
     rng = np.random.default_rng(seed)
     seen, out = set(), []
     tries = 0
-    total = E ** (h - 1)
+    total = (E - pf_min + 1) * E ** (h - 2)
     if n > total:
         raise ValueError("fewer admissible p-vectors than requested")
     while len(out) < n:
@@ -171,6 +189,7 @@ def random_pvecs(inst: Instance, n: int, E: int, seed: int) -> np.ndarray:
         if tries > 50 * n + 1000:
             raise RuntimeError("could not draw enough distinct p-vectors")
         t = rng.integers(1, E + 1, h - 1)
+        t[0] = rng.integers(pf_min, E + 1)
         key = tuple(int(x) for x in t)
         if key in seen:
             continue
@@ -206,7 +225,8 @@ def config_batch(n_per=4096, E=12, members=BATCH_MEMBERS, seed_offset=0):
     out = []
     for (h, h21, s) in members:
         inst = make_G(h, h21, s)
-        ps = random_pvecs(inst, n_per, E, seed=1000 + s + seed_offset)
+        ps = random_pvecs(inst, n_per, E, seed=1000 + s + seed_offset,
+                          pf_min=pf_floor(inst.Q, 4))
         dil = np.random.default_rng(2000 + s + seed_offset).integers(1, 5, len(ps))
         out.append((inst, ps, dil))
     return out
@@ -215,4 +235,4 @@ def config_batch(n_per=4096, E=12, members=BATCH_MEMBERS, seed_offset=0):
 def config_large_tree(n=2048, E=20):
     """G(30, 300, 6): n p-vectors with cone coordinates <= E, dilation 3."""
     inst = make_G(30, 300, 6)
-    return inst, random_pvecs(inst, n, E, seed=1006), 3
+    return inst, random_pvecs(inst, n, E, seed=1006, pf_min=pf_floor(inst.Q, 3)), 3

@@ -1,0 +1,65 @@
+"""HIP path parity: the plain and batched GPU searches against the golden
+kernel vectors and the oracle, bit-exact (points, order after the canonical
+sort, q), with node counts equal to the oracle's (same basis and pruning)."""
+
+import numpy as np
+import pytest
+
+from helpers import canon_points, kernel_problems
+from oracle import cpu
+from pfvs_amd.instances import make_G, pf_floor, random_pvecs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("mode", ["plain"])
+def test_gpu_kernel_batch_matches_golden(need_gpu, mode):
+    probs, exps = kernel_problems()
+    print(f"kernel batch: {len(probs)} problems, mode {mode}", flush=True)
+    res, st, _ = need_gpu.kernel_batch(probs, mode=mode)
+    n_ok = 0
+    for (pts, qs), exp, s, p in zip(res, exps, st, probs):
+        if s == 1:
+            assert max(abs(int(x)) for r in p["mat"] for x in r) > 2**53
+            continue
+        assert s == 0
+        assert pts.tolist() == exp["pts"] and qs.tolist() == exp["qs"]
+        n_ok += 1
+    assert n_ok >= 800
+
+
+CASES = [(6, 90, 2, 12, 4, 200), (8, 120, 1, 6, 3, 200), (10, 150, 3, 12, 4, 120),
+         (14, 200, 4, 12, 4, 40)]
+
+
+@pytest.mark.parametrize("mode", ["plain", "batched"])
+@pytest.mark.parametrize("h,h21,seed,E,dil,n", CASES)
+def test_gpu_pipeline_matches_oracle(need_gpu, mode, h, h21, seed, E, dil, n):
+    inst = make_G(h, h21, seed)
+    ps = random_pvecs(inst, n, E, seed=1000 + seed, pf_min=pf_floor(inst.Q, 4) if h > 6 else 1)
+    pf = np.hstack([np.zeros((len(ps), 1), np.int64), ps])
+    kap, Mb = inst.data.kappa_cob, inst.data.M_lattice()
+    M, K, q, pidx, st, cnt = cpu.coni_batch(kap, Mb, pf, inst.Q, cpu.qmax_of(inst.Q, dil))
+    geo = dict(kappa=kap, Mbasis=Mb, Q=inst.Q, dilation=dil, M0min=13)
+    print(f"h={h} n={n} dil={dil} mode={mode}: oracle {len(q)} points, {cnt[0]} nodes", flush=True)
+    r = need_gpu.run([geo], pf, np.zeros(len(pf), np.int32), mode=mode, filter=False)
+    assert np.all(st == 0) and np.all(r["pstat"] == 0)
+    assert canon_points(r["M"], r["Kn"], r["q"], r["pidx"], h) == canon_points(M, K, q, pidx, h)
+    assert int(r["stats"]["nodes"]) == cnt[0]
+
+
+def test_gpu_batched_small_budget_spills(need_gpu):
+    """Tiny stacks and budgets force donations to the frontier ring and many
+    passes; the result must not change."""
+    inst = make_G(10, 150, 3)
+    ps = random_pvecs(inst, 80, 12, seed=1003, pf_min=1)
+    pf = np.hstack([np.zeros((len(ps), 1), np.int64), ps])
+    geo = dict(kappa=inst.data.kappa_cob, Mbasis=inst.data.M_lattice(), Q=inst.Q, dilation=4,
+               M0min=13)
+    a = need_gpu.run([geo], pf, np.zeros(len(pf), np.int32), mode="plain", filter=False)
+    b = need_gpu.run([geo], pf, np.zeros(len(pf), np.int32), mode="batched", filter=False,
+                     budget=8, stack_cap=96, waves_per_cu=1)
+    assert b["stats"]["passes"] > 1
+    assert canon_points(a["M"], a["Kn"], a["q"], a["pidx"], 10) == \
+        canon_points(b["M"], b["Kn"], b["q"], b["pidx"], 10)
+    assert a["stats"]["nodes"] == b["stats"]["nodes"]
