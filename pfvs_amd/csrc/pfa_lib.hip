@@ -668,6 +668,18 @@ static DevCtx *ctx_for(int device)
     return g_ctx[device];
 }
 
+// stream and device facts, once per device (call with C->mu held)
+static void ctx_init(DevCtx *C, int device, char *errbuf)
+{
+    HIPCHK(hipSetDevice(device));
+    if (C->stream) return;
+    HIPCHK(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    C->cus = prop.multiProcessorCount;
+    C->lds_max = prop.sharedMemPerBlock;
+}
+
 static void *dev_get(DevCtx *C, int slot, size_t bytes, char *errbuf)
 {
     if ((int)C->bufs.size() <= slot) C->bufs.resize(slot + 1);
@@ -815,14 +827,7 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     auto t_wall = std::chrono::steady_clock::now();
     DevCtx *C = ctx_for(in->device);
     std::lock_guard<std::mutex> lk(C->mu);
-    HIPCHK(hipSetDevice(in->device));
-    if (!C->stream) {
-        HIPCHK(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
-        hipDeviceProp_t prop;
-        HIPCHK(hipGetDeviceProperties(&prop, in->device));
-        C->cus = prop.multiProcessorCount;
-        C->lds_max = prop.sharedMemPerBlock;
-    }
+    ctx_init(C, in->device, errbuf);
     hipStream_t s = C->stream;
     const int MH = PFA_MAX_H11, D = batch_dmax(in);
     const int64_t n_p = in->n_p;
@@ -1378,8 +1383,7 @@ extern "C" int pfa_kernel_batch(const pfa_kernel_input *in, pfa_kernel_output *o
         } else {
             DevCtx *C = ctx_for(in->device);
             std::lock_guard<std::mutex> lk(C->mu);
-            HIPCHK(hipSetDevice(in->device));
-            if (!C->stream) HIPCHK(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+            ctx_init(C, in->device, errbuf);
             hipStream_t s = C->stream;
             int64_t *d_recs = (int64_t *)dev_get(C, B_RECS, 8 * recs.size(), errbuf);
             HIPCHK(hipMemcpyAsync(d_recs, recs.data(), 8 * recs.size(), hipMemcpyHostToDevice, s));
