@@ -272,10 +272,16 @@ struct EntL {
 };
 static EntL ent_layout(int D) { return EntL{D, ((144 + 4 * D) + 15) & ~15}; }
 
+// Frontier ring shared by all wavefronts: ticket t lives in slot t % cap;
+// seq[slot] == t + 1 once the entry of ticket t is published.
 struct QueueArgs {
     char *ring;
+    unsigned long long *seq;
     unsigned long long cap;
-    unsigned long long *ctl;      // [0] head [1] tail [2] lim [3] head at pass start [4] overflow
+    // [0] head (consumer tickets) [1] tail (producer tickets) [2] outstanding
+    // work units [3] waiting wavefronts [4] consumed [5] error (1 ring full,
+    // 2 deadline)
+    unsigned long long *ctl;
 };
 
 struct PointArgs {
@@ -312,19 +318,15 @@ __global__ void k_roots(int64_t n, int64_t *recs, RecL L, QueueArgs Qa, EntL E, 
     for (int j = 0; j < PFA_DMAX; ++j) c[j] = 0;
     if (pfa_enter(P, c, nd)) { rec_status(rec) = PFA_I32; return; }
     atomicAdd(&C.c[0], 1ULL);
-    unsigned long long slot = atomicAdd(&Qa.ctl[1], 1ULL);
-    char *e = Qa.ring + (slot % Qa.cap) * E.bytes;
+    atomicAdd(&Qa.ctl[2], 1ULL);                        // one outstanding unit
+    unsigned long long t = atomicAdd(&Qa.ctl[1], 1ULL);
+    char *e = Qa.ring + (t % Qa.cap) * E.bytes;
     *(PfaNode *)e = nd;
     ((int32_t *)(e + 128))[0] = (int32_t)ip;
     int32_t *ec = (int32_t *)(e + 144);
     for (int j = 0; j < E.D; ++j) ec[j] = 0;
-}
-
-// snapshot the ring window of the next pass
-__global__ void k_snapshot(QueueArgs Qa)
-{
-    Qa.ctl[2] = Qa.ctl[1];
-    Qa.ctl[3] = Qa.ctl[0];
+    __threadfence();                                    // entry before its ticket
+    atomicExch(&Qa.seq[t % Qa.cap], t + 1);
 }
 
 // ----------------------------------------------------------- plain form
@@ -354,16 +356,28 @@ __global__ void k_plain(int64_t n, int64_t *recs, RecL L, PointArgs Pt, int D, C
 }
 
 // --------------------------------------------------------- batched form
+// One persistent launch.  Each wavefront owns a deque of nodes (stack top
+// for depth-first expansion, bottom for donation).  A step expands up to 64
+// candidates taken from the top nodes; children are pushed with a ballot;
+// leaves are written with one atomic per wavefront.  Wavefronts whose deque
+// runs dry take a ticket of the frontier ring and wait for its entry; busy
+// wavefronts that see waiting ones donate their bottom (shallowest, largest)
+// nodes.  `work` counts queued + in-process units, so waiting wavefronts
+// leave exactly when the whole search is done; a wall-clock deadline bounds
+// every wait and the launch itself.
 struct BatchArgs {
     int64_t *recs;
     RecL L;
     EntL E;
     QueueArgs Q;
     char *stacks;
-    int stack_cap, budget;
+    int stack_cap, donate_max;
     PointArgs Pt;
     CountArgs C;
+    unsigned long long deadline;      // wall-clock ticks (0: none)
 };
+
+#define PFA_CSTRIDE (PFA_DMAX + 1)        // odd stride: conflict-free LDS rows
 
 __device__ static inline int wave_scan_incl(int x, int lane)
 {
@@ -374,21 +388,48 @@ __device__ static inline int wave_scan_incl(int x, int lane)
     return x;
 }
 
-// spill entries [0, top) of the stack into the ring; returns 0 or 1 overflow
-__device__ static inline void stack_spill(const BatchArgs &a, char *stk, int top, int lane,
-                                          unsigned long long *s_base)
+__device__ static inline unsigned long long ld_agent(unsigned long long *p)
 {
-    if (lane == 0) *s_base = atomicAdd(&a.Q.ctl[1], (unsigned long long)top);
-    __syncthreads();
-    unsigned long long base = *s_base;
-    if (base + top > a.Q.ctl[3] + a.Q.cap) {
-        if (lane == 0) atomicExch((unsigned long long *)&a.Q.ctl[4], 1ULL);
-    } else {
-        for (int k = lane; k < top; k += 64)
-            entry_copy(a.Q.ring + ((base + k) % a.Q.cap) * a.E.bytes, stk + (size_t)k * a.E.bytes,
-                       a.E.bytes);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// publish stack entries [from, from + k) at tickets t .. t + k - 1 (lane l: entry l)
+__device__ static inline void ring_publish(const BatchArgs &a, const char *stk, int from, int k,
+                                           unsigned long long t, int lane)
+{
+    const int ES = a.E.bytes;
+    for (int l = lane; l < k; l += 64) {
+        unsigned long long tk = t + l, slot = tk % a.Q.cap;
+        entry_copy(a.Q.ring + slot * ES, stk + (size_t)(from + l) * ES, ES);
+        __threadfence();
+        atomicExch(&a.Q.seq[slot], tk + 1);
+    }
+}
+
+// hand k bottom entries to the ring (lane 0 reserves); returns k or 0
+__device__ static inline int donate(const BatchArgs &a, char *stk, int bot, int k, int forced,
+                                    int lane, unsigned long long *s_u64)
+{
+    if (lane == 0) {
+        unsigned long long tail = ld_agent(&a.Q.ctl[1]), used = ld_agent(&a.Q.ctl[4]);
+        unsigned long long room = a.Q.cap - (tail - used);
+        unsigned long long t = ~0ULL;
+        // keep room for every other wavefront's largest donation
+        if (room > (unsigned long long)k + 64ULL * 4096ULL) {
+            atomicAdd(&a.Q.ctl[2], (unsigned long long)k);          // units before tickets
+            t = atomicAdd(&a.Q.ctl[1], (unsigned long long)k);
+        } else if (forced) {
+            atomicExch(&a.Q.ctl[5], 1ULL);
+        }
+        s_u64[1] = t;
     }
     __syncthreads();
+    unsigned long long t = s_u64[1];
+    __syncthreads();
+    if (t == ~0ULL) return 0;
+    ring_publish(a, stk, bot, k, t, lane);
+    __syncthreads();
+    return k;
 }
 
 __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
@@ -397,27 +438,60 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
     const int ES = a.E.bytes, D = a.E.D;
     char *stk = a.stacks + (size_t)blockIdx.x * a.stack_cap * ES;
     __shared__ int s_incl[64];
-    __shared__ unsigned long long s_base;
+    __shared__ unsigned long long s_u64[2];
+    __shared__ int32_t s_c[64 * PFA_CSTRIDE];
+    int32_t *c = s_c + lane * PFA_CSTRIDE;
+    const unsigned long long t0 = wall_clock64();
     unsigned long long n_nodes = 0, n_cand = 0, n_leaf = 0;
-    const unsigned long long lim = a.Q.ctl[2];
-    int top = 0, steps = 0;
+    int bot = 0, top = 0, have_unit = 0, steps = 0;
     for (;;) {
-        if (top == 0) {
-            if (steps >= a.budget) break;
-            if (lane == 0) s_base = atomicAdd(&a.Q.ctl[0], 64ULL);
+        if (top == bot) {
+            if (lane == 0) {
+                if (have_unit) atomicAdd(&a.Q.ctl[2], ~0ULL);          // unit done
+                unsigned long long h = atomicAdd(&a.Q.ctl[0], 1ULL), slot = h % a.Q.cap;
+                atomicAdd(&a.Q.ctl[3], 1ULL);
+                int got = 0;
+                for (unsigned it = 0;; ++it) {
+                    if (ld_agent(&a.Q.seq[slot]) == h + 1) { got = 1; break; }
+                    if (ld_agent(&a.Q.ctl[2]) == 0) break;
+                    if (ld_agent(&a.Q.ctl[5])) break;
+                    if ((it & 63) == 0 && a.deadline && wall_clock64() - t0 > a.deadline) {
+                        atomicExch(&a.Q.ctl[5], 2ULL);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(4);
+                }
+                atomicAdd(&a.Q.ctl[3], ~0ULL);
+                s_u64[0] = got ? h : ~0ULL;
+            }
+            have_unit = 0;
             __syncthreads();
-            unsigned long long base = s_base;
+            const unsigned long long h = s_u64[0];
             __syncthreads();
-            long long avail = (long long)lim - (long long)base;
-            int n = avail <= 0 ? 0 : (avail > 64 ? 64 : (int)avail);
-            if (n == 0) break;
-            if (lane < n) entry_copy(stk + (size_t)lane * ES, a.Q.ring + ((base + lane) % a.Q.cap) * ES, ES);
-            top = n;
+            if (h == ~0ULL) break;
+            __threadfence();                               // acquire the entry
+            const int4 *src = (const int4 *)(a.Q.ring + (h % a.Q.cap) * ES);
+            int4 *dst = (int4 *)stk;
+            for (int k = lane; k < ES / 16; k += 64) dst[k] = src[k];
             __syncthreads();
+            if (lane == 0) atomicAdd(&a.Q.ctl[4], 1ULL);
+            bot = 0;
+            top = 1;
+            have_unit = 1;
         }
-        steps++;
+        if ((++steps & 255) == 0 && a.deadline) {
+            if (lane == 0) s_u64[0] = (wall_clock64() - t0 > a.deadline || ld_agent(&a.Q.ctl[5])) ? 1 : 0;
+            __syncthreads();
+            const int stop = (int)s_u64[0];
+            __syncthreads();
+            if (stop) {
+                if (lane == 0) atomicExch(&a.Q.ctl[5], 2ULL);
+                break;
+            }
+        }
         // gather up to 64 candidates from the top nodes (lane l: node top-1-l)
-        const int ntop = top < 64 ? top : 64;
+        const int live = top - bot;
+        const int ntop = live < 64 ? live : 64;
         int cur = 0, hi = -1;
         if (lane < ntop) {
             const PfaNode *nd = (const PfaNode *)(stk + (size_t)(top - 1 - lane) * ES);
@@ -425,15 +499,14 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
             hi = nd->hi;
         }
         long long cl = lane < ntop ? (long long)hi - cur + 1 : 0;
-        int cnt = cl <= 0 ? 0 : (cl > 64 ? 65 : (int)cl);     // 65 = "more than a step"
+        int cnt = cl <= 0 ? 0 : (cl > 64 ? 65 : (int)cl);
         int incl = wave_scan_incl(cnt, lane);
         s_incl[lane] = incl;
         __syncthreads();
         const int total = s_incl[63];
         const int take = total < 64 ? total : 64;
-        unsigned long long full = __ballot(lane < ntop && incl <= 64);
+        const unsigned long long full = __ballot(lane < ntop && incl <= 64);
         const int F = __popcll(full);
-        // my candidate: slot s = lane -> node j = first lane with incl > s
         int j = -1;
         if (lane < take) {
             int lo = 0, hi2 = ntop - 1;
@@ -444,30 +517,25 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
             j = lo;
         }
         PfaNode nd;
-        int32_t c[PFA_DMAX];
-        int32_t pidx = 0;
-        int32_t v = 0;
+        int32_t pidx = 0, v = 0;
         if (j >= 0) {
             const char *e = stk + (size_t)(top - 1 - j) * ES;
             nd = *(const PfaNode *)e;
             pidx = ((const int32_t *)(e + 128))[0];
             const int32_t *ec = (const int32_t *)(e + 144);
-            for (int k = 0; k < D; ++k) c[k] = ec[k];
-            int excl = (j > 0 ? s_incl[j - 1] : 0);
-            v = nd.cur + (lane - excl);
+            for (int k = nd.i + 1; k < D; ++k) c[k] = ec[k];
+            for (int k = 0; k <= nd.i; ++k) c[k] = 0;
+            v = nd.cur + (lane - (j > 0 ? s_incl[j - 1] : 0));
         }
-        // partially consumed node (lane F): its new cur
         int part_new_cur = 0;
         if (F < ntop && lane == 0) {
             const PfaNode *pn = (const PfaNode *)(stk + (size_t)(top - 1 - F) * ES);
-            int exclF = F > 0 ? s_incl[F - 1] : 0;
-            part_new_cur = pn->cur + (64 - exclF);
+            part_new_cur = pn->cur + (64 - (F > 0 ? s_incl[F - 1] : 0));
         }
         __syncthreads();                     // every parent read before any write
         int newtop = top - F;
         if (F < ntop && lane == 0)
             ((PfaNode *)(stk + (size_t)(top - 1 - F) * ES))->cur = part_new_cur;
-        // test the candidate; children and leaves
         int has_child = 0, emit = 0;
         int64_t q = 0;
         PfaNode ch;
@@ -492,13 +560,12 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
                 }
             }
         }
-        // leaves: one atomic per wavefront
-        unsigned long long em = __ballot(emit);
+        const unsigned long long em = __ballot(emit);
         if (em) {
-            if (lane == 0) s_base = atomicAdd(a.Pt.count, (unsigned long long)__popcll(em));
+            if (lane == 0) s_u64[0] = atomicAdd(a.Pt.count, (unsigned long long)__popcll(em));
             __syncthreads();
             if (emit) {
-                unsigned long long k = s_base + __popcll(em & ((1ULL << lane) - 1));
+                unsigned long long k = s_u64[0] + __popcll(em & ((1ULL << lane) - 1));
                 if (k < a.Pt.cap) {
                     a.Pt.pidx[k] = pidx;
                     a.Pt.q[k] = q;
@@ -509,13 +576,32 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
             }
             __syncthreads();
         }
-        // children: make room, then push with a ballot
-        unsigned long long cm = __ballot(has_child);
+        const unsigned long long cm = __ballot(has_child);
         const int nch = __popcll(cm);
         if (newtop + nch > a.stack_cap) {
             __syncthreads();
-            stack_spill(a, stk, newtop, lane, &s_base);
-            newtop = 0;
+            if (bot > 0) {
+                // compact [bot, newtop) to 0 in chunks of bot entries
+                for (int base = bot; base < newtop; base += bot) {
+                    int n = newtop - base < bot ? newtop - base : bot;
+                    for (int k = lane; k < n; k += 64)
+                        entry_copy(stk + (size_t)(base - bot + k) * ES, stk + (size_t)(base + k) * ES, ES);
+                    __syncthreads();
+                }
+                newtop -= bot;
+                bot = 0;
+            }
+            if (newtop + nch > a.stack_cap) {
+                int k = newtop / 2;
+                if (!donate(a, stk, 0, k, 1, lane, s_u64)) break;     // ring full: error set
+                for (int base = k; base < newtop; base += k) {
+                    int n = newtop - base < k ? newtop - base : k;
+                    for (int t = lane; t < n; t += 64)
+                        entry_copy(stk + (size_t)(base - k + t) * ES, stk + (size_t)(base + t) * ES, ES);
+                    __syncthreads();
+                }
+                newtop -= k;
+            }
         }
         if (has_child) {
             int slot = newtop + __popcll(cm & ((1ULL << lane) - 1));
@@ -523,14 +609,22 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
             *(PfaNode *)e = ch;
             ((int32_t *)(e + 128))[0] = pidx;
             int32_t *ec = (int32_t *)(e + 144);
-            for (int k = 0; k < D; ++k) ec[k] = k < ch.i + 1 ? 0 : c[k];
+            for (int k = ch.i + 1; k < D; ++k) ec[k] = c[k];
         }
         top = newtop + nch;
         __syncthreads();                     // writes visible to the next step
-        if (steps >= a.budget && top > 0) {
-            stack_spill(a, stk, top, lane, &s_base);
-            top = 0;
-            break;
+        // feed waiting wavefronts from the bottom of the deque
+        if (top - bot >= 2) {
+            if (lane == 0) s_u64[0] = ld_agent(&a.Q.ctl[3]);
+            __syncthreads();
+            unsigned long long w = s_u64[0];
+            __syncthreads();
+            if (w > 0) {
+                int k = (top - bot) / 2;
+                if ((unsigned long long)k > w) k = (int)w;
+                if (k > a.donate_max) k = a.donate_max;
+                bot += donate(a, stk, bot, k, 0, lane, s_u64);
+            }
         }
     }
     if (n_nodes) atomicAdd(&a.C.c[0], n_nodes);
@@ -697,7 +791,7 @@ static void *dev_get(DevCtx *C, int slot, size_t bytes, char *errbuf)
 
 enum {
     B_GEO_H, B_GEO_Q, B_GEO_QMAX, B_GEO_LIN, B_KAPPA, B_MBASIS, B_KOFS, B_MOFS, B_PS, B_PGEO,
-    B_RECS, B_RING, B_STACKS, B_CTL, B_PT_PIDX, B_PT_Q, B_PT_C, B_CNT, B_M, B_KN, B_NEXP, B_BAD,
+    B_RECS, B_RING, B_SEQ, B_STACKS, B_CTL, B_PT_PIDX, B_PT_Q, B_PT_C, B_CNT, B_M, B_KN, B_NEXP, B_BAD,
     B_PRIMES, B_NW, B_WW, B_NONSING, B_NBUF
 };
 
@@ -876,11 +970,18 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     // points buffer: grow and rerun the search on overflow
     unsigned long long pcap = std::max<int64_t>(1 << 16, 16 * n_p);
     const int waves_per_cu = in->waves_per_cu > 0 ? in->waves_per_cu : 8;
-    const int budget = in->budget > 0 ? in->budget : 4096;
-    const int stack_cap = in->stack_cap > 0 ? in->stack_cap : 64 * (D + 2);
+    const int donate_max = in->budget > 0 ? in->budget : 16;
+    const int stack_cap = in->stack_cap > 0 ? std::max(in->stack_cap, 128) : 64 * (D + 2);
     const int nwaves = std::max(1, C->cus * waves_per_cu);
-    unsigned long long ring_cap = (unsigned long long)std::max<int64_t>(2 * n_p + 64,
-                                  (int64_t)nwaves * stack_cap * 2);
+    // room for every root plus the donation reserve kept by donate()
+    unsigned long long ring_cap = (unsigned long long)std::max<int64_t>(2 * n_p + 64, 1 << 20) +
+                                  2 * 64ULL * 4096ULL;
+    int wclk_khz = 0;
+    HIPCHK(hipDeviceGetAttribute(&wclk_khz, hipDeviceAttributeWallClockRate, in->device));
+    const char *dl_env = getenv("PFVS_SEARCH_DEADLINE_S");
+    const double deadline_s = dl_env ? atof(dl_env) : 120.0;
+    const unsigned long long deadline =
+        (unsigned long long)(deadline_s * 1000.0 * (double)std::max(wclk_khz, 1));
     double ms_roots = 0, ms_search = 0;
     int passes = 0, launches = 0;
     unsigned long long npts = 0, queue_peak = 0;
@@ -925,42 +1026,37 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
             passes = 1;
         } else {
             char *ring = (char *)dev_get(C, B_RING, (size_t)ring_cap * E.bytes, errbuf);
+            unsigned long long *seq = (unsigned long long *)dev_get(C, B_SEQ, 8 * ring_cap, errbuf);
             char *stacks = (char *)dev_get(C, B_STACKS, (size_t)nwaves * stack_cap * E.bytes, errbuf);
-            QueueArgs Qa{ring, ring_cap, d_ctl};
+            HIPCHK(hipMemsetAsync(seq, 0, 8 * ring_cap, s));
+            QueueArgs Qa{ring, seq, ring_cap, d_ctl};
             HIPCHK(hipEventRecord(ev[2], s));
             if (n_p > 0) {
                 hipLaunchKernelGGL(k_roots, dim3((n_p + 255) / 256), dim3(256), 0, s, n_p, d_recs, L, Qa, E, Cn);
                 HIPCHK(hipGetLastError());
             }
             HIPCHK(hipEventRecord(ev[3], s));
-            BatchArgs A{d_recs, L, E, Qa, stacks, stack_cap, budget, Pt, Cn};
-            unsigned long long hctl[8];
-            for (;;) {
-                hipLaunchKernelGGL(k_snapshot, dim3(1), dim3(1), 0, s, Qa);
-                hipLaunchKernelGGL(k_batched, dim3(nwaves), dim3(64), 0, s, A);
-                HIPCHK(hipGetLastError());
-                launches++;
-                passes++;
-                HIPCHK(hipMemcpyAsync(hctl, d_ctl, 64, hipMemcpyDeviceToHost, s));
-                HIPCHK(hipStreamSynchronize(s));
-                unsigned long long head = std::min(hctl[0], hctl[2]);
-                unsigned long long live = hctl[1] - head;
-                queue_peak = std::max(queue_peak, live);
-                if (hctl[4]) {
-                    snprintf(errbuf, 256, "frontier queue overflow (capacity %llu)", ring_cap);
-                    return 1;
-                }
-                if (live == 0) break;
-                // consumed head may overshoot lim by a grab: clamp it
-                if (hctl[0] > hctl[2]) {
-                    HIPCHK(hipMemcpyAsync(d_ctl, &hctl[2], 8, hipMemcpyHostToDevice, s));
-                }
-                if (passes > 100000) {
-                    snprintf(errbuf, 256, "batched search did not finish");
-                    return 1;
-                }
-            }
+            BatchArgs A{d_recs, L, E, Qa, stacks, stack_cap, donate_max, Pt, Cn, deadline};
+            hipLaunchKernelGGL(k_batched, dim3(nwaves), dim3(64), 0, s, A);
+            HIPCHK(hipGetLastError());
+            launches = passes = 1;
             HIPCHK(hipEventRecord(ev[4], s));
+            unsigned long long hctl[8];
+            HIPCHK(hipMemcpyAsync(hctl, d_ctl, 64, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            queue_peak = hctl[1];              // tickets issued: roots + donations
+            if (hctl[5] == 1) {
+                snprintf(errbuf, 256, "frontier ring full (capacity %llu)", ring_cap);
+                return 1;
+            }
+            if (hctl[5] == 2) {
+                snprintf(errbuf, 256, "batched search exceeded its %.0f s deadline", deadline_s);
+                return 1;
+            }
+            if (hctl[2] != 0) {
+                snprintf(errbuf, 256, "batched search ended with %lld units open", (long long)hctl[2]);
+                return 1;
+            }
         }
         HIPCHK(hipMemcpyAsync(&npts, d_ctl + 6, 8, hipMemcpyDeviceToHost, s));
         int ovf = 0;

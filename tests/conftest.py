@@ -4,6 +4,8 @@ import sys
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+# a batched search that has not finished by then stops with an error
+os.environ.setdefault("PFVS_SEARCH_DEADLINE_S", "60")
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
