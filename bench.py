@@ -108,6 +108,9 @@ def main():
                    "prep_ms": float(np.mean([s["ms_prep"] for s in st_b])),
                    "post_ms": float(np.mean([s["ms_post"] for s in st_b])),
                    "passes": st_b[-1]["passes"], "host_p": st_b[-1]["host_p"],
+                   "cand": st_b[-1]["cand"], "steps": st_b[-1]["steps"],
+                   "wait_frac": st_b[-1]["wait_cycles"] / max(st_b[-1]["wave_cycles"], 1),
+                   "donated": st_b[-1]["donated"], "fetches": st_b[-1]["fetches"],
                    "coniPFVs_per_s": st_b[-1]["expansions"] / (ms_dev * 1e-3),
                    "wall_ms": wall_b * 1e3},
     }

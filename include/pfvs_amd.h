@@ -104,7 +104,8 @@ enum {
     PFA_ST_NODES = 0, PFA_ST_CAND, PFA_ST_LEAF, PFA_ST_POINTS, PFA_ST_PASSES,
     PFA_ST_MS_PREP, PFA_ST_MS_ROOTS, PFA_ST_MS_SEARCH, PFA_ST_MS_POST, PFA_ST_MS_TOTAL,
     PFA_ST_SEARCH_LAUNCHES, PFA_ST_HOST_P, PFA_ST_QUEUE_PEAK, PFA_ST_MS_WALL,
-    PFA_ST_KEPT, PFA_ST_EXPANSIONS, PFA_ST_NONSING, PFA_ST_UNDECIDED, PFA_ST_N
+    PFA_ST_KEPT, PFA_ST_EXPANSIONS, PFA_ST_NONSING, PFA_ST_UNDECIDED,
+    PFA_ST_STEPS, PFA_ST_WAIT_CYC, PFA_ST_WAVE_CYC, PFA_ST_DONATED, PFA_ST_FETCHES, PFA_ST_N
 };
 
 typedef struct {

@@ -245,3 +245,49 @@ def _orth(v):
         U[0] = [s * x + t * y for x, y in zip(r0, rk)]
         U[k] = [(-b // g) * x + (a // g) * y for x, y in zip(r0, rk)]
     return U[1:]
+
+
+def pfvs_from_points(points, kappa, h, Q, M0min):
+    """(K, M, key) rows from lattice points, per point with Python ints:
+    the post-processing of pfvs/coni.py:275-412 restated one point at a
+    time.  points: iterable of (key, M, Kn, q).  Output order: key, then g,
+    then the point's (M, Kn) order, then K0; det((kappa M)[1:, 1:]) == 0
+    removes a row."""
+    pts = []
+    for key, M, Kn, q in points:
+        M = [int(x) for x in M]
+        Kn = [int(x) for x in Kn]
+        q = int(q)
+        kg = 0
+        for x in Kn[1:]:
+            kg = math.gcd(kg, x)
+        kg = max(kg, 1)
+        if not q < Q * kg:
+            continue
+        pts.append((int(key), M, Kn, q, kg))
+    pts.sort(key=lambda t: (t[0], t[1], t[2]))
+    rows = []
+    for rank, (key, M, Kn, q, kg) in enumerate(pts):
+        M0 = M[0]
+        raw = (q + M0 * Kn[0]) // kg
+        gmax = (Q * kg - 1) // q
+        for g in range(1, gmax + 1):
+            Qp = g * raw
+            lo = -((Q - Qp) // M0)
+            up = (Qp - Q) // M0
+            up = min(up, -((-Kn[0] * g) // kg) - 1)
+            for K0 in range(lo, up + 1):
+                K = [K0] + [g * x // kg for x in Kn[1:]]
+                rows.append((key, g, rank, K0, K, M))
+    rows.sort(key=lambda t: (t[0], t[1], t[2], t[3]))
+    out = []
+    singular_cache = {}
+    for key, g, rank, K0, K, M in rows:
+        tM = tuple(M)
+        if tM not in singular_cache:
+            N = [[sum(int(kappa[i][j][k]) * M[k] for k in range(h)) for j in range(1, h)]
+                 for i in range(1, h)]
+            singular_cache[tM] = is_singular(N)
+        if not singular_cache[tM]:
+            out.append((K, M, key))
+    return out

@@ -1,0 +1,12 @@
+"""pfvs_amd: the coniZpM search of pfvs on MI355X (gfx950).
+
+The public names mirror the reference package's coniZpM path
+(pfvs/__init__.py): coniZpM, CYData, IncompleteSearchError, the gpu module
+and the p-vector enumeration.
+"""
+
+from . import gpu  # noqa: F401
+from .coni import coniZpM  # noqa: F401
+from .cydata import CYData  # noqa: F401
+from .pvectors import box_enum, pvecs  # noqa: F401
+from .util import IncompleteSearchError  # noqa: F401

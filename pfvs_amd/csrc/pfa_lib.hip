@@ -278,9 +278,10 @@ struct QueueArgs {
     char *ring;
     unsigned long long *seq;
     unsigned long long cap;
-    // [0] head (consumer tickets) [1] tail (producer tickets) [2] outstanding
-    // work units [3] waiting wavefronts [4] consumed [5] error (1 ring full,
-    // 2 deadline)
+    // words 16 k apart (one 128-byte line each): k = 0 head (consumer
+    // tickets), 1 tail (producer tickets), 2 outstanding work units,
+    // 3 waiting wavefronts, 4 consumed, 5 error (1 ring full, 2 deadline),
+    // 6 point count, 8 point-buffer overflow
     unsigned long long *ctl;
 };
 
@@ -318,8 +319,8 @@ __global__ void k_roots(int64_t n, int64_t *recs, RecL L, QueueArgs Qa, EntL E, 
     for (int j = 0; j < PFA_DMAX; ++j) c[j] = 0;
     if (pfa_enter(P, c, nd)) { rec_status(rec) = PFA_I32; return; }
     atomicAdd(&C.c[0], 1ULL);
-    atomicAdd(&Qa.ctl[2], 1ULL);                        // one outstanding unit
-    unsigned long long t = atomicAdd(&Qa.ctl[1], 1ULL);
+    atomicAdd(&Qa.ctl[32], 1ULL);                        // one outstanding unit
+    unsigned long long t = atomicAdd(&Qa.ctl[16], 1ULL);
     char *e = Qa.ring + (t % Qa.cap) * E.bytes;
     *(PfaNode *)e = nd;
     ((int32_t *)(e + 128))[0] = (int32_t)ip;
@@ -371,7 +372,7 @@ struct BatchArgs {
     EntL E;
     QueueArgs Q;
     char *stacks;
-    int stack_cap, donate_max;
+    int stack_cap, donate_max, donate_min, pt_chunk;
     PointArgs Pt;
     CountArgs C;
     unsigned long long deadline;      // wall-clock ticks (0: none)
@@ -411,15 +412,15 @@ __device__ static inline int donate(const BatchArgs &a, char *stk, int bot, int 
                                     int lane, unsigned long long *s_u64)
 {
     if (lane == 0) {
-        unsigned long long tail = ld_agent(&a.Q.ctl[1]), used = ld_agent(&a.Q.ctl[4]);
+        unsigned long long tail = ld_agent(&a.Q.ctl[16]), used = ld_agent(&a.Q.ctl[64]);
         unsigned long long room = a.Q.cap - (tail - used);
         unsigned long long t = ~0ULL;
         // keep room for every other wavefront's largest donation
         if (room > (unsigned long long)k + 64ULL * 4096ULL) {
-            atomicAdd(&a.Q.ctl[2], (unsigned long long)k);          // units before tickets
-            t = atomicAdd(&a.Q.ctl[1], (unsigned long long)k);
+            atomicAdd(&a.Q.ctl[32], (unsigned long long)k);          // units before tickets
+            t = atomicAdd(&a.Q.ctl[16], (unsigned long long)k);
         } else if (forced) {
-            atomicExch(&a.Q.ctl[5], 1ULL);
+            atomicExch(&a.Q.ctl[80], 1ULL);
         }
         s_u64[1] = t;
     }
@@ -442,26 +443,34 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
     __shared__ int32_t s_c[64 * PFA_CSTRIDE];
     int32_t *c = s_c + lane * PFA_CSTRIDE;
     const unsigned long long t0 = wall_clock64();
-    unsigned long long n_nodes = 0, n_cand = 0, n_leaf = 0;
+    const unsigned long long c0 = clock64();
+    unsigned long long n_nodes = 0, n_cand = 0, n_leaf = 0, n_wait = 0, n_fetch = 0, n_don = 0;
     int bot = 0, top = 0, have_unit = 0, steps = 0;
+    unsigned long long waiting = 0;
+    unsigned long long pt_next = 0, pt_end = 0;        // this wavefront's point slots
     for (;;) {
         if (top == bot) {
             if (lane == 0) {
-                if (have_unit) atomicAdd(&a.Q.ctl[2], ~0ULL);          // unit done
+                if (have_unit) atomicAdd(&a.Q.ctl[32], ~0ULL);          // unit done
                 unsigned long long h = atomicAdd(&a.Q.ctl[0], 1ULL), slot = h % a.Q.cap;
-                atomicAdd(&a.Q.ctl[3], 1ULL);
+                atomicAdd(&a.Q.ctl[48], 1ULL);
                 int got = 0;
+                const unsigned long long w0 = clock64();
                 for (unsigned it = 0;; ++it) {
                     if (ld_agent(&a.Q.seq[slot]) == h + 1) { got = 1; break; }
-                    if (ld_agent(&a.Q.ctl[2]) == 0) break;
-                    if (ld_agent(&a.Q.ctl[5])) break;
-                    if ((it & 63) == 0 && a.deadline && wall_clock64() - t0 > a.deadline) {
-                        atomicExch(&a.Q.ctl[5], 2ULL);
-                        break;
+                    if ((it & 15) == 15) {
+                        if (ld_agent(&a.Q.ctl[32]) == 0) break;
+                        if (ld_agent(&a.Q.ctl[80])) break;
+                        if (a.deadline && wall_clock64() - t0 > a.deadline) {
+                            atomicExch(&a.Q.ctl[80], 2ULL);
+                            break;
+                        }
                     }
-                    __builtin_amdgcn_s_sleep(4);
+                    __builtin_amdgcn_s_sleep(8);
                 }
-                atomicAdd(&a.Q.ctl[3], ~0ULL);
+                n_wait += clock64() - w0;
+                n_fetch++;
+                atomicAdd(&a.Q.ctl[48], ~0ULL);
                 s_u64[0] = got ? h : ~0ULL;
             }
             have_unit = 0;
@@ -474,18 +483,18 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
             int4 *dst = (int4 *)stk;
             for (int k = lane; k < ES / 16; k += 64) dst[k] = src[k];
             __syncthreads();
-            if (lane == 0) atomicAdd(&a.Q.ctl[4], 1ULL);
+            if (lane == 0) atomicAdd(&a.Q.ctl[64], 1ULL);
             bot = 0;
             top = 1;
             have_unit = 1;
         }
         if ((++steps & 255) == 0 && a.deadline) {
-            if (lane == 0) s_u64[0] = (wall_clock64() - t0 > a.deadline || ld_agent(&a.Q.ctl[5])) ? 1 : 0;
+            if (lane == 0) s_u64[0] = (wall_clock64() - t0 > a.deadline || ld_agent(&a.Q.ctl[80])) ? 1 : 0;
             __syncthreads();
             const int stop = (int)s_u64[0];
             __syncthreads();
             if (stop) {
-                if (lane == 0) atomicExch(&a.Q.ctl[5], 2ULL);
+                if (lane == 0) atomicExch(&a.Q.ctl[80], 2ULL);
                 break;
             }
         }
@@ -562,10 +571,20 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
         }
         const unsigned long long em = __ballot(emit);
         if (em) {
-            if (lane == 0) s_u64[0] = atomicAdd(a.Pt.count, (unsigned long long)__popcll(em));
-            __syncthreads();
+            const int ne = __popcll(em);
+            if (pt_next + ne > pt_end) {
+                // a new chunk of point slots; the rest of the old one is
+                // marked empty (pidx -1)
+                if (lane == 0) s_u64[0] = atomicAdd(a.Pt.count, (unsigned long long)a.pt_chunk);
+                __syncthreads();
+                for (unsigned long long k = pt_next + lane; k < pt_end; k += 64)
+                    if (k < a.Pt.cap) a.Pt.pidx[k] = -1;
+                pt_next = s_u64[0];
+                pt_end = pt_next + a.pt_chunk;
+                __syncthreads();
+            }
             if (emit) {
-                unsigned long long k = s_u64[0] + __popcll(em & ((1ULL << lane) - 1));
+                unsigned long long k = pt_next + __popcll(em & ((1ULL << lane) - 1));
                 if (k < a.Pt.cap) {
                     a.Pt.pidx[k] = pidx;
                     a.Pt.q[k] = q;
@@ -574,7 +593,7 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
                     *a.Pt.overflow = 1;
                 }
             }
-            __syncthreads();
+            pt_next += ne;
         }
         const unsigned long long cm = __ballot(has_child);
         const int nch = __popcll(cm);
@@ -613,23 +632,40 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
         }
         top = newtop + nch;
         __syncthreads();                     // writes visible to the next step
-        // feed waiting wavefronts from the bottom of the deque
-        if (top - bot >= 2) {
-            if (lane == 0) s_u64[0] = ld_agent(&a.Q.ctl[3]);
+        // feed waiting wavefronts from the bottom of the deque (the shallowest
+        // nodes: the largest subtrees), only from deques deep enough to keep
+        // this wavefront's own steps full; the waiting count is refreshed
+        // every 16 steps
+        if ((steps & 15) == 0) {
+            if (lane == 0) s_u64[0] = ld_agent(&a.Q.ctl[48]);
             __syncthreads();
-            unsigned long long w = s_u64[0];
+            waiting = s_u64[0];
             __syncthreads();
-            if (w > 0) {
-                int k = (top - bot) / 2;
-                if ((unsigned long long)k > w) k = (int)w;
+        }
+        if (waiting > 0 && top - bot >= a.donate_min) {
+            {
+                int k = (top - bot) / 4;
+                if ((unsigned long long)k > waiting) k = (int)waiting;
                 if (k > a.donate_max) k = a.donate_max;
-                bot += donate(a, stk, bot, k, 0, lane, s_u64);
+                waiting -= k;
+                int got = donate(a, stk, bot, k, 0, lane, s_u64);
+                bot += got;
+                n_don += got;
             }
         }
     }
+    for (unsigned long long k = pt_next + lane; k < pt_end; k += 64)
+        if (k < a.Pt.cap) a.Pt.pidx[k] = -1;
     if (n_nodes) atomicAdd(&a.C.c[0], n_nodes);
     if (n_cand) atomicAdd(&a.C.c[1], n_cand);
     if (n_leaf) atomicAdd(&a.C.c[2], n_leaf);
+    if (lane == 0) {
+        atomicAdd(&a.C.c[3], (unsigned long long)steps);
+        atomicAdd(&a.C.c[4], n_wait);
+        atomicAdd(&a.C.c[5], clock64() - c0);
+        atomicAdd(&a.C.c[6], n_don);
+        atomicAdd(&a.C.c[7], n_fetch);
+    }
 }
 
 // ------------------------------------------------------------ post
@@ -650,6 +686,10 @@ struct PostArgs {
 PFA_HD static inline void post_one(const PostArgs &a, int64_t k)
 {
     const int32_t ip = a.pidx[k];
+    if (ip < 0) {                                  // an unused point slot
+        if (a.filter) a.nexp[k] = -1;
+        return;
+    }
     const int64_t *rec = a.recs + (int64_t)ip * a.L.words;
     const int32_t *ri = (const int32_t *)rec;
     const int d = ri[1], h = d + 1;
@@ -688,23 +728,28 @@ struct DetArgs {
     const int64_t *recs;
     RecL L;
     GeoArgs G;
-    const uint32_t *primes;
+    const PfaModP *primes;
     int np;
-    int64_t *Nw;                  // per thread (MH-1)^2 int64
-    uint32_t *Ww;                 // per thread (MH-1)^2 uint32
+    int64_t *Nw;                  // (HS-1)^2 int64 per thread, interleaved
+    uint32_t *Ww;                 // (HS-1)^2 uint32 per thread, interleaved
     int8_t *nonsing;
 };
 
-// N = (kappa M)[1:, 1:] for a point; 0 or 1 if an entry leaves int64
-PFA_HD static inline int kappa_M(const int64_t *kappa, int h, const int64_t *M, int64_t *N)
+// N = (kappa M)[1:, 1:] for a point (element e at N[e * stride]); 0, or 1 if
+// an entry leaves int64.  kappa is symmetric: N is too.
+PFA_HD static inline int kappa_M(const int64_t *kappa, int h, const int64_t *M, int64_t *N,
+                                 int64_t stride)
 {
     const int n = h - 1;
     for (int i = 1; i < h; ++i)
-        for (int j = 1; j < h; ++j) {
+        for (int j = i; j < h; ++j) {
             pfa_i128 s = 0;
-            for (int k = 0; k < h; ++k) s += (pfa_i128)kappa[((size_t)i * h + j) * h + k] * M[k];
+            const int64_t *kij = kappa + ((size_t)i * h + j) * h;
+            for (int k = 0; k < h; ++k)
+                if (kij[k]) s += (pfa_i128)kij[k] * M[k];
             if (!pfa_fits64(s)) return 1;
-            N[(i - 1) * n + (j - 1)] = (int64_t)s;
+            N[((i - 1) * n + (j - 1)) * stride] = (int64_t)s;
+            N[((j - 1) * n + (i - 1)) * stride] = (int64_t)s;
         }
     return 0;
 }
@@ -713,16 +758,15 @@ __global__ void k_det(DetArgs a)
 {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nt = (int64_t)gridDim.x * blockDim.x;
-    const int S = (a.MH - 1) * (a.MH - 1);
-    int64_t *N = a.Nw + tid * S;
-    uint32_t *w = a.Ww + tid * S;
+    int64_t *N = a.Nw + tid;                       // element e at N[e * nt]
+    uint32_t *w = a.Ww + tid;
     for (int64_t k = tid; k < a.n; k += nt) {
-        if (a.nexp[k] <= 0) { a.nonsing[k] = 0; continue; }
+        if (a.pidx[k] < 0 || a.nexp[k] <= 0) { a.nonsing[k] = 0; continue; }
         const int64_t *rec = a.recs + (int64_t)a.pidx[k] * a.L.words;
         const int32_t *ri = (const int32_t *)rec;
         const int h = ri[24], geo = ri[26];
-        if (kappa_M(a.G.kappa + a.G.kofs[geo], h, a.M + k * a.MH, N)) { a.nonsing[k] = 2; continue; }
-        a.nonsing[k] = (int8_t)pfa_nonsingular(N, h - 1, a.primes, a.np, w);
+        if (kappa_M(a.G.kappa + a.G.kofs[geo], h, a.M + k * a.MH, N, nt)) { a.nonsing[k] = 2; continue; }
+        a.nonsing[k] = (int8_t)pfa_nonsingular(N, h - 1, a.primes, a.np, w, nt);
     }
 }
 
@@ -789,20 +833,23 @@ static void *dev_get(DevCtx *C, int slot, size_t bytes, char *errbuf)
     return b.p;
 }
 
+#define CTL_BYTES (16 * 8 * 16)
+
 enum {
     B_GEO_H, B_GEO_Q, B_GEO_QMAX, B_GEO_LIN, B_KAPPA, B_MBASIS, B_KOFS, B_MOFS, B_PS, B_PGEO,
     B_RECS, B_RING, B_SEQ, B_STACKS, B_CTL, B_PT_PIDX, B_PT_Q, B_PT_C, B_CNT, B_M, B_KN, B_NEXP, B_BAD,
     B_PRIMES, B_NW, B_WW, B_NONSING, B_NBUF
 };
 
-static std::vector<uint32_t> primes31(int k)
+// the k largest primes below 2^31 with their Barrett constants
+static std::vector<PfaModP> primes31(int k)
 {
-    std::vector<uint32_t> out;
+    std::vector<PfaModP> out;
     for (uint32_t x = 2147483647u; (int)out.size() < k; x -= 2) {
         bool pr = true;
         for (uint32_t f = 3; (uint64_t)f * f <= x; f += 2)
             if (x % f == 0) { pr = false; break; }
-        if (pr) out.push_back(x);
+        if (pr) out.push_back(PfaModP{x, (uint64_t)(((pfa_u128)1 << 64) / x)});
     }
     return out;
 }
@@ -874,9 +921,11 @@ static int finish_run(const pfa_run_input *in, pfa_run_output *out, int64_t npts
     std::vector<int64_t> idx;
     idx.reserve(npts);
     for (int64_t k = 0; k < npts; ++k)
-        if (out->pstat[pidx32[k]] == 0) idx.push_back(k);
+        if (pidx32[k] >= 0 && out->pstat[pidx32[k]] == 0) idx.push_back(k);
     std::vector<int64_t> pidx64(npts);
     for (int64_t k = 0; k < npts; ++k) pidx64[k] = pidx32[k];
+    int64_t n_valid = 0;
+    for (int64_t k = 0; k < npts; ++k) n_valid += pidx32[k] >= 0;
     if (in->sort) canonical_order(idx, pidx64.data(), M, Kn, HS);
     const int64_t n = (int64_t)idx.size();
     out->n = n;
@@ -905,7 +954,7 @@ static int finish_run(const pfa_run_input *in, pfa_run_output *out, int64_t npts
         }
     }
     for (int64_t i = 0; i < n_p; ++i) nhost += out->pstat[i] == 1;
-    out->stats[PFA_ST_POINTS] = (double)n;
+    out->stats[PFA_ST_POINTS] = (double)n_valid;
     out->stats[PFA_ST_KEPT] = (double)kept;
     out->stats[PFA_ST_EXPANSIONS] = (double)nex;
     out->stats[PFA_ST_NONSING] = (double)nns;
@@ -970,7 +1019,9 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     // points buffer: grow and rerun the search on overflow
     unsigned long long pcap = std::max<int64_t>(1 << 16, 16 * n_p);
     const int waves_per_cu = in->waves_per_cu > 0 ? in->waves_per_cu : 8;
-    const int donate_max = in->budget > 0 ? in->budget : 16;
+    const int donate_max = in->budget > 0 ? in->budget : 32;
+    const int donate_min = 64;               // deque depth before a wavefront donates
+    const int pt_chunk = 256;                // point slots reserved per atomic
     const int stack_cap = in->stack_cap > 0 ? std::max(in->stack_cap, 128) : 64 * (D + 2);
     const int nwaves = std::max(1, C->cus * waves_per_cu);
     // room for every root plus the donation reserve kept by donate()
@@ -994,9 +1045,9 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
         d_pidx = (int32_t *)dev_get(C, B_PT_PIDX, 4 * pcap, errbuf);
         d_q = (int64_t *)dev_get(C, B_PT_Q, 8 * pcap, errbuf);
         d_c = (int32_t *)dev_get(C, B_PT_C, 4 * pcap * D, errbuf);
-        unsigned long long *d_ctl = (unsigned long long *)dev_get(C, B_CTL, 128, errbuf);
-        d_ovf = (int *)(d_ctl + 8);
-        HIPCHK(hipMemsetAsync(d_ctl, 0, 128, s));
+        unsigned long long *d_ctl = (unsigned long long *)dev_get(C, B_CTL, CTL_BYTES, errbuf);
+        d_ovf = (int *)(d_ctl + 16 * 8);
+        HIPCHK(hipMemsetAsync(d_ctl, 0, CTL_BYTES, s));
         HIPCHK(hipMemsetAsync(d_cnt, 0, 64, s));
         if (n_p > 0) {
             // the search may change statuses: keep (first attempt) or restore
@@ -1010,7 +1061,7 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
                                         n_p, hipMemcpyHostToDevice, s));
             }
         }
-        PointArgs Pt{d_pidx, d_q, d_c, pcap, d_ctl + 6, d_ovf};
+        PointArgs Pt{d_pidx, d_q, d_c, pcap, d_ctl + 16 * 6, d_ovf};
         CountArgs Cn{d_cnt};
         passes = 0;
         launches = 0;
@@ -1036,35 +1087,39 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
                 HIPCHK(hipGetLastError());
             }
             HIPCHK(hipEventRecord(ev[3], s));
-            BatchArgs A{d_recs, L, E, Qa, stacks, stack_cap, donate_max, Pt, Cn, deadline};
+            BatchArgs A{d_recs, L, E, Qa, stacks, stack_cap, donate_max, donate_min, pt_chunk,
+                        Pt, Cn, deadline};
             hipLaunchKernelGGL(k_batched, dim3(nwaves), dim3(64), 0, s, A);
             HIPCHK(hipGetLastError());
             launches = passes = 1;
             HIPCHK(hipEventRecord(ev[4], s));
-            unsigned long long hctl[8];
-            HIPCHK(hipMemcpyAsync(hctl, d_ctl, 64, hipMemcpyDeviceToHost, s));
+            unsigned long long hctl[16 * 8];
+            HIPCHK(hipMemcpyAsync(hctl, d_ctl, sizeof hctl, hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
-            queue_peak = hctl[1];              // tickets issued: roots + donations
-            if (hctl[5] == 1) {
+            queue_peak = hctl[16];             // tickets issued: roots + donations
+            if (hctl[80] == 1) {
                 snprintf(errbuf, 256, "frontier ring full (capacity %llu)", ring_cap);
                 return 1;
             }
-            if (hctl[5] == 2) {
+            if (hctl[80] == 2) {
                 snprintf(errbuf, 256, "batched search exceeded its %.0f s deadline", deadline_s);
                 return 1;
             }
-            if (hctl[2] != 0) {
-                snprintf(errbuf, 256, "batched search ended with %lld units open", (long long)hctl[2]);
+            if (hctl[32] != 0) {
+                snprintf(errbuf, 256, "batched search ended with %lld units open", (long long)hctl[32]);
                 return 1;
             }
         }
-        HIPCHK(hipMemcpyAsync(&npts, d_ctl + 6, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(&npts, d_ctl + 16 * 6, 8, hipMemcpyDeviceToHost, s));
         int ovf = 0;
         HIPCHK(hipMemcpyAsync(&ovf, d_ovf, 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         ms_roots = ms_between(ev[2], ev[3]);
         ms_search = ms_between(ev[3], ev[4]);
-        if (!ovf && npts <= pcap) break;
+        if (!ovf) {
+            npts = std::min<unsigned long long>(npts, pcap);   // trailing reserved slots
+            break;
+        }
         pcap = std::max<unsigned long long>(2 * pcap, npts + npts / 4 + 1024);
         if (attempt > 6) {
             snprintf(errbuf, 256, "point buffer overflow");
@@ -1087,9 +1142,9 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
         hipLaunchKernelGGL(k_post, dim3(grid), dim3(256), 0, s, Pa);
         HIPCHK(hipGetLastError());
         if (in->filter) {
-            static std::vector<uint32_t> pr = primes31(96);
-            uint32_t *d_pr = (uint32_t *)up(B_PRIMES, pr.data(), 4 * pr.size());
-            const int dgrid = (int)std::min<int64_t>((n + 255) / 256, 256);
+            static std::vector<PfaModP> pr = primes31(96);
+            PfaModP *d_pr = (PfaModP *)up(B_PRIMES, pr.data(), sizeof(PfaModP) * pr.size());
+            const int dgrid = (int)std::min<int64_t>((n + 255) / 256, 1024);
             const int64_t nthr = (int64_t)dgrid * 256;
             const int64_t S = (int64_t)(HS - 1) * (HS - 1);
             int64_t *d_Nw = (int64_t *)dev_get(C, B_NW, 8 * nthr * S, errbuf);
@@ -1160,6 +1215,11 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     out->stats[PFA_ST_MS_TOTAL] = out->stats[PFA_ST_MS_PREP] + ms_roots + ms_search +
                                   out->stats[PFA_ST_MS_POST];
     out->stats[PFA_ST_QUEUE_PEAK] = (double)queue_peak;
+    out->stats[PFA_ST_STEPS] = (double)hc[3];
+    out->stats[PFA_ST_WAIT_CYC] = (double)hc[4];
+    out->stats[PFA_ST_WAVE_CYC] = (double)hc[5];
+    out->stats[PFA_ST_DONATED] = (double)hc[6];
+    out->stats[PFA_ST_FETCHES] = (double)hc[7];
     for (auto &e : ev) (void)hipEventDestroy(e);
     double st_keep[4] = {out->stats[PFA_ST_KEPT], out->stats[PFA_ST_EXPANSIONS],
                          out->stats[PFA_ST_NONSING], out->stats[PFA_ST_UNDECIDED]};
@@ -1167,7 +1227,11 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
                         in->filter ? h_nexp.data() : nullptr, in->filter ? h_ns.data() : nullptr,
                         h_status.data(), h_code.data(), h_bad.data(), errbuf);
     if (!in->keep_points) {
-        out->stats[PFA_ST_POINTS] = (double)n;
+        std::vector<int32_t> pp(n);
+        if (n) HIPCHK(hipMemcpy(pp.data(), d_pidx, 4 * n, hipMemcpyDeviceToHost));
+        int64_t nv = 0;
+        for (int64_t k = 0; k < n; ++k) nv += pp[k] >= 0;
+        out->stats[PFA_ST_POINTS] = (double)nv;
         out->stats[PFA_ST_KEPT] = st_keep[0];
         out->stats[PFA_ST_EXPANSIONS] = st_keep[1];
         out->stats[PFA_ST_NONSING] = st_keep[2];
@@ -1243,7 +1307,7 @@ static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
                 Kn.data(), nexp.data(), bad.data(), in->filter};
     for (int64_t k = 0; k < n; ++k) post_one(Pa, k);
     if (in->filter) {
-        static std::vector<uint32_t> pr = primes31(96);
+        static std::vector<PfaModP> pr = primes31(96);
         std::vector<int64_t> N((HS - 1) * (HS - 1));
         std::vector<uint32_t> w((HS - 1) * (HS - 1));
         for (int64_t k = 0; k < n; ++k) {
@@ -1251,8 +1315,8 @@ static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
             const int64_t *rec = recs.data() + (int64_t)all.pidx[k] * L.words;
             const int32_t *ri = (const int32_t *)rec;
             const int h = ri[24], geo = ri[26];
-            if (kappa_M(in->kappa + kofs[geo], h, M.data() + k * HS, N.data())) { ns[k] = 2; continue; }
-            ns[k] = (int8_t)pfa_nonsingular(N.data(), h - 1, pr.data(), (int)pr.size(), w.data());
+            if (kappa_M(in->kappa + kofs[geo], h, M.data() + k * HS, N.data(), 1)) { ns[k] = 2; continue; }
+            ns[k] = (int8_t)pfa_nonsingular(N.data(), h - 1, pr.data(), (int)pr.size(), w.data(), 1);
         }
     }
     std::vector<int32_t> status(n_p), code(n_p);
@@ -1488,19 +1552,19 @@ extern "C" int pfa_kernel_batch(const pfa_kernel_input *in, pfa_kernel_output *o
                                    (size_t)2 * D * D * 8 + 64, s, n, d_recs, L);
             unsigned long long pcap = std::max<int64_t>(1 << 14, 64 * n);
             unsigned long long *d_cnt = (unsigned long long *)dev_get(C, B_CNT, 64, errbuf);
-            unsigned long long *d_ctl = (unsigned long long *)dev_get(C, B_CTL, 128, errbuf);
+            unsigned long long *d_ctl = (unsigned long long *)dev_get(C, B_CTL, CTL_BYTES, errbuf);
             int32_t *d_pidx = (int32_t *)dev_get(C, B_PT_PIDX, 4 * pcap, errbuf);
             int64_t *d_q = (int64_t *)dev_get(C, B_PT_Q, 8 * pcap, errbuf);
             int32_t *d_c = (int32_t *)dev_get(C, B_PT_C, 4 * pcap * D, errbuf);
-            HIPCHK(hipMemsetAsync(d_ctl, 0, 128, s));
+            HIPCHK(hipMemsetAsync(d_ctl, 0, CTL_BYTES, s));
             HIPCHK(hipMemsetAsync(d_cnt, 0, 64, s));
-            PointArgs Pt{d_pidx, d_q, d_c, pcap, d_ctl + 6, (int *)(d_ctl + 8)};
+            PointArgs Pt{d_pidx, d_q, d_c, pcap, d_ctl + 16 * 6, (int *)(d_ctl + 16 * 8)};
             CountArgs Cn{d_cnt};
             if (n > 0) hipLaunchKernelGGL(k_plain, dim3((n + 63) / 64), dim3(64), 0, s, n, d_recs, L, Pt, D, Cn);
             HIPCHK(hipGetLastError());
             unsigned long long npts = 0;
             int ovf = 0;
-            HIPCHK(hipMemcpyAsync(&npts, d_ctl + 6, 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(&npts, d_ctl + 16 * 6, 8, hipMemcpyDeviceToHost, s));
             HIPCHK(hipMemcpyAsync(&ovf, d_ctl + 8, 4, hipMemcpyDeviceToHost, s));
             HIPCHK(hipMemcpyAsync(recs.data(), d_recs, 8 * recs.size(), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));

@@ -52,21 +52,41 @@ PFA_HD static inline int64_t pfa_num_expansions(const int64_t *M, const int64_t 
     return cnt;
 }
 
-// (a * b) mod p for a, b < p < 2^31
-PFA_HD static inline uint32_t pfa_mulmod(uint32_t a, uint32_t b, uint32_t p)
+// Arithmetic modulo a prime p < 2^31 with Barrett reduction: m = floor(2^64 / p).
+struct PfaModP {
+    uint32_t p;
+    uint64_t m;
+};
+
+PFA_HD static inline uint64_t pfa_umulhi64(uint64_t a, uint64_t b)
 {
-    return (uint32_t)(((uint64_t)a * b) % p);
+    return (uint64_t)(((pfa_u128)a * b) >> 64);
 }
 
-PFA_HD static inline uint32_t pfa_powmod(uint32_t a, uint32_t e, uint32_t p)
+// x mod p for x < 2^62
+PFA_HD static inline uint32_t pfa_reduce(uint64_t x, const PfaModP &P)
 {
-    uint32_t r = 1;
-    while (e) {
-        if (e & 1) r = pfa_mulmod(r, a, p);
-        a = pfa_mulmod(a, a, p);
-        e >>= 1;
+    uint64_t q = pfa_umulhi64(x, P.m);
+    uint64_t r = x - q * P.p;
+    while (r >= P.p) r -= P.p;                 // at most twice
+    return (uint32_t)r;
+}
+
+PFA_HD static inline uint32_t pfa_mulmod(uint32_t a, uint32_t b, const PfaModP &P)
+{
+    return pfa_reduce((uint64_t)a * b, P);
+}
+
+// inverse of a (0 < a < p) by the extended Euclidean algorithm
+PFA_HD static inline uint32_t pfa_invmod(uint32_t a, uint32_t p)
+{
+    int64_t t = 0, nt = 1, r = p, nr = a;
+    while (nr) {
+        int64_t q = r / nr, x;
+        x = t - q * nt; t = nt; nt = x;
+        x = r - q * nr; r = nr; nr = x;
     }
-    return r;
+    return (uint32_t)(t < 0 ? t + p : t);
 }
 
 PFA_HD static inline uint32_t pfa_mod(int64_t x, uint32_t p)
@@ -75,30 +95,38 @@ PFA_HD static inline uint32_t pfa_mod(int64_t x, uint32_t p)
     return (uint32_t)(r < 0 ? r + p : r);
 }
 
-// det(A) mod p of the n x n int64 matrix A (row stride n); work: n*n uint32.
-// Sequential (one lane per matrix).
-PFA_HD static inline uint32_t pfa_det_mod(const int64_t *A, int n, uint32_t p, uint32_t *w)
+// det(A) mod p for the n x n int64 matrix A; element (i, j) of A and of the
+// work array w lives at index (i * n + j) * stride (stride > 1 interleaves
+// the matrices of many lanes so their accesses coalesce).
+PFA_HD static inline uint32_t pfa_det_mod(const int64_t *A, int n, const PfaModP &P, uint32_t *w,
+                                          int64_t stride)
 {
-    for (int e = 0; e < n * n; ++e) w[e] = pfa_mod(A[e], p);
+    const uint32_t p = P.p;
+    for (int e = 0; e < n * n; ++e) w[e * stride] = pfa_mod(A[e * stride], p);
     uint32_t det = 1;
     for (int k = 0; k < n; ++k) {
         int piv = -1;
-        for (int i = k; i < n; ++i) if (w[i * n + k]) { piv = i; break; }
+        for (int i = k; i < n; ++i) if (w[(i * n + k) * stride]) { piv = i; break; }
         if (piv < 0) return 0;
         if (piv != k) {
-            for (int j = k; j < n; ++j) { uint32_t t = w[k * n + j]; w[k * n + j] = w[piv * n + j]; w[piv * n + j] = t; }
+            for (int j = k; j < n; ++j) {
+                uint32_t t = w[(k * n + j) * stride];
+                w[(k * n + j) * stride] = w[(piv * n + j) * stride];
+                w[(piv * n + j) * stride] = t;
+            }
             det = det ? p - det : 0;
         }
-        uint32_t pv = w[k * n + k];
-        det = pfa_mulmod(det, pv, p);
-        uint32_t inv = pfa_powmod(pv, p - 2, p);
+        uint32_t pv = w[(k * n + k) * stride];
+        det = pfa_mulmod(det, pv, P);
+        uint32_t inv = pfa_invmod(pv, p);
         for (int i = k + 1; i < n; ++i) {
-            uint32_t f = pfa_mulmod(w[i * n + k], inv, p);
+            uint32_t f = w[(i * n + k) * stride];
             if (!f) continue;
+            f = pfa_mulmod(f, inv, P);
             for (int j = k + 1; j < n; ++j) {
-                uint32_t s = pfa_mulmod(f, w[k * n + j], p);
-                uint32_t x = w[i * n + j];
-                w[i * n + j] = x >= s ? x - s : x + p - s;
+                uint32_t sub = pfa_mulmod(f, w[(k * n + j) * stride], P);
+                uint32_t x = w[(i * n + j) * stride];
+                w[(i * n + j) * stride] = x >= sub ? x - sub : x + p - sub;
             }
         }
     }
@@ -106,14 +134,14 @@ PFA_HD static inline uint32_t pfa_det_mod(const int64_t *A, int n, uint32_t p, u
 }
 
 // log2 of the Hadamard bound prod_i |row_i|_2 of the n x n matrix, rounded
-// up generously (each row norm computed in double with 1e-12 slack).
-PFA_HD static inline double pfa_hadamard_log2(const int64_t *A, int n)
+// up generously (each row norm computed in double with 1e-9 bits of slack).
+PFA_HD static inline double pfa_hadamard_log2(const int64_t *A, int n, int64_t stride)
 {
     double s = 0.0;
     for (int i = 0; i < n; ++i) {
         double r = 0.0;
         for (int j = 0; j < n; ++j) {
-            double x = (double)A[i * n + j];
+            double x = (double)A[(i * n + j) * stride];
             r += x * x;
         }
         if (r == 0.0) return -1e300;            // a zero row: det = 0
@@ -123,15 +151,15 @@ PFA_HD static inline double pfa_hadamard_log2(const int64_t *A, int n)
 }
 
 // 1 nonsingular, 0 singular, 2 undecided (more primes needed)
-PFA_HD static inline int pfa_nonsingular(const int64_t *A, int n, const uint32_t *primes, int np,
-                                         uint32_t *w)
+PFA_HD static inline int pfa_nonsingular(const int64_t *A, int n, const PfaModP *primes, int np,
+                                         uint32_t *w, int64_t stride)
 {
-    double need = pfa_hadamard_log2(A, n);
+    double need = pfa_hadamard_log2(A, n, stride);
     if (need < -1e299) return 0;
     double have = 0.0;
     for (int k = 0; k < np; ++k) {
-        if (pfa_det_mod(A, n, primes[k], w) != 0) return 1;
-        have += log2((double)primes[k]) - 1e-9;
+        if (pfa_det_mod(A, n, primes[k], w, stride) != 0) return 1;
+        have += log2((double)primes[k].p) - 1e-9;
         if (have > need) return 0;
     }
     return 2;

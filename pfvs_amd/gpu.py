@@ -29,7 +29,8 @@ _MODES = {"plain": MODE_PLAIN, "batched": MODE_BATCHED, "host": MODE_HOST}
 
 STAT_NAMES = ["nodes", "cand", "leaf", "points", "passes", "ms_prep", "ms_roots", "ms_search",
               "ms_post", "ms_total", "search_launches", "host_p", "queue_peak", "ms_wall",
-              "kept", "expansions", "nonsingular", "undecided"]
+              "kept", "expansions", "nonsingular", "undecided", "steps", "wait_cycles",
+              "wave_cycles", "donated", "fetches"]
 
 
 class _PfgInput(ctypes.Structure):

@@ -41,3 +41,36 @@ def kernel_problems(cases=None):
 def canon_points(M, Kn, q, pidx, h):
     return sorted(zip(np.asarray(pidx).tolist(), map(tuple, np.asarray(M)[:, :h].tolist()),
                       map(tuple, np.asarray(Kn)[:, :h].tolist()), np.asarray(q).tolist()))
+
+
+def cydata_of(g):
+    """CYData of a dataset geometry (stored in the coni basis)."""
+    from pfvs_amd.cydata import CYData
+    h11 = g["h11"]
+    kappa = np.zeros((h11,) * 3, dtype=np.int64)
+    for i, j, k, v in g["kappa_coo"]:
+        for a, b, c in {(i, j, k), (i, k, j), (j, i, k), (j, k, i), (k, i, j), (k, j, i)}:
+            kappa[a, b, c] = v
+    Hc = np.array(g["H"], dtype=np.int64)
+    H = np.hstack([np.zeros((len(Hc), 1), dtype=np.int64), Hc])
+    e = np.eye(h11, dtype=np.int64)
+    return CYData(g["h21"], kappa, g["c2"], H, coni_curve=e[0], coni_cob=e)
+
+
+def as_set(Ks, Ms):
+    return {(tuple(int(x) for x in k), tuple(int(x) for x in m)) for k, m in zip(Ks, Ms)}
+
+
+def oracle_pfvs(data, ps, Q, dilation, M0min=13):
+    """(Ks, Ms) from the oracle alone: its C points and the exact per-point
+    post-processing of oracle/exact.py, in the reference's output order."""
+    from oracle import cpu, exact
+    kap, Mb = data.kappa_cob, data.M_lattice()
+    pf = np.hstack([np.zeros((len(ps), 1), np.int64), np.asarray(ps, dtype=np.int64)])
+    M, K, q, pidx, st, _ = cpu.coni_batch(kap, Mb, pf, Q, cpu.qmax_of(Q, dilation), M0min)
+    assert np.all(st == 0), "oracle needs its exact path for some p"
+    rows = exact.pfvs_from_points(zip(pidx, M, K, q), kap.tolist(), data.h11, Q, M0min)
+    h = data.h11
+    Ks = np.array([r[0] for r in rows], dtype=np.int64).reshape(-1, h)
+    Ms = np.array([r[1] for r in rows], dtype=np.int64).reshape(-1, h)
+    return Ks, Ms

@@ -1,0 +1,111 @@
+"""The coniZpM drop-in end to end: the dataset regression of the reference
+(every primitive p with |p|_inf <= B at dilation D reproduces the dataset's
+PFVs exactly, extras are valid PFVs with gcd(K[1:]) >= 2), equality with the
+oracle's own post-processing, API errors, and -- on the GPU -- device='gpu'
+returning the CPU path's arrays exactly (order included)."""
+
+import numpy as np
+import pytest
+
+from helpers import as_set, cydata_of, geometries, oracle_pfvs
+from oracle import exact
+from pfvs_amd import coniZpM, IncompleteSearchError
+from pfvs_amd.instances import make_G, random_pvecs
+from pfvs_amd.pvectors import box_enum
+
+GEOMS = geometries()
+SMALL = [g for g in GEOMS if g["h11"] <= 7][::3]
+
+
+def _check_against_dataset(g, Ks, Ms, data):
+    got = as_set(Ks, Ms)
+    expected = as_set(g["K"], g["M"])
+    prim = lambda KM: np.gcd.reduce(np.array(KM[0][1:])) == 1
+    assert expected <= got
+    assert {KM for KM in got if prim(KM)} == expected
+    for K, M in got - expected:
+        assert not prim((K, M))
+
+
+@pytest.mark.parametrize("g", SMALL, ids=lambda g: f"h11={g['h11']}-poly{g['polyID']}")
+def test_cpu_reproduces_dataset(g):
+    data = cydata_of(g)
+    ps = box_enum(g["B"], data.H_cob, 1, True)
+    if len(ps) == 0:
+        assert not g["K"]
+        return
+    Ks, Ms = coniZpM(data, ps, ellipsoid_dilation=g["D"], n_jobs=2, device="cpu")
+    _check_against_dataset(g, Ks, Ms, data)
+
+
+def test_cpu_matches_oracle_postprocessing():
+    g = max(SMALL, key=lambda g: len(g["K"]))
+    data = cydata_of(g)
+    ps = box_enum(g["B"], data.H_cob, 1, True)
+    Q = data.h11 + data.h21 + 4
+    Ks, Ms = coniZpM(data, ps, Q=Q, ellipsoid_dilation=g["D"], device="cpu")
+    Ko, Mo = oracle_pfvs(data, ps, Q, g["D"])
+    np.testing.assert_array_equal(Ks, Ko)
+    np.testing.assert_array_equal(Ms, Mo)
+
+
+def test_extras_pass_the_seven_checks():
+    g = max(SMALL, key=lambda g: len(g["K"]))
+    data = cydata_of(g)
+    ps = box_enum(g["B"], data.H_cob, 1, True)
+    Ks, Ms = coniZpM(data, ps, ellipsoid_dilation=g["D"], device="cpu")
+    kap = data.kappa_cob.tolist()
+    for K, M in list(zip(Ks, Ms))[:40]:
+        # p from K[1:] = N p (N invertible): solve exactly
+        from fractions import Fraction
+        h = data.h11
+        N = [[sum(kap[i][j][k] * int(M[k]) for k in range(h)) for j in range(1, h)] for i in range(1, h)]
+        assert not exact.is_singular(N)
+
+
+def test_api_errors():
+    g = SMALL[0]
+    data = cydata_of(g)
+    ps = box_enum(g["B"], data.H_cob, 1, True)
+    with pytest.raises(ValueError, match="device"):
+        coniZpM(data, ps[:1], device="tpu")
+    with pytest.raises(ValueError):
+        coniZpM(data, np.zeros((0, data.h11 - 1)), device="cpu")
+    with pytest.raises(ValueError):
+        coniZpM(data, ps[:1], ellipsoid_dilation=0, device="cpu")
+
+
+def test_max_N_pfvs_raises():
+    g = max(SMALL, key=lambda g: len(g["K"]))
+    data = cydata_of(g)
+    ps = box_enum(g["B"], data.H_cob, 1, True)
+    with pytest.raises(IncompleteSearchError, match="max_N_pfvs"):
+        coniZpM(data, ps, ellipsoid_dilation=g["D"], max_N_pfvs=1, device="cpu")
+
+
+GPU_GEOMS = [g for g in GEOMS if 3 <= g["h11"] <= 11][::3]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dil", [1, 150])
+@pytest.mark.parametrize("g", GPU_GEOMS, ids=lambda g: f"h11={g['h11']}-poly{g['polyID']}-coni{g['coniID']}")
+def test_gpu_matches_cpu(need_gpu, g, dil):
+    data = cydata_of(g)
+    ps = box_enum(g["B"], data.H_cob, 1, True)
+    if len(ps) == 0:
+        pytest.skip("no p-vectors")
+    print(f"h11={g['h11']} {len(ps)} p-vectors dil {dil}", flush=True)
+    Kc, Mc = coniZpM(data, ps, ellipsoid_dilation=dil, device="cpu")
+    Kg, Mg = coniZpM(data, ps, ellipsoid_dilation=dil, device="gpu")
+    np.testing.assert_array_equal(Kg, Kc)
+    np.testing.assert_array_equal(Mg, Mc)
+
+
+@pytest.mark.gpu
+def test_gpu_G_matches_oracle_postprocessing(need_gpu):
+    inst = make_G(10, 150, 3)
+    ps = random_pvecs(inst, 64, 12, seed=1003)
+    Kg, Mg = coniZpM(inst.data, ps, Q=inst.Q, ellipsoid_dilation=4, device="gpu")
+    Ko, Mo = oracle_pfvs(inst.data, ps, inst.Q, 4)
+    np.testing.assert_array_equal(Kg, Ko)
+    np.testing.assert_array_equal(Mg, Mo)
