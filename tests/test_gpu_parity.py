@@ -29,7 +29,7 @@ def test_gpu_kernel_batch_matches_golden(need_gpu, mode):
 
 
 CASES = [(6, 90, 2, 12, 4, 200), (8, 120, 1, 6, 3, 200), (10, 150, 3, 12, 4, 120),
-         (14, 200, 4, 12, 4, 40)]
+         (14, 200, 4, 12, 4, 12)]
 
 
 @pytest.mark.parametrize("mode", ["plain", "batched"])
