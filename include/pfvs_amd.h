@@ -152,6 +152,17 @@ typedef struct {
 int pfa_kernel_batch(const pfa_kernel_input *in, pfa_kernel_output *out);
 void pfa_kernel_output_free(pfa_kernel_output *out);
 
+/* The exact filter det((kappa M)[1:, 1:]) != 0 of coniZpM's
+ * post-processing (pfvs/coni.py:397-407) on its own, for `count` vectors M
+ * (row stride max(h)) of geometries geo[k] (kappa of geometry g: h[g]^3
+ * int64, concatenated): out[k] = 1 nonsingular, 0 singular, 2 undecided
+ * (an entry of kappa M outside int64).  device >= 0 runs the HIP kernel,
+ * device < 0 the host code.  Returns 0, or 1 with a message in err (256
+ * bytes).  No counterpart in the reference, which evaluates the filter
+ * with flint inside coniZpM; exported for testing the filter alone. */
+int pfa_det_filter(int n_geo, const int32_t *h, const int64_t *kappa, int64_t count,
+                   const int32_t *geo, const int64_t *M, int device, int8_t *out, char *err);
+
 /* Library facts: number of compute units of `device` and the build's
  * architecture string. */
 int pfa_device_cus(int device);

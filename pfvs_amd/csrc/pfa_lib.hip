@@ -719,22 +719,6 @@ __global__ void k_post(PostArgs a)
         post_one(a, k);
 }
 
-struct DetArgs {
-    int64_t n;
-    const int32_t *pidx;
-    const int64_t *M;
-    const int64_t *nexp;
-    int MH;
-    const int64_t *recs;
-    RecL L;
-    GeoArgs G;
-    const PfaModP *primes;
-    int np;
-    int64_t *Nw;                  // (HS-1)^2 int64 per thread, interleaved
-    uint32_t *Ww;                 // (HS-1)^2 uint32 per thread, interleaved
-    int8_t *nonsing;
-};
-
 // N = (kappa M)[1:, 1:] for a point (element e at N[e * stride]); 0, or 1 if
 // an entry leaves int64.  kappa is symmetric: N is too.
 PFA_HD static inline int kappa_M(const int64_t *kappa, int h, const int64_t *M, int64_t *N,
@@ -754,19 +738,306 @@ PFA_HD static inline int kappa_M(const int64_t *kappa, int h, const int64_t *M, 
     return 0;
 }
 
-__global__ void k_det(DetArgs a)
+// ------------------------------------------------------------------ k_det
+// det((kappa M)[1:, 1:]) != 0, exactly, for every point with n_exp > 0.
+// A wavefront holds R = 64 / nl points (nl: the largest n = h - 1 of the
+// launch); lane j of group r keeps column j of its point's N, rows
+// s < NMAX in registers.  k_det, per point:
+//   1. N in double when max|kappa| * sum|M| < 2^52 (every product and
+//      partial sum is then an exact integer), from kappa as float (exact:
+//      |kappa| < 2^24 is required, else step 3), staged in LDS for the
+//      geometry of the block's points;
+//   2. fraction-free elimination (Bareiss) in double, every product and
+//      difference checked below 2^53 so each value is the exact integer
+//      minor: a column without a pivot among the unused rows proves
+//      det = 0, n pivots prove det != 0.
+// Points that step 1 or 2 cannot settle go on a list for k_det_mod:
+//   3. N in int128 from the int64 kappa (an entry outside int64 leaves the
+//      point undecided, 2), then det modulo 31-bit primes with Montgomery
+//      products (rows are only ever scaled by units mod p, which keeps
+//      det != 0 invariant) until a residue is nonzero (det != 0) or the
+//      primes' product exceeds the columns' Hadamard bound (det = 0).
+// Pivot rows are marked used instead of swapped (only the sign of det
+// changes).
+struct DetArgs {
+    int64_t n;
+    const int32_t *pidx;
+    const int64_t *M;
+    const int64_t *nexp;
+    int MH;
+    const int64_t *recs;
+    RecL L;
+    GeoArgs G;                    // int64 kappa: step 3
+    const float *kT;              // per distinct kappa: kT[(k n + i) n + j] = kappa[i+1][j+1][k]
+    const int64_t *kTofs;         // per geometry (identical kappas share one copy)
+    const double *kTmax;          // per geometry max |kappa| (inf when >= 2^24)
+    int lds_floats;               // LDS floats per block (0: no staging)
+    const PfaModP *primes;
+    int np;
+    int nl;                       // lanes per point group
+    int64_t *modlist;             // points left to step 3
+    unsigned long long *modcount;
+    int8_t *nonsing;
+};
+
+#define PFA_TWO52 4503599627370496.0
+#define PFA_TWO53 9007199254740992.0
+
+struct DetLanes {
+    int lane, nl, R, r, j, src0;
+    bool inlane;
+    uint64_t gm;                  // ballot bits of this lane's group
+};
+
+__device__ static inline DetLanes det_lanes(int nl)
 {
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nt = (int64_t)gridDim.x * blockDim.x;
-    int64_t *N = a.Nw + tid;                       // element e at N[e * nt]
-    uint32_t *w = a.Ww + tid;
-    for (int64_t k = tid; k < a.n; k += nt) {
-        if (a.pidx[k] < 0 || a.nexp[k] <= 0) { a.nonsing[k] = 0; continue; }
-        const int64_t *rec = a.recs + (int64_t)a.pidx[k] * a.L.words;
-        const int32_t *ri = (const int32_t *)rec;
-        const int h = ri[24], geo = ri[26];
-        if (kappa_M(a.G.kappa + a.G.kofs[geo], h, a.M + k * a.MH, N, nt)) { a.nonsing[k] = 2; continue; }
-        a.nonsing[k] = (int8_t)pfa_nonsingular(N, h - 1, a.primes, a.np, w, nt);
+    DetLanes d;
+    d.lane = threadIdx.x & 63;
+    d.nl = nl;
+    d.R = 64 / nl;
+    d.r = d.lane / nl;
+    d.j = d.lane - d.r * nl;
+    d.inlane = d.r < d.R;
+    d.gm = d.inlane ? (((nl >= 64) ? ~0ull : ((1ull << nl) - 1)) << (d.r * nl)) : 0ull;
+    d.src0 = d.inlane ? d.r * nl : d.lane;
+    return d;
+}
+
+// Geometry row of point kp: (h, geo), or h = 0 when the point needs no test.
+__device__ static inline int det_point(const DetArgs &a, int64_t kp, int &geo)
+{
+    if (kp >= a.n || a.pidx[kp] < 0 || a.nexp[kp] <= 0) return 0;
+    const int32_t *ri = (const int32_t *)(a.recs + (int64_t)a.pidx[kp] * a.L.words);
+    geo = ri[26];
+    return ri[24];
+}
+
+// w[s] = N[s][j] = sum_k kappa[s+1][j+1][k] M[k] for the column j of a lane
+// (kt = that column's kT entries, LDS or global), msum = sum_k |M[k]|.
+// M[k] comes from lane src0 + k (mj) or, for k = n, from mlast.  Every
+// lane runs the shuffles.
+template <int NMAX>
+__device__ __forceinline__ void det_build(double *w, double &msum, const float *kt, double mj,
+                                          double mlast, int src0, int nl, int n, bool col)
+{
+    for (int k = 0; k <= nl; ++k) {
+        const double mk0 = __shfl(mj, src0 + (k < nl ? k : 0));
+        if (!col || k > n) continue;
+        const double mk = k < n ? mk0 : mlast;
+        msum += fabs(mk);
+        const float *kk = kt + (int64_t)k * n * n;
+#pragma unroll
+        for (int s = 0; s < NMAX; ++s)
+            if (s < n) w[s] = fma((double)kk[s * n], mk, w[s]);
+    }
+}
+
+template <int NMAX>
+__global__ __launch_bounds__(256) void k_det(DetArgs a)
+{
+    extern __shared__ float lds_kT[];
+    __shared__ int64_t lds_ofs;
+    __shared__ int lds_cnt;
+    const DetLanes L = det_lanes(a.nl);
+    const int R = L.R, j = L.j;
+    const int wv = threadIdx.x >> 6;
+    // each block takes a contiguous span of rounds (4 wavefronts x R points)
+    const int64_t rounds = (a.n + 4 * R - 1) / (4 * R);
+    const int64_t span = (rounds + gridDim.x - 1) / gridDim.x;
+    const int64_t rd0 = (int64_t)blockIdx.x * span;
+    const int64_t rd1 = rd0 + span < rounds ? rd0 + span : rounds;
+    if (rd0 >= rd1) return;
+    // stage kappa of the geometry of the span's first tested point
+    int64_t cofs = -1;
+    if (a.lds_floats > 0) {
+        if (threadIdx.x == 0) {
+            int64_t o = -1;
+            int c = 0;
+            const int64_t end = rd1 * 4 * R < a.n ? rd1 * 4 * R : a.n;
+            for (int64_t kp = rd0 * 4 * R; kp < end && kp < rd0 * 4 * R + 4096; ++kp) {
+                int g = 0;
+                const int h = det_point(a, kp, g);
+                if (h <= 1) continue;
+                if ((int64_t)(h - 1) * (h - 1) * h <= a.lds_floats && a.kTmax[g] < PFA_TWO52) {
+                    o = a.kTofs[g];
+                    c = (h - 1) * (h - 1) * h;
+                }
+                break;
+            }
+            lds_ofs = o;
+            lds_cnt = c;
+        }
+        __syncthreads();
+        cofs = lds_ofs;
+        const int c = lds_cnt;
+        for (int e = threadIdx.x; e < c; e += blockDim.x) lds_kT[e] = a.kT[cofs + e];
+        __syncthreads();
+    }
+    for (int64_t rd = rd0; rd < rd1; ++rd) {
+        const int64_t kp = (rd * 4 + wv) * R + L.r;
+        int geo = 0;
+        const int h = L.inlane ? det_point(a, kp, geo) : 0;
+        const bool need = h > 0;
+        const int n = need ? h - 1 : 0;
+        const bool col = need && j < n;
+        const int64_t *Mrow = a.M + (need ? kp : 0) * a.MH;
+        int res = (need && n == 0) ? 1 : 0;
+
+        // 1. N in double
+        double w[NMAX];
+#pragma unroll
+        for (int s = 0; s < NMAX; ++s) w[s] = 0.0;
+        const double mj = col ? (double)Mrow[j] : 0.0;
+        const double mlast = need ? (double)Mrow[n] : 0.0;
+        const bool lds = col && a.kTofs[geo] == cofs;
+        double msum = 0.0;
+        if (!__ballot(col && !lds))
+            det_build<NMAX>(w, msum, lds_kT + j, mj, mlast, L.src0, a.nl, n, col);
+        else
+            det_build<NMAX>(w, msum, a.kT + (col ? a.kTofs[geo] : 0) + j, mj, mlast, L.src0, a.nl, n, col);
+        const bool exact = !col || a.kTmax[geo] * msum < PFA_TWO52;
+        const bool exact_g = !(__ballot(col && !exact) & L.gm);
+
+        // 2. Bareiss in double
+        bool run = need && n > 0 && exact_g, trip = false;
+        uint64_t used = 0;
+        double prev = 1.0;
+        for (int k = 0; k < a.nl; ++k) {
+            if (run && k >= n) { run = false; res = 1; }
+            if (!__ballot(run)) break;
+            int cand = -1;
+#pragma unroll
+            for (int s = NMAX - 1; s >= 0; --s)
+                if (s < n && !((used >> s) & 1) && w[s] != 0.0) cand = s;
+            const int srck = L.inlane ? L.src0 + k : L.lane;
+            const int p = __shfl(cand, srck);
+            bool act = run;
+            if (act && p < 0) { run = false; act = false; res = 0; }
+            double pj = 0.0;
+#pragma unroll
+            for (int s = 0; s < NMAX; ++s)
+                if (s == p) pj = w[s];
+            const double pk = __shfl(pj, srck);
+            bool ok = true;
+#pragma unroll
+            for (int s = 0; s < NMAX; ++s) {
+                const double ik = __shfl(w[s], srck);
+                if (act && col && s < n && s != p && !((used >> s) & 1)) {
+                    const double x1 = w[s] * pk, x2 = ik * pj, x = x1 - x2;
+                    ok = ok && fabs(x1) < PFA_TWO53 && fabs(x2) < PFA_TWO53 && fabs(x) < PFA_TWO53;
+                    w[s] = k == 0 ? x : x / prev;
+                }
+            }
+            if (act && (__ballot(!ok) & L.gm)) { run = false; act = false; trip = true; }
+            if (act) {
+                used |= 1ull << p;
+                prev = pk;
+            }
+        }
+        if (run) res = 1;
+        if (L.inlane && j == 0 && kp < a.n) {
+            if (need && n > 0 && (!exact_g || trip)) {
+                a.nonsing[kp] = 2;                 // until k_det_mod decides
+                const unsigned long long t = atomicAdd(a.modcount, 1ull);
+                a.modlist[t] = kp;                 // capacity: one entry per point slot
+            } else {
+                a.nonsing[kp] = (int8_t)res;
+            }
+        }
+    }
+}
+
+// Step 3 for the points k_det listed.
+template <int NMAX>
+__global__ __launch_bounds__(256) void k_det_mod(DetArgs a)
+{
+    const DetLanes L = det_lanes(a.nl);
+    const int R = L.R, j = L.j;
+    const int64_t cnt = (int64_t)*a.modcount;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t base = wave * R; base < cnt; base += nwaves * R) {
+        const int64_t t = base + L.r;
+        const int64_t kp = (L.inlane && t < cnt) ? a.modlist[t] : a.n;
+        int geo = 0;
+        const int h = L.inlane ? det_point(a, kp, geo) : 0;
+        const bool need = h > 1;
+        const int n = need ? h - 1 : 0;
+        const bool col = need && j < n;
+        const int64_t *Mrow = a.M + (need ? kp : 0) * a.MH;
+        int res = 2;
+        bool modn = need;
+        int64_t Ni[NMAX];
+#pragma unroll
+        for (int s = 0; s < NMAX; ++s) Ni[s] = 0;
+        bool fits = true;
+        double cn = 0.0;
+        if (col) {
+            const int64_t *kap = a.G.kappa + a.G.kofs[geo];
+#pragma unroll
+            for (int s = 0; s < NMAX; ++s) {
+                if (s >= n) continue;
+                const int64_t *kij = kap + ((int64_t)(s + 1) * h + (j + 1)) * h;
+                pfa_i128 acc = 0;
+                for (int k = 0; k < h; ++k) {
+                    pfa_i128 u;
+                    if (pfa_add_ovf(acc, (pfa_i128)kij[k] * Mrow[k], &u)) fits = false;
+                    acc = u;
+                }
+                if (!pfa_fits64(acc)) fits = false;
+                Ni[s] = (int64_t)acc;
+                cn += (double)Ni[s] * (double)Ni[s];
+            }
+        }
+        if (modn && (__ballot(modn && !fits) & L.gm)) { modn = false; res = 2; }
+        if (modn && (__ballot(modn && col && cn == 0.0) & L.gm)) { modn = false; res = 0; }
+        const double tb = (modn && col) ? 0.5 * log2(cn) + 1e-9 : 0.0;
+        double need_bits = 1.0;
+        for (int q = 0; q < a.nl; ++q) need_bits += __shfl(tb, L.inlane ? L.src0 + q : L.lane);
+        double have = 0.0;
+        for (int pi = 0; pi < a.np; ++pi) {
+            if (!__ballot(modn)) break;
+            const PfaModP P = a.primes[pi];
+            uint32_t wm[NMAX];
+#pragma unroll
+            for (int s = 0; s < NMAX; ++s)
+                wm[s] = (modn && col && s < n) ? pfa_mod_fast(Ni[s], P) : 0u;
+            bool runm = modn, zero = false;
+            uint64_t usedm = 0;
+            for (int k = 0; k < a.nl; ++k) {
+                if (runm && k >= n) runm = false;
+                if (!__ballot(runm)) break;
+                int cand = -1;
+#pragma unroll
+                for (int s = NMAX - 1; s >= 0; --s)
+                    if (s < n && !((usedm >> s) & 1) && wm[s] != 0u) cand = s;
+                const int srck = L.inlane ? L.src0 + k : L.lane;
+                const int p = __shfl(cand, srck);
+                bool act = runm;
+                if (act && p < 0) { runm = false; act = false; zero = true; }
+                uint32_t pj = 0;
+#pragma unroll
+                for (int s = 0; s < NMAX; ++s)
+                    if (s == p) pj = wm[s];
+                const uint32_t pk = (uint32_t)__shfl((int)pj, srck);
+#pragma unroll
+                for (int s = 0; s < NMAX; ++s) {
+                    const uint32_t ik = (uint32_t)__shfl((int)wm[s], srck);
+                    if (act && col && s < n && s != p && !((usedm >> s) & 1)) {
+                        const uint32_t x1 = pfa_mont(pk, wm[s], P.p, P.pinv);
+                        const uint32_t x2 = pfa_mont(ik, pj, P.p, P.pinv);
+                        wm[s] = x1 >= x2 ? x1 - x2 : x1 + (P.p - x2);
+                    }
+                }
+                if (act) usedm |= 1ull << p;
+            }
+            if (modn && !zero) { res = 1; modn = false; }
+            if (modn) {
+                have += log2((double)P.p) - 1e-9;
+                if (have > need_bits) { res = 0; modn = false; }
+            }
+        }
+        if (need && j == 0) a.nonsing[kp] = (int8_t)res;
     }
 }
 
@@ -793,6 +1064,7 @@ struct DevCtx {
     std::vector<DevBuf> bufs;
     int cus = 0;
     size_t lds_max = 0;
+    unsigned long long pcap_hint = 0;      // largest point count seen
 };
 
 static std::mutex g_ctx_mu;
@@ -838,7 +1110,7 @@ static void *dev_get(DevCtx *C, int slot, size_t bytes, char *errbuf)
 enum {
     B_GEO_H, B_GEO_Q, B_GEO_QMAX, B_GEO_LIN, B_KAPPA, B_MBASIS, B_KOFS, B_MOFS, B_PS, B_PGEO,
     B_RECS, B_RING, B_SEQ, B_STACKS, B_CTL, B_PT_PIDX, B_PT_Q, B_PT_C, B_CNT, B_M, B_KN, B_NEXP, B_BAD,
-    B_PRIMES, B_NW, B_WW, B_NONSING, B_NBUF
+    B_PRIMES, B_KT, B_KTOFS, B_KTMAX, B_MODCNT, B_MODLIST, B_NONSING, B_NBUF
 };
 
 // the k largest primes below 2^31 with their Barrett constants
@@ -849,7 +1121,7 @@ static std::vector<PfaModP> primes31(int k)
         bool pr = true;
         for (uint32_t f = 3; (uint64_t)f * f <= x; f += 2)
             if (x % f == 0) { pr = false; break; }
-        if (pr) out.push_back(PfaModP{x, (uint64_t)(((pfa_u128)1 << 64) / x)});
+        if (pr) out.push_back(PfaModP{x, pfa_mont_pinv(x), (uint64_t)(((pfa_u128)1 << 64) / x)});
     }
     return out;
 }
@@ -900,6 +1172,111 @@ static void fill_geo_offsets(const pfa_run_input *in, std::vector<int64_t> &kofs
         ktot += h * h * h;
         mtot += h * h;
     }
+}
+
+// kappa[i+1][j+1][k] as float at kT[ofs + (k n + i) n + j] (n = h - 1; the
+// lanes of a point group read consecutive j), one copy per distinct kappa
+// (kTofs per geometry), and the largest |kappa| of each geometry
+// (infinity when one reaches 2^24, where float stops being exact).
+static void build_kT(const pfa_run_input *in, const std::vector<int64_t> &kofs,
+                     std::vector<float> &kT, std::vector<int64_t> &kTofs,
+                     std::vector<double> &kTmax)
+{
+    kTofs.assign(in->n_geo, 0);
+    kTmax.assign(in->n_geo, 0.0);
+    kT.clear();
+    std::vector<std::pair<uint64_t, int>> seen;    // (hash, first geometry)
+    for (int g = 0; g < in->n_geo; ++g) {
+        const int64_t h = in->h[g], n = h - 1;
+        const int64_t *kap = in->kappa + kofs[g];
+        uint64_t hs = 1469598103934665603ull ^ (uint64_t)h;
+        for (int64_t e = 0; e < h * h * h; ++e) hs = (hs ^ (uint64_t)kap[e]) * 1099511628211ull;
+        int same = -1;
+        for (auto &pr : seen)
+            if (pr.first == hs && in->h[pr.second] == h &&
+                !memcmp(in->kappa + kofs[pr.second], kap, 8 * h * h * h)) {
+                same = pr.second;
+                break;
+            }
+        if (same >= 0) {
+            kTofs[g] = kTofs[same];
+            kTmax[g] = kTmax[same];
+            continue;
+        }
+        seen.push_back({hs, g});
+        kTofs[g] = (int64_t)kT.size();
+        kT.resize(kT.size() + h * n * n);
+        double mx = 0.0;
+        for (int64_t k = 0; k < h; ++k)
+            for (int64_t i = 0; i < n; ++i)
+                for (int64_t jj = 0; jj < n; ++jj) {
+                    const int64_t v = kap[((i + 1) * h + (jj + 1)) * h + k];
+                    kT[kTofs[g] + (k * n + i) * n + jj] = (float)v;
+                    mx = std::max(mx, std::fabs((double)v));
+                }
+        kTmax[g] = mx < 16777216.0 ? mx : INFINITY;
+    }
+    if (kT.empty()) kT.push_back(0.f);
+}
+
+static void *upload(DevCtx *C, int slot, const void *src, size_t bytes, hipStream_t s, char *errbuf)
+{
+    void *d = dev_get(C, slot, std::max<size_t>(bytes, 16), errbuf);
+    if (bytes) HIPCHK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, s));
+    return d;
+}
+
+// The exact filter over n point slots (k_det, then k_det_mod on the points
+// it could not settle); nonsing[] receives 0 / 1 / 2 on stream s.
+static void det_run(DevCtx *C, hipStream_t s, const pfa_run_input *gin,
+                    const std::vector<int64_t> &kofs, const GeoArgs &G, int64_t n,
+                    const int32_t *d_pidx, const int64_t *d_M, const int64_t *d_nexp, int MH,
+                    const int64_t *d_recs, RecL L, int D, int8_t *d_ns, char *errbuf)
+{
+    static const std::vector<PfaModP> pr = primes31(96);
+    std::vector<float> kT;
+    std::vector<double> kTmax;
+    std::vector<int64_t> kTofs;
+    build_kT(gin, kofs, kT, kTofs, kTmax);
+    int lds_floats = 0;                    // largest kappa slab of the launch, if it fits
+    for (int g = 0; g < gin->n_geo; ++g) {
+        const int h = gin->h[g];
+        lds_floats = std::max(lds_floats, (h - 1) * (h - 1) * h);
+    }
+    const int lds_cap = 96 * 1024 / 4;
+    if (lds_floats > lds_cap) lds_floats = 0;
+    unsigned long long *d_mc = (unsigned long long *)dev_get(C, B_MODCNT, 8, errbuf);
+    HIPCHK(hipMemsetAsync(d_mc, 0, 8, s));
+    DetArgs a{n, d_pidx, d_M, d_nexp, MH, d_recs, L, G,
+              (const float *)upload(C, B_KT, kT.data(), 4 * kT.size(), s, errbuf),
+              (const int64_t *)upload(C, B_KTOFS, kTofs.data(), 8 * kTofs.size(), s, errbuf),
+              (const double *)upload(C, B_KTMAX, kTmax.data(), 8 * kTmax.size(), s, errbuf),
+              lds_floats,
+              (const PfaModP *)upload(C, B_PRIMES, pr.data(), sizeof(PfaModP) * pr.size(), s, errbuf),
+              (int)pr.size(), std::max(D, 1),
+              (int64_t *)dev_get(C, B_MODLIST, 8 * std::max<int64_t>(n, 1), errbuf), d_mc, d_ns};
+    if (n <= 0) return;
+    const int R = 64 / a.nl;
+    const int64_t rounds = (n + 4 * R - 1) / (4 * R);
+    const int grid = (int)std::min<int64_t>(rounds, 16384);
+    const size_t lds = (size_t)lds_floats * 4;
+#define PFA_DET_LAUNCH(NM)                                                                     \
+    do {                                                                                        \
+        HIPCHK(hipFuncSetAttribute((const void *)k_det<NM>,                                    \
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));      \
+        hipLaunchKernelGGL(k_det<NM>, dim3(grid), dim3(256), lds, s, a);                        \
+        HIPCHK(hipGetLastError());                                                              \
+        hipLaunchKernelGGL(k_det_mod<NM>, dim3(512), dim3(256), 0, s, a);                       \
+        HIPCHK(hipGetLastError());                                                              \
+    } while (0)
+    if (a.nl <= 8) PFA_DET_LAUNCH(8);
+    else if (a.nl <= 12) PFA_DET_LAUNCH(12);
+    else if (a.nl <= 16) PFA_DET_LAUNCH(16);
+    else if (a.nl <= 20) PFA_DET_LAUNCH(20);
+    else if (a.nl <= 24) PFA_DET_LAUNCH(24);
+    else if (a.nl <= 32) PFA_DET_LAUNCH(32);
+    else PFA_DET_LAUNCH(40);
+#undef PFA_DET_LAUNCH
 }
 
 // Final assembly shared by the device and host runs: statuses, sort, copy.
@@ -1017,7 +1394,8 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     HIPCHK(hipEventRecord(ev[1], s));
 
     // points buffer: grow and rerun the search on overflow
-    unsigned long long pcap = std::max<int64_t>(1 << 16, 16 * n_p);
+    unsigned long long pcap = std::max<unsigned long long>(
+        std::max<int64_t>(1 << 16, 16 * n_p), C->pcap_hint + C->pcap_hint / 8);
     const int waves_per_cu = in->waves_per_cu > 0 ? in->waves_per_cu : 8;
     const int donate_max = in->budget > 0 ? in->budget : 32;
     const int donate_min = 64;               // deque depth before a wavefront donates
@@ -1033,7 +1411,7 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     const double deadline_s = dl_env ? atof(dl_env) : 120.0;
     const unsigned long long deadline =
         (unsigned long long)(deadline_s * 1000.0 * (double)std::max(wclk_khz, 1));
-    double ms_roots = 0, ms_search = 0;
+    double ms_roots = 0, ms_search = 0, attempts_wasted_ms = 0;
     int passes = 0, launches = 0;
     unsigned long long npts = 0, queue_peak = 0;
     int32_t *d_pidx = nullptr;
@@ -1116,10 +1494,12 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
         HIPCHK(hipStreamSynchronize(s));
         ms_roots = ms_between(ev[2], ev[3]);
         ms_search = ms_between(ev[3], ev[4]);
+        C->pcap_hint = std::max(C->pcap_hint, npts);
         if (!ovf) {
             npts = std::min<unsigned long long>(npts, pcap);   // trailing reserved slots
             break;
         }
+        attempts_wasted_ms += ms_roots + ms_search;
         pcap = std::max<unsigned long long>(2 * pcap, npts + npts / 4 + 1024);
         if (attempt > 6) {
             snprintf(errbuf, 256, "point buffer overflow");
@@ -1142,16 +1522,7 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
         hipLaunchKernelGGL(k_post, dim3(grid), dim3(256), 0, s, Pa);
         HIPCHK(hipGetLastError());
         if (in->filter) {
-            static std::vector<PfaModP> pr = primes31(96);
-            PfaModP *d_pr = (PfaModP *)up(B_PRIMES, pr.data(), sizeof(PfaModP) * pr.size());
-            const int dgrid = (int)std::min<int64_t>((n + 255) / 256, 1024);
-            const int64_t nthr = (int64_t)dgrid * 256;
-            const int64_t S = (int64_t)(HS - 1) * (HS - 1);
-            int64_t *d_Nw = (int64_t *)dev_get(C, B_NW, 8 * nthr * S, errbuf);
-            uint32_t *d_Ww = (uint32_t *)dev_get(C, B_WW, 4 * nthr * S, errbuf);
-            DetArgs Da{n, d_pidx, d_M, d_nexp, HS, d_recs, L, G, d_pr, (int)pr.size(), d_Nw, d_Ww, d_ns};
-            hipLaunchKernelGGL(k_det, dim3(dgrid), dim3(256), 0, s, Da);
-            HIPCHK(hipGetLastError());
+            det_run(C, s, in, kofs, G, n, d_pidx, d_M, d_nexp, HS, d_recs, L, D, d_ns, errbuf);
         }
     }
     HIPCHK(hipEventRecord(ev[6], s));
@@ -1212,8 +1583,9 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     out->stats[PFA_ST_MS_ROOTS] = ms_roots;
     out->stats[PFA_ST_MS_SEARCH] = ms_search;
     out->stats[PFA_ST_MS_POST] = ms_between(ev[5], ev[6]);
+    // (a search rerun after the point buffer overflowed counts too)
     out->stats[PFA_ST_MS_TOTAL] = out->stats[PFA_ST_MS_PREP] + ms_roots + ms_search +
-                                  out->stats[PFA_ST_MS_POST];
+                                  out->stats[PFA_ST_MS_POST] + attempts_wasted_ms;
     out->stats[PFA_ST_QUEUE_PEAK] = (double)queue_peak;
     out->stats[PFA_ST_STEPS] = (double)hc[3];
     out->stats[PFA_ST_WAIT_CYC] = (double)hc[4];
@@ -1310,13 +1682,15 @@ static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
         static std::vector<PfaModP> pr = primes31(96);
         std::vector<int64_t> N((HS - 1) * (HS - 1));
         std::vector<uint32_t> w((HS - 1) * (HS - 1));
+        std::vector<pfa_i128> bw((HS - 1) * (HS - 1));
         for (int64_t k = 0; k < n; ++k) {
             if (nexp[k] <= 0) { ns[k] = 0; continue; }
             const int64_t *rec = recs.data() + (int64_t)all.pidx[k] * L.words;
             const int32_t *ri = (const int32_t *)rec;
             const int h = ri[24], geo = ri[26];
             if (kappa_M(in->kappa + kofs[geo], h, M.data() + k * HS, N.data(), 1)) { ns[k] = 2; continue; }
-            ns[k] = (int8_t)pfa_nonsingular(N.data(), h - 1, pr.data(), (int)pr.size(), w.data(), 1);
+            ns[k] = (int8_t)pfa_nonsingular(N.data(), h - 1, pr.data(), (int)pr.size(), w.data(),
+                                            bw.data(), 1);
         }
     }
     std::vector<int32_t> status(n_p), code(n_p);
@@ -1452,6 +1826,83 @@ extern "C" int pfa_device_cus(int device)
 extern "C" const char *pfa_arch(void) { return "gfx950"; }
 
 // ------------------------------------------------- single-problem batch
+extern "C" int pfa_det_filter(int n_geo, const int32_t *h, const int64_t *kappa, int64_t count,
+                              const int32_t *geo, const int64_t *M, int device, int8_t *out,
+                              char *err)
+{
+    char errbuf[256] = {0};
+    try {
+        int hmax = 1;
+        for (int g = 0; g < n_geo; ++g) {
+            if (h[g] < 1 || h[g] > PFA_MAX_H11) {
+                snprintf(err, 256, "h11 outside 1..%d", PFA_MAX_H11);
+                return 1;
+            }
+            hmax = std::max(hmax, (int)h[g]);
+        }
+        for (int64_t k = 0; k < count; ++k)
+            if (geo[k] < 0 || geo[k] >= n_geo) {
+                snprintf(err, 256, "geometry index out of range");
+                return 1;
+            }
+        pfa_run_input gin;
+        memset(&gin, 0, sizeof gin);
+        gin.n_geo = n_geo;
+        gin.h = h;
+        gin.kappa = kappa;
+        std::vector<int64_t> kofs, mofs;
+        int64_t ktot = 0, mtot = 0;
+        fill_geo_offsets(&gin, kofs, mofs, ktot, mtot);
+        const int MH = hmax;
+        if (device < 0) {
+            static std::vector<PfaModP> pr = primes31(96);
+            std::vector<int64_t> N((size_t)MH * MH);
+            std::vector<uint32_t> w((size_t)MH * MH);
+            std::vector<pfa_i128> bw((size_t)MH * MH);
+            for (int64_t k = 0; k < count; ++k) {
+                const int hh = h[geo[k]];
+                if (kappa_M(kappa + kofs[geo[k]], hh, M + k * MH, N.data(), 1)) { out[k] = 2; continue; }
+                out[k] = (int8_t)pfa_nonsingular(N.data(), hh - 1, pr.data(), (int)pr.size(), w.data(),
+                                                 bw.data(), 1);
+            }
+            return 0;
+        }
+        DevCtx *C = ctx_for(device);
+        std::lock_guard<std::mutex> lk(C->mu);
+        ctx_init(C, device, errbuf);
+        hipStream_t s = C->stream;
+        auto up = [&](int slot, const void *src, size_t bytes) -> void * {
+            void *d = dev_get(C, slot, std::max<size_t>(bytes, 16), errbuf);
+            if (bytes) HIPCHK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, s));
+            return d;
+        };
+        const int D = std::max(hmax - 1, 1);
+        const RecL L = rec_layout(D);
+        std::vector<int64_t> recs((size_t)L.words * std::max(n_geo, 1), 0);
+        for (int g = 0; g < n_geo; ++g) {
+            int32_t *ri = (int32_t *)(recs.data() + (size_t)g * L.words);
+            ri[24] = h[g];
+            ri[26] = g;
+        }
+        std::vector<int64_t> one((size_t)std::max<int64_t>(count, 1), 1);
+        GeoArgs G{};
+        G.kappa = (const int64_t *)up(B_KAPPA, kappa, 8 * ktot);
+        G.kofs = (const int64_t *)up(B_KOFS, kofs.data(), 8 * kofs.size());
+        const int32_t *d_geo = (const int32_t *)up(B_PT_PIDX, geo, 4 * count);
+        const int64_t *d_M = (const int64_t *)up(B_M, M, 8 * count * MH);
+        const int64_t *d_one = (const int64_t *)up(B_NEXP, one.data(), 8 * count);
+        const int64_t *d_recs = (const int64_t *)up(B_RECS, recs.data(), 8 * recs.size());
+        int8_t *d_ns = (int8_t *)dev_get(C, B_NONSING, std::max<int64_t>(count, 1), errbuf);
+        det_run(C, s, &gin, kofs, G, count, d_geo, d_M, d_one, MH, d_recs, L, D, d_ns, errbuf);
+        if (count > 0) HIPCHK(hipMemcpyAsync(out, d_ns, count, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return 0;
+    } catch (...) {
+        snprintf(err, 256, "%s", errbuf);
+        return 1;
+    }
+}
+
 extern "C" int pfa_kernel_batch(const pfa_kernel_input *in, pfa_kernel_output *out)
 {
     memset(out, 0, sizeof(*out));
@@ -1565,7 +2016,7 @@ extern "C" int pfa_kernel_batch(const pfa_kernel_input *in, pfa_kernel_output *o
             unsigned long long npts = 0;
             int ovf = 0;
             HIPCHK(hipMemcpyAsync(&npts, d_ctl + 16 * 6, 8, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipMemcpyAsync(&ovf, d_ctl + 8, 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(&ovf, d_ctl + 16 * 8, 4, hipMemcpyDeviceToHost, s));
             HIPCHK(hipMemcpyAsync(recs.data(), d_recs, 8 * recs.size(), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
             if (ovf || npts > pcap) { snprintf(errbuf, 256, "point buffer overflow"); return 1; }

@@ -52,18 +52,37 @@ PFA_HD static inline int64_t pfa_num_expansions(const int64_t *M, const int64_t 
     return cnt;
 }
 
-// Arithmetic modulo a prime p < 2^31 with Barrett reduction: m = floor(2^64 / p).
+// Arithmetic modulo an odd prime p < 2^31: Barrett reduction with
+// m = floor(2^64 / p), and Montgomery products with pinv = -p^-1 mod 2^32.
 struct PfaModP {
     uint32_t p;
+    uint32_t pinv;
     uint64_t m;
 };
+
+// -p^-1 mod 2^32 for odd p (Newton iteration on the 2-adic inverse)
+PFA_HD static inline uint32_t pfa_mont_pinv(uint32_t p)
+{
+    uint32_t x = p;                        // correct to 3 bits: p * p = 1 mod 8
+    for (int i = 0; i < 5; ++i) x *= 2u - p * x;
+    return 0u - x;
+}
+
+// a b 2^-32 mod p for a, b < p (result < p)
+PFA_HD static inline uint32_t pfa_mont(uint32_t a, uint32_t b, uint32_t p, uint32_t pinv)
+{
+    const uint64_t t = (uint64_t)a * b;                    // < 2^62
+    const uint32_t m = (uint32_t)t * pinv;
+    const uint64_t u = (t + (uint64_t)m * p) >> 32;       // < 2 p
+    return (uint32_t)(u >= p ? u - p : u);
+}
 
 PFA_HD static inline uint64_t pfa_umulhi64(uint64_t a, uint64_t b)
 {
     return (uint64_t)(((pfa_u128)a * b) >> 64);
 }
 
-// x mod p for x < 2^62
+// x mod p for any x < 2^64 (q is at most one short of floor(x / p))
 PFA_HD static inline uint32_t pfa_reduce(uint64_t x, const PfaModP &P)
 {
     uint64_t q = pfa_umulhi64(x, P.m);
@@ -93,6 +112,14 @@ PFA_HD static inline uint32_t pfa_mod(int64_t x, uint32_t p)
 {
     int64_t r = x % (int64_t)p;
     return (uint32_t)(r < 0 ? r + p : r);
+}
+
+// x mod p in [0, p) without a 64-bit division
+PFA_HD static inline uint32_t pfa_mod_fast(int64_t x, const PfaModP &P)
+{
+    const uint64_t u = x < 0 ? (uint64_t)(-(x + 1)) + 1u : (uint64_t)x;     // |x|
+    const uint32_t r = u < P.p ? (uint32_t)u : pfa_reduce(u, P);
+    return (x < 0 && r) ? P.p - r : r;
 }
 
 // det(A) mod p for the n x n int64 matrix A; element (i, j) of A and of the
@@ -150,14 +177,56 @@ PFA_HD static inline double pfa_hadamard_log2(const int64_t *A, int n, int64_t s
     return s + 1.0;
 }
 
-// 1 nonsingular, 0 singular, 2 undecided (more primes needed)
+// Exact fraction-free (Bareiss) elimination in checked int128 (same
+// element layout as pfa_det_mod): 0 singular, 1 nonsingular, 2 an
+// intermediate left int128.  On rank-deficient matrices it stops as soon as
+// a column has no pivot, after r eliminations of minors of size <= r + 1.
+PFA_HD static inline int pfa_bareiss_decide(const int64_t *A, int n, pfa_i128 *w, int64_t stride)
+{
+    for (int e = 0; e < n * n; ++e) w[e * stride] = A[e * stride];
+    pfa_i128 prev = 1;
+    for (int k = 0; k < n; ++k) {
+        int piv = -1;
+        for (int i = k; i < n; ++i) if (w[(i * n + k) * stride] != 0) { piv = i; break; }
+        if (piv < 0) return 0;
+        if (piv != k)
+            for (int j = k; j < n; ++j) {
+                pfa_i128 t = w[(k * n + j) * stride];
+                w[(k * n + j) * stride] = w[(piv * n + j) * stride];
+                w[(piv * n + j) * stride] = t;
+            }
+        const pfa_i128 pk = w[(k * n + k) * stride];
+        for (int i = k + 1; i < n; ++i) {
+            const pfa_i128 ik = w[(i * n + k) * stride];
+            for (int j = k + 1; j < n; ++j) {
+                pfa_i128 a, b, x;
+                if (pfa_mul_ovf(w[(i * n + j) * stride], pk, &a) ||
+                    pfa_mul_ovf(ik, w[(k * n + j) * stride], &b) || pfa_sub_ovf(a, b, &x))
+                    return 2;
+                w[(i * n + j) * stride] = x / prev;        // exact (Sylvester's identity)
+            }
+        }
+        prev = pk;
+    }
+    return 1;
+}
+
+// 1 nonsingular, 0 singular, 2 undecided.  One modular determinant settles
+// almost every nonsingular matrix; exact Bareiss settles low-rank ones;
+// otherwise determinants modulo further primes until their product
+// exceeds the Hadamard bound.
 PFA_HD static inline int pfa_nonsingular(const int64_t *A, int n, const PfaModP *primes, int np,
-                                         uint32_t *w, int64_t stride)
+                                         uint32_t *w, pfa_i128 *bw, int64_t stride)
 {
     double need = pfa_hadamard_log2(A, n, stride);
     if (need < -1e299) return 0;
-    double have = 0.0;
-    for (int k = 0; k < np; ++k) {
+    if (np < 1) return 2;
+    if (pfa_det_mod(A, n, primes[0], w, stride) != 0) return 1;
+    int r = pfa_bareiss_decide(A, n, bw, stride);
+    if (r != 2) return r;
+    double have = log2((double)primes[0].p) - 1e-9;
+    if (have > need) return 0;
+    for (int k = 1; k < np; ++k) {
         if (pfa_det_mod(A, n, primes[k], w, stride) != 0) return 1;
         have += log2((double)primes[k].p) - 1e-9;
         if (have > need) return 0;

@@ -125,6 +125,9 @@ def _load():
     lib.pfa_device_cus.argtypes = [ctypes.c_int]
     lib.pfa_device_cus.restype = ctypes.c_int
     lib.pfa_arch.restype = ctypes.c_char_p
+    lib.pfa_det_filter.argtypes = [ctypes.c_int, _i32p, _i64p, ctypes.c_int64, _i32p, _i64p,
+                                   ctypes.c_int, ctypes.c_void_p, ctypes.c_char_p]
+    lib.pfa_det_filter.restype = ctypes.c_int
     _lib, _lib_path = lib, path
     return lib
 
@@ -350,3 +353,28 @@ def kernel_batch(problems, mode="plain", device=0):
         sel = pidx == k
         res.append((pts[sel, :dims[k]], qs[sel]))
     return res, status, nodes
+
+
+def det_filter(kappas, M, geo=None, device=0):
+    """det((kappa M)[1:, 1:]) != 0 for each row of M (the exact filter of
+    pfvs/coni.py:397-407): int8 array, 1 nonsingular, 0 singular, 2
+    undecided.  kappas: one (h, h, h) int array per geometry; M: (count, h)
+    rows (padded with zeros to the largest h); geo: geometry of each row
+    (default 0).  device < 0 runs the host code."""
+    lib = require()
+    hs = np.array([np.asarray(k).shape[0] for k in kappas], dtype=np.int32)
+    kap = np.ascontiguousarray(np.concatenate([np.asarray(k, dtype=np.int64).ravel() for k in kappas]))
+    hmax = int(hs.max())
+    M = np.asarray(M, dtype=np.int64)
+    count = M.shape[0]
+    Mp = np.zeros((count, hmax), dtype=np.int64)
+    Mp[:, :M.shape[1]] = M
+    g = np.zeros(count, np.int32) if geo is None else np.ascontiguousarray(geo, dtype=np.int32)
+    out = np.zeros(max(count, 1), dtype=np.int8)
+    err = ctypes.create_string_buffer(256)
+    rc = lib.pfa_det_filter(len(kappas), hs.ctypes.data_as(_i32p), _p64(kap), count,
+                            g.ctypes.data_as(_i32p), _p64(Mp), int(device),
+                            out.ctypes.data_as(ctypes.c_void_p), err)
+    if rc != 0:
+        raise RuntimeError("det filter failed: " + err.value.decode(errors="replace"))
+    return out[:count]
