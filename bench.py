@@ -7,6 +7,16 @@ sample of the same p-vectors.
 One step = one pass of the device pipeline (preparation, frontier roots,
 search, post-processing with the exact filter) over the whole batch, inputs
 resident in HBM.  Prints one JSON line.
+
+Roofline of the search kernel (k_batched): VALU issue.  Its algorithmic
+work W per launch is counted in lane-ops on the host path, which visits the
+same nodes (asserted):
+  W = sum_nodes [4k + 4k [i >= m0_level] + 4k r + 24]
+    + sum_cand [12 + r (4 + 5 s)] + sum_leaf [4 d (d + 1) / 2 + 8]
+(k = d - 1 - i coordinates set above a node at level i, r = 1 where an H
+row leads at column i, s = binary-gcd iterations), and divided by the
+kernel's duration from HIP events on the stream it runs on; the peak is
+CUs x 64 lanes x peak clock.
 """
 
 import argparse
@@ -50,6 +60,29 @@ def time_mode(gpu, geoms, pf, pgeo, mode, steps, warmup, **kw):
     return wall, stats, r
 
 
+def search_work(gpu, geoms, pf, pgeo):
+    """Algorithmic lane-ops of the search over the batch (host path, count
+    only), with its node count and the threads used."""
+    threads = max(1, min(len(os.sched_getaffinity(0)), 16))
+    r = gpu.run(geoms, pf, pgeo, mode="host", host_threads=threads, keep_points=False,
+                filter=False)
+    return r["stats"]["work"], r["stats"]["nodes"], threads
+
+
+def measured_traffic(kernel="k_batched"):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated with a copy kernel), or
+    None."""
+    path = os.path.join(ROOT, "profiles", "k_batched_h20_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(inst, pf, dil, budget_s=15.0):
     """The oracle (single thread) over p-vectors of the same batch, in a
     fixed seeded order, until budget_s seconds have elapsed."""
@@ -78,6 +111,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only time a plain device copy of MIB MiB (the reference "
+                         "kernel for HBM traffic counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1"}))
@@ -85,6 +121,12 @@ def main():
     from pfvs_amd import gpu
     if not gpu.available():
         raise SystemExit("HIP backend unavailable: " + gpu.unavailable_reason())
+    if args.copy_probe:
+        nb = args.copy_probe << 20
+        ms = gpu.copy_probe(nb, reps=args.steps)
+        print(json.dumps({"copy_bytes": nb, "copies": args.steps + 1, "ms_per_copy": ms,
+                          "GB_per_s": 2 * nb / (ms * 1e-3) / 1e9}))
+        return
     inst, geoms, pf, pgeo, dil = workload(args.n_per)
     print(f"# workload: G(20,250,5) {len(pf)} p-vectors", file=sys.stderr, flush=True)
     wall_b, st_b, r_b = time_mode(gpu, geoms, pf, pgeo, "batched", args.steps, args.warmup)
@@ -114,6 +156,17 @@ def main():
                    "coniPFVs_per_s": st_b[-1]["expansions"] / (ms_dev * 1e-3),
                    "wall_ms": wall_b * 1e3},
     }
+    work, wnodes, wthreads = search_work(gpu, geoms, pf, pgeo)
+    assert wnodes == nodes, "host count and device search visited different nodes"
+    cus, clk_khz = gpu.device_facts(0)
+    peak = cus * 64 * clk_khz * 1e3                 # lane-ops/s
+    achieved = work / (ms_search * 1e-3)
+    out["roofline"] = {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
+                       "unit": "T lane-op/s", "frac": achieved / peak,
+                       "traffic": measured_traffic("k_batched"),
+                       "kernel": "k_batched", "work_per_launch": work,
+                       "ms_per_launch": ms_search,
+                       "peak_basis": f"{cus} CUs x 64 lanes x {clk_khz / 1e6:.2f} GHz"}
     if not args.no_plain:
         wall_p, st_p, _ = time_mode(gpu, geoms, pf, pgeo, "plain", 1, 0)
         ms_p = st_p[-1]["ms_total"]

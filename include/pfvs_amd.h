@@ -105,7 +105,10 @@ enum {
     PFA_ST_MS_PREP, PFA_ST_MS_ROOTS, PFA_ST_MS_SEARCH, PFA_ST_MS_POST, PFA_ST_MS_TOTAL,
     PFA_ST_SEARCH_LAUNCHES, PFA_ST_HOST_P, PFA_ST_QUEUE_PEAK, PFA_ST_MS_WALL,
     PFA_ST_KEPT, PFA_ST_EXPANSIONS, PFA_ST_NONSING, PFA_ST_UNDECIDED,
-    PFA_ST_STEPS, PFA_ST_WAIT_CYC, PFA_ST_WAVE_CYC, PFA_ST_DONATED, PFA_ST_FETCHES, PFA_ST_N
+    PFA_ST_STEPS, PFA_ST_WAIT_CYC, PFA_ST_WAVE_CYC, PFA_ST_DONATED, PFA_ST_FETCHES,
+    /* host mode only: terms of the algorithmic work (lane-ops) of the search */
+    PFA_ST_SUM_K, PFA_ST_SUM_K_M0, PFA_ST_SUM_K_R, PFA_ST_SUM_R, PFA_ST_SUM_S, PFA_ST_SUM_LEAF,
+    PFA_ST_WORK, PFA_ST_N
 };
 
 typedef struct {
@@ -163,9 +166,16 @@ void pfa_kernel_output_free(pfa_kernel_output *out);
 int pfa_det_filter(int n_geo, const int32_t *h, const int64_t *kappa, int64_t count,
                    const int32_t *geo, const int64_t *M, int device, int8_t *out, char *err);
 
-/* Library facts: number of compute units of `device` and the build's
- * architecture string. */
+/* Copies `bytes` (rounded down to 16) between two fresh device buffers
+ * with a plain copy kernel, `reps` timed times after one untimed;
+ * *ms_per_copy gets the mean.  The reference point for HBM traffic
+ * counters.  No counterpart in the reference.  Returns 0 or 1. */
+int pfa_copy_probe(int device, int64_t bytes, int reps, double *ms_per_copy);
+
+/* Library facts: number of compute units and peak engine clock (kHz) of
+ * `device`, and the build's architecture string. */
 int pfa_device_cus(int device);
+int pfa_device_clock_khz(int device);
 const char *pfa_arch(void);
 
 #ifdef __cplusplus

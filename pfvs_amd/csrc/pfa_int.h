@@ -95,7 +95,9 @@ PFA_HD static inline uint64_t pfa_gcd64(uint64_t u, uint64_t v)
 // gcd(u, v) if it is >= need, else 0 (u, v not both 0).  Once the odd part
 // of the smaller operand drops below ceil(need / 2^shift) the gcd cannot
 // reach need (the same early exit as fpk_gcd64_ge, fpk_common.h:113-128).
-PFA_HD static inline uint64_t pfa_gcd64_ge(uint64_t u, uint64_t v, uint64_t need)
+template <bool CNT>
+PFA_HD static inline uint64_t pfa_gcd64_ge_t(uint64_t u, uint64_t v, uint64_t need,
+                                             unsigned long long *iters)
 {
     if (u == 0) return v >= need ? v : 0;
     if (v == 0) return u >= need ? u : 0;
@@ -103,12 +105,18 @@ PFA_HD static inline uint64_t pfa_gcd64_ge(uint64_t u, uint64_t v, uint64_t need
     uint64_t ns = (need >> sh) + ((need & ((1ULL << sh) - 1)) != 0);
     u >>= pfa_ctz64(u);
     do {
+        if (CNT) ++*iters;
         v >>= pfa_ctz64(v);
         if (u > v) { uint64_t t = v; v = u; u = t; }
         if (u < ns) return 0;
         v -= u;
     } while (v);
     return u << sh;
+}
+
+PFA_HD static inline uint64_t pfa_gcd64_ge(uint64_t u, uint64_t v, uint64_t need)
+{
+    return pfa_gcd64_ge_t<false>(u, v, need, nullptr);
 }
 
 PFA_HD static inline pfa_u128 pfa_gcd128(pfa_u128 u, pfa_u128 v)

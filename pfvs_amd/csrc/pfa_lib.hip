@@ -1630,6 +1630,9 @@ static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
     nth = std::max(1, std::min<int>(nth, (int)std::max<int64_t>(n_p, 1)));
     std::vector<HostPts> tp(nth);
     std::vector<PfaCounts> tc(nth, PfaCounts{0, 0, 0});
+    std::vector<PfaWork> tw(nth, PfaWork{0, 0, 0, 0, 0, 0});
+    std::vector<int64_t> tn(nth, 0);                 // points found per thread
+    const bool store = in->keep_points || in->filter; // else counts only
     std::atomic<int64_t> next{0};
     auto worker = [&](int t) {
         std::vector<char> ws(pfa_ws_bytes(D + 1) + 64);
@@ -1645,18 +1648,23 @@ static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
             PfaProb P = rec_view(rec, L);
             HostPts &hp = tp[t];
             size_t mark = hp.q.size();
+            int64_t found = 0;
             auto emit = [&](const int32_t *cc, int64_t q) -> int {
+                found++;
+                if (!store) return 0;
                 hp.pidx.push_back((int32_t)ip);
                 hp.q.push_back(q);
                 for (int j = 0; j < D; ++j) hp.c.push_back(j < P.d ? cc[j] : 0);
                 return 0;
             };
-            int r = pfa_dfs(P, st.data(), c, tc[t], emit);
+            int r = pfa_dfs_t<true>(P, st.data(), c, tc[t], emit, &tw[t]);
             if (r) {
                 rec_status(rec) = r;
                 hp.pidx.resize(mark);
                 hp.q.resize(mark);
                 hp.c.resize(mark * D);
+            } else {
+                tn[t] += found;
             }
         }
     };
@@ -1677,13 +1685,17 @@ static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
     std::vector<int8_t> ns(n, 2), bad(std::max<int64_t>(n_p, 1), 0);
     PostArgs Pa{n, all.pidx.data(), all.q.data(), all.c.data(), D, HS, recs.data(), L, M.data(),
                 Kn.data(), nexp.data(), bad.data(), in->filter};
-    for (int64_t k = 0; k < n; ++k) post_one(Pa, k);
-    if (in->filter) {
-        static std::vector<PfaModP> pr = primes31(96);
+    static const std::vector<PfaModP> pr = primes31(96);
+    // M, Kn, the integrality count and the det filter, in contiguous chunks
+    // over the host threads (a point's bad flag is a plain store of 1)
+    auto post_worker = [&](int t) {
+        const int64_t k0 = n * t / nth, k1 = n * (t + 1) / nth;
         std::vector<int64_t> N((HS - 1) * (HS - 1));
         std::vector<uint32_t> w((HS - 1) * (HS - 1));
         std::vector<pfa_i128> bw((HS - 1) * (HS - 1));
-        for (int64_t k = 0; k < n; ++k) {
+        for (int64_t k = k0; k < k1; ++k) {
+            post_one(Pa, k);
+            if (!in->filter) continue;
             if (nexp[k] <= 0) { ns[k] = 0; continue; }
             const int64_t *rec = recs.data() + (int64_t)all.pidx[k] * L.words;
             const int32_t *ri = (const int32_t *)rec;
@@ -1692,6 +1704,11 @@ static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
             ns[k] = (int8_t)pfa_nonsingular(N.data(), h - 1, pr.data(), (int)pr.size(), w.data(),
                                             bw.data(), 1);
         }
+    };
+    {
+        std::vector<std::thread> pth;
+        for (int t = 0; t < nth; ++t) pth.emplace_back(post_worker, t);
+        for (auto &x : pth) x.join();
     }
     std::vector<int32_t> status(n_p), code(n_p);
     for (int64_t ip = 0; ip < n_p; ++ip) {
@@ -1701,6 +1718,23 @@ static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
     }
     PfaCounts tot{0, 0, 0};
     for (auto &x : tc) { tot.nodes += x.nodes; tot.cand += x.cand; tot.leaf += x.leaf; }
+    PfaWork wk{0, 0, 0, 0, 0, 0};
+    for (auto &x : tw) {
+        wk.sk += x.sk; wk.skm0 += x.skm0; wk.skr += x.skr;
+        wk.sr += x.sr; wk.ss += x.ss; wk.sleaf += x.sleaf;
+    }
+    out->stats[PFA_ST_SUM_K] = (double)wk.sk;
+    out->stats[PFA_ST_SUM_K_M0] = (double)wk.skm0;
+    out->stats[PFA_ST_SUM_K_R] = (double)wk.skr;
+    out->stats[PFA_ST_SUM_R] = (double)wk.sr;
+    out->stats[PFA_ST_SUM_S] = (double)wk.ss;
+    out->stats[PFA_ST_SUM_LEAF] = (double)wk.sleaf;
+    // lane-ops: 4 per coordinate of off/aoff, M0 bound and H prefix updates
+    // plus 24 per node; 12 per candidate plus 4 + 5 s where an H row leads;
+    // 4 per entry of the leaf's exact form plus 8
+    out->stats[PFA_ST_WORK] = 4.0 * (double)(wk.sk + wk.skm0 + wk.skr) + 24.0 * (double)tot.nodes +
+                              12.0 * (double)tot.cand + 4.0 * (double)wk.sr + 5.0 * (double)wk.ss +
+                              4.0 * (double)wk.sleaf + 8.0 * (double)tot.leaf;
     out->stats[PFA_ST_NODES] = (double)tot.nodes;
     out->stats[PFA_ST_CAND] = (double)tot.cand;
     out->stats[PFA_ST_LEAF] = (double)tot.leaf;
@@ -1709,6 +1743,11 @@ static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
     int rc = finish_run(in, out, n, HS, all.pidx.data(), all.q.data(), M.data(), Kn.data(),
                         in->filter ? nexp.data() : nullptr, in->filter ? ns.data() : nullptr,
                         status.data(), code.data(), bad.data(), errbuf);
+    if (!store) {
+        int64_t np = 0;
+        for (int64_t x : tn) np += x;
+        out->stats[PFA_ST_POINTS] = (double)np;
+    }
     out->stats[PFA_ST_MS_WALL] =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     out->stats[PFA_ST_MS_TOTAL] = out->stats[PFA_ST_MS_WALL];
@@ -1823,9 +1862,59 @@ extern "C" int pfa_device_cus(int device)
     return prop.multiProcessorCount;
 }
 
+// A plain copy of 16-byte words: the reference point for HBM counters.
+__global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n16)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+extern "C" int pfa_copy_probe(int device, int64_t bytes, int reps, double *ms_per_copy)
+{
+    char errbuf[256];
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 1;
+    try {
+        const int64_t n16 = bytes / 16;
+        if (n16 <= 0 || reps <= 0) return 1;
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipMalloc(&a, 16 * n16));
+        HIPCHK(hipMalloc(&b, 16 * n16));
+        HIPCHK(hipMemset(a, 1, 16 * n16));
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        const int grid = (int)std::min<int64_t>((n16 + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, (const uint4 *)a, (uint4 *)b, n16);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(e0, 0));
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, (const uint4 *)a, (uint4 *)b, n16);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(e1, 0));
+        HIPCHK(hipEventSynchronize(e1));
+        *ms_per_copy = ms_between(e0, e1) / reps;
+        rc = 0;
+    } catch (int) {
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}
+
+extern "C" int pfa_device_clock_khz(int device)
+{
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
+    return prop.clockRate;
+}
+
 extern "C" const char *pfa_arch(void) { return "gfx950"; }
 
-// ------------------------------------------------- single-problem batch
+// ------------------------------------------------------- exact det filter
 extern "C" int pfa_det_filter(int n_geo, const int32_t *h, const int64_t *kappa, int64_t count,
                               const int32_t *geo, const int64_t *M, int device, int8_t *out,
                               char *err)

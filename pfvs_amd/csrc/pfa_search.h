@@ -63,6 +63,15 @@ struct PfaCounts {
     unsigned long long nodes, cand, leaf;
 };
 
+// Terms of the algorithmic work of a search (host path only), with
+// k = d - 1 - i the coordinates set above a node at level i, r = 1 if an H
+// row leads at column i, s = iterations of the binary gcd:
+//   sk = sum_nodes k, skm0 = sum_nodes k [i >= m0_level], skr = sum_nodes k r,
+//   sr = sum_cand r, ss = sum_cand s, sleaf = sum_leaf d (d + 1) / 2.
+struct PfaWork {
+    unsigned long long sk, skm0, skr, sr, ss, sleaf;
+};
+
 // fpk_search_impl.h:43-51: smallest gcd passing the cut when q >= q_lb,
 // rounded down so it never demands more than the exact cut.
 PFA_HD static inline uint64_t pfa_need(double q_lb, int64_t Q, int strict)
@@ -163,7 +172,9 @@ PFA_HD static inline int pfa_enter(const PfaProb &P, const int32_t *c, PfaNode &
 // state (level i-1, or the leaf when i == 0) and return 1; 0 if cut;
 // PFA_CAND_OVF if exact int128 arithmetic overflowed.
 #define PFA_CAND_OVF 2
-PFA_HD static inline int pfa_cand(const PfaProb &P, const PfaNode &nd, int32_t v, PfaNode &ch)
+template <bool CNT = false>
+PFA_HD static inline int pfa_cand(const PfaProb &P, const PfaNode &nd, int32_t v, PfaNode &ch,
+                                  unsigned long long *gcd_iters = nullptr)
 {
     const int i = nd.i, d = P.d;
     const double slack = P.slack, Kerr = P.Kerr;
@@ -185,7 +196,7 @@ PFA_HD static inline int pfa_cand(const PfaProb &P, const PfaNode &nd, int32_t v
             pfa_u128 a = pfa_abs128(nd.pre + (pfa_i128)P.H[(size_t)i * d + i] * v);
             if ((gn >> 64) == 0 && (a >> 64) == 0 && (gn | a) != 0) {
                 uint64_t need = pfa_need(q_lb, P.Q, P.strict);
-                uint64_t gg = pfa_gcd64_ge((uint64_t)gn, (uint64_t)a, need);
+                uint64_t gg = pfa_gcd64_ge_t<CNT>((uint64_t)gn, (uint64_t)a, need, gcd_iters);
                 if (!gg) return 0;
                 gn = gg;
             } else {
@@ -291,18 +302,28 @@ PFA_HD static inline int pfa_trivially_empty(const PfaProb &P)
 
 // The whole search of one problem, depth first, in the reference's order:
 // the plain form (one thread per problem) and the host path.  emit(c, q)
-// returns nonzero to stop (-2).  Returns a status.
-template <class Emit>
-PFA_HD static inline int pfa_dfs(const PfaProb &P, PfaNode *st, int32_t *c, PfaCounts &cnt,
-                                 Emit &emit)
+// returns nonzero to stop (-2).  Returns a status.  With CNT the terms of
+// the algorithmic work are added to *wk.
+template <bool CNT, class Emit>
+PFA_HD static inline int pfa_dfs_t(const PfaProb &P, PfaNode *st, int32_t *c, PfaCounts &cnt,
+                                   Emit &emit, PfaWork *wk)
 {
     const int d = P.d;
     if (pfa_trivially_empty(P)) return 0;
     for (int j = 0; j < d; ++j) c[j] = 0;
+    auto entered = [&](int lev) {
+        cnt.nodes++;
+        if (CNT) {
+            const unsigned long long k = (unsigned long long)(d - 1 - lev);
+            wk->sk += k;
+            if (P.m0_level >= 0 && lev >= P.m0_level) wk->skm0 += k;
+            if ((P.hmask >> lev) & 1) wk->skr += k;
+        }
+    };
     int i = d - 1;
     pfa_root(P, st[i]);
     if (pfa_enter(P, c, st[i])) return PFA_I32;
-    cnt.nodes++;
+    entered(i);
     while (i < d) {
         PfaNode &nd = st[i];
         if (nd.cur > nd.hi) { c[i] = 0; i++; continue; }
@@ -310,7 +331,13 @@ PFA_HD static inline int pfa_dfs(const PfaProb &P, PfaNode *st, int32_t *c, PfaC
         c[i] = v;
         cnt.cand++;
         PfaNode ch;
-        int r = pfa_cand(P, nd, v, ch);
+        int r;
+        if (CNT) {
+            wk->sr += (P.hmask >> i) & 1;
+            r = pfa_cand<true>(P, nd, v, ch, &wk->ss);
+        } else {
+            r = pfa_cand(P, nd, v, ch);
+        }
         if (r == 0) continue;
         if (r == PFA_CAND_OVF) return PFA_HOST;
         if (i > 0) {
@@ -322,14 +349,22 @@ PFA_HD static inline int pfa_dfs(const PfaProb &P, PfaNode *st, int32_t *c, PfaC
             st[i - 1].qp = ch.qp;
             i--;
             if (pfa_enter(P, c, st[i])) return PFA_I32;
-            cnt.nodes++;
+            entered(i);
             continue;
         }
         cnt.leaf++;
+        if (CNT) wk->sleaf += (unsigned long long)d * (d + 1) / 2;
         int64_t q;
         int lr = pfa_leaf(P, ch, c, &q);
         if (lr == PFA_CAND_OVF) return PFA_HOST;
         if (lr == 1 && emit(c, q)) return PFA_TOO_MANY;
     }
     return 0;
+}
+
+template <class Emit>
+PFA_HD static inline int pfa_dfs(const PfaProb &P, PfaNode *st, int32_t *c, PfaCounts &cnt,
+                                 Emit &emit)
+{
+    return pfa_dfs_t<false>(P, st, c, cnt, emit, (PfaWork *)nullptr);
 }

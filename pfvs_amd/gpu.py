@@ -30,7 +30,8 @@ _MODES = {"plain": MODE_PLAIN, "batched": MODE_BATCHED, "host": MODE_HOST}
 STAT_NAMES = ["nodes", "cand", "leaf", "points", "passes", "ms_prep", "ms_roots", "ms_search",
               "ms_post", "ms_total", "search_launches", "host_p", "queue_peak", "ms_wall",
               "kept", "expansions", "nonsingular", "undecided", "steps", "wait_cycles",
-              "wave_cycles", "donated", "fetches"]
+              "wave_cycles", "donated", "fetches", "sum_k", "sum_k_m0", "sum_k_r", "sum_r",
+              "sum_s", "sum_leaf", "work"]
 
 
 class _PfgInput(ctypes.Structure):
@@ -125,6 +126,11 @@ def _load():
     lib.pfa_device_cus.argtypes = [ctypes.c_int]
     lib.pfa_device_cus.restype = ctypes.c_int
     lib.pfa_arch.restype = ctypes.c_char_p
+    lib.pfa_copy_probe.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_double)]
+    lib.pfa_copy_probe.restype = ctypes.c_int
+    lib.pfa_device_clock_khz.argtypes = [ctypes.c_int]
+    lib.pfa_device_clock_khz.restype = ctypes.c_int
     lib.pfa_det_filter.argtypes = [ctypes.c_int, _i32p, _i64p, ctypes.c_int64, _i32p, _i64p,
                                    ctypes.c_int, ctypes.c_void_p, ctypes.c_char_p]
     lib.pfa_det_filter.restype = ctypes.c_int
@@ -168,6 +174,21 @@ def release(device: int = 0) -> None:
     lib = _load()
     if lib is not None:
         lib.pfg_release(int(device))
+
+
+def device_facts(device: int = 0):
+    """(compute units, peak engine clock in kHz) of `device`."""
+    lib = require()
+    return int(lib.pfa_device_cus(device)), int(lib.pfa_device_clock_khz(device))
+
+
+def copy_probe(nbytes, reps=10, device=0):
+    """Mean milliseconds of a plain device copy of nbytes (pfa_copy_probe)."""
+    lib = require()
+    ms = ctypes.c_double(0.0)
+    if lib.pfa_copy_probe(int(device), int(nbytes), int(reps), ctypes.byref(ms)) != 0:
+        raise RuntimeError("copy probe failed")
+    return ms.value
 
 
 def max_h11() -> int:
