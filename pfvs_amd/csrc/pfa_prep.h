@@ -138,11 +138,34 @@ PFA_HD static inline int pfa_unimodular(const PfaTeam &T, const int64_t *v, int 
     return pfa_team_any(T, flag, bad) ? -1 : 0;
 }
 
-// Gram-Schmidt row kk from the exact Gram matrix (every lane, same order)
-PFA_HD static inline void pfa_gs_row(const PfaWS &W, int kk, int L, const pfa_i128 *GG)
+// Gram-Schmidt row kk (mu[kk][0:kk], bb[kk]) from the exact Gram matrix:
+//   mu[kk][j] = (G[kk][j] - sum_{l<j} mu[j][l] mu[kk][l] bb[l]) / bb[j],
+//   bb[kk]    =  G[kk][kk] - sum_{l<kk} mu[kk][l] mu[kk][l] bb[l].
+// On the device the lanes run it right-looking: lane j < kk keeps the
+// running numerator of column j, lane kk that of bb[kk]; at step l lane l's
+// value is final and is broadcast.  Each lane subtracts its terms in the
+// host's order (l ascending) with the host's association, so every value is
+// bit-identical to the sequential host loop.
+PFA_HD static inline void pfa_gs_row(const PfaTeam &T, const PfaWS &W, int kk, int L,
+                                     const pfa_i128 *GG)
 {
     double *mu = W.mu, *bb = W.bb;
     const int S = W.h;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (T.n > kk) {
+        const int j = T.lane;
+        double r = j <= kk ? (double)GG[kk * L + j] : 0.0;
+        for (int l = 0; l < kk; ++l) {
+            const double m = __shfl(r, l) / bb[l];
+            if (j == l) mu[kk * S + l] = m;
+            else if (j > l && j < kk) r -= mu[j * S + l] * m * bb[l];
+            else if (j == kk) r -= m * m * bb[l];
+        }
+        if (j == kk) bb[kk] = r;
+        PFA_SYNC();
+        return;
+    }
+#endif
     for (int j = 0; j < kk; ++j) {
         double dd = (double)GG[kk * L + j];
         for (int l = 0; l < j; ++l) dd -= mu[j * S + l] * mu[kk * S + l] * bb[l];
@@ -150,7 +173,37 @@ PFA_HD static inline void pfa_gs_row(const PfaWS &W, int kk, int L, const pfa_i1
     }
     double dd = (double)GG[kk * L + kk];
     for (int l = 0; l < kk; ++l) dd -= mu[kk * S + l] * mu[kk * S + l] * bb[l];
-    bb[kk] = dd;
+    if (T.lane == 0) bb[kk] = dd;
+    PFA_SYNC();
+}
+
+// Coefficients mw[0:m] of the row `last` (row m of GG) on the reduced
+// basis: mw[j] = (G[m][j] - sum_{l<j} mu[j][l] mw[l] bb[l]) / bb[j], right-
+// looking over the lanes on the device as pfa_gs_row (bit-identical).
+PFA_HD static inline void pfa_gs_last(const PfaTeam &T, const PfaWS &W, int m, int L,
+                                      const pfa_i128 *GG)
+{
+    double *mu = W.mu, *bb = W.bb, *mw = W.mw;
+    const int S = W.h;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (T.n >= m) {
+        const int j = T.lane;
+        double r = j < m ? (double)GG[m * L + j] : 0.0;
+        for (int l = 0; l < m; ++l) {
+            const double w = __shfl(r, l) / bb[l];
+            if (j == l) mw[l] = w;
+            else if (j > l && j < m) r -= mu[j * S + l] * w * bb[l];
+        }
+        PFA_SYNC();
+        return;
+    }
+#endif
+    for (int j = 0; j < m; ++j) {
+        double dd = (double)GG[m * L + j];
+        for (int l = 0; l < j; ++l) dd -= mu[j * S + l] * mw[l] * bb[l];
+        if (T.lane == 0) mw[j] = dd / bb[j];
+        PFA_SYNC();
+    }
 }
 
 // b_k -= q b_j on the rows (columns split over lanes) and on GG (exact).
@@ -195,10 +248,10 @@ PFA_HD static inline int pfa_lll_core(const PfaTeam &T, const PfaWS &W, int64_t 
     int iters = 0;
     pfa_team_clear(T, W.flag);
     if (m >= 1) {
-        pfa_gs_row(W, 0, L, GG);
+        pfa_gs_row(T, W, 0, L, GG);
         if (!(bb[0] > 0)) return -2;
     }
-    if (m >= 2) pfa_gs_row(W, 1, L, GG);
+    if (m >= 2) pfa_gs_row(T, W, 1, L, GG);
     PFA_SYNC();
     int k = 1;
     while (k < m) {
@@ -210,13 +263,13 @@ PFA_HD static inline int pfa_lll_core(const PfaTeam &T, const PfaWS &W, int64_t 
                 if (q == 0.0) continue;
                 if (!(fabs(q) <= 9.0e18)) return -1;
                 if (pfa_lll_sub(T, W, B, last, m, n, GG, L, M, k, j, (int64_t)q)) return -1;
-                for (int l = 0; l < j; ++l) mu[k * S + l] -= q * mu[j * S + l];
-                mu[k * S + j] -= q;
+                for (int l = T.lane; l < j; l += T.n) mu[k * S + l] -= q * mu[j * S + l];
+                if (T.lane == 0) mu[k * S + j] -= q;
                 changed = 1;
                 PFA_SYNC();
             }
             if (!changed) break;
-            pfa_gs_row(W, k, L, GG);
+            pfa_gs_row(T, W, k, L, GG);
             PFA_SYNC();
         }
         if (!(bb[k] > 0)) return -2;
@@ -239,32 +292,27 @@ PFA_HD static inline int pfa_lll_core(const PfaTeam &T, const PfaWS &W, int64_t 
             }
             PFA_SYNC();
             k = k > 1 ? k - 1 : 1;
-            if (k == 1) pfa_gs_row(W, 0, L, GG);
-            pfa_gs_row(W, k - 1, L, GG);
-            pfa_gs_row(W, k, L, GG);
+            if (k == 1) pfa_gs_row(T, W, 0, L, GG);
+            pfa_gs_row(T, W, k - 1, L, GG);
+            pfa_gs_row(T, W, k, L, GG);
             PFA_SYNC();
         } else {
             k++;
-            if (k < m) pfa_gs_row(W, k, L, GG);
+            if (k < m) pfa_gs_row(T, W, k, L, GG);
             PFA_SYNC();
         }
     }
     if (last && m > 0) {
         for (int pass = 0; pass < 4; ++pass) {
-            for (int j = 0; j < m; ++j) {
-                double dd = (double)GG[m * L + j];
-                for (int l = 0; l < j; ++l) dd -= mu[j * S + l] * mw[l] * bb[l];
-                mw[j] = dd / bb[j];
-            }
-            PFA_SYNC();
+            pfa_gs_last(T, W, m, L, GG);
             int changed = 0;
             for (int j = m - 1; j >= 0; --j) {
                 double q = nearbyint(mw[j]);
                 if (q == 0.0) continue;
                 if (!(fabs(q) <= 9.0e18)) return -1;
                 if (pfa_lll_sub(T, W, B, last, m, n, GG, L, M, m, j, (int64_t)q)) return -1;
-                for (int l = 0; l < j; ++l) mw[l] -= q * mu[j * S + l];
-                mw[j] -= q;
+                for (int l = T.lane; l < j; l += T.n) mw[l] -= q * mu[j * S + l];
+                if (T.lane == 0) mw[j] -= q;
                 changed = 1;
                 PFA_SYNC();
             }
