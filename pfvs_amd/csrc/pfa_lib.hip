@@ -1,7 +1,8 @@
 // pfa_lib.hip -- libpfvs_amd.so: the coniZpM search on MI355X (gfx950).
 //
 // Pipeline per call (one HIP stream, inputs uploaded once):
-//   k_prep      one wavefront (64 lanes, LDS working set) per p-vector:
+//   k_prep      one wavefront (64 lanes; the LLL/HNF working set in LDS,
+//               the matrix products' in global memory) per p-vector:
 //               lattice, HNF, FP64 Cholesky with its error margin, record.
 //   k_roots     one lane per p-vector: top-level node -> frontier queue.
 //   search      PLAIN: k_plain, one lane per p-vector runs the whole
@@ -116,7 +117,7 @@ struct GeoArgs {
 };
 
 // --------------------------------------------------- preparation (team)
-PFA_HD static inline void prep_one(const PfaTeam &T, char *wsmem, const GeoArgs &G,
+PFA_HD static inline void prep_one(const PfaTeam &T, char *ws_hot, char *ws_cold, const GeoArgs &G,
                                    const int64_t *ps, int MH, const int32_t *pgeo, int64_t ip,
                                    int64_t *rec, RecL L)
 {
@@ -136,7 +137,7 @@ PFA_HD static inline void prep_one(const PfaTeam &T, char *wsmem, const GeoArgs 
         ri[26] = geo;
     }
     PfaWS W;
-    pfa_ws_carve(W, wsmem, h);
+    pfa_ws_carve(W, ws_hot, ws_cold, h);
     int st = pfa_build_lattice(T, W, G.kappa + G.kofs[geo], G.Mbasis + G.mofs[geo], p);
     if (st) {
         PFA_SYNC();
@@ -145,7 +146,7 @@ PFA_HD static inline void prep_one(const PfaTeam &T, char *wsmem, const GeoArgs 
     }
     // H = HNF(ZB[1:]) in the GG region; v, w, pivots in W1..W3
     pfa_i128 *H = W.GG;
-    int hst = pfa_hnf(T, W, W.ZB + d, d, d, H, (pfa_i128 *)W.W1, (pfa_i128 *)W.W2, (int *)W.W3);
+    int hst = pfa_hnf(T, W, W.ZB + d, d, d, H, W.hv, W.hw, W.hpiv);
     double *U = (double *)W.W1, *X = (double *)W.W2, phi = 0.0;
     PFA_SYNC();
     int cst = pfa_cholesky(T, W.mat, d, U, X, &phi, W.flag);
@@ -220,12 +221,14 @@ PFA_HD static inline void prep_one(const PfaTeam &T, char *wsmem, const GeoArgs 
 }
 
 __global__ __launch_bounds__(64) void k_prep(GeoArgs G, const int64_t *ps, int MH,
-                                             const int32_t *pgeo, int64_t n, int64_t *recs, RecL L)
+                                             const int32_t *pgeo, int64_t n, int64_t *recs, RecL L,
+                                             char *cold, size_t cold_bytes)
 {
     extern __shared__ __align__(16) char smem[];
     PfaTeam T{(int)threadIdx.x, 64};
+    char *mine = cold + (size_t)blockIdx.x * cold_bytes;
     for (int64_t ip = blockIdx.x; ip < n; ip += gridDim.x) {
-        prep_one(T, smem, G, ps, MH, pgeo, ip, recs + ip * L.words, L);
+        prep_one(T, smem, mine, G, ps, MH, pgeo, ip, recs + ip * L.words, L);
         __syncthreads();
     }
 }
@@ -1110,7 +1113,7 @@ static void *dev_get(DevCtx *C, int slot, size_t bytes, char *errbuf)
 enum {
     B_GEO_H, B_GEO_Q, B_GEO_QMAX, B_GEO_LIN, B_KAPPA, B_MBASIS, B_KOFS, B_MOFS, B_PS, B_PGEO,
     B_RECS, B_RING, B_SEQ, B_STACKS, B_CTL, B_PT_PIDX, B_PT_Q, B_PT_C, B_CNT, B_M, B_KN, B_NEXP, B_BAD,
-    B_PRIMES, B_KT, B_KTOFS, B_KTMAX, B_MODCNT, B_MODLIST, B_NONSING, B_NBUF
+    B_PRIMES, B_KT, B_KTOFS, B_KTMAX, B_MODCNT, B_MODLIST, B_NONSING, B_PREPCOLD, B_NBUF
 };
 
 // the k largest primes below 2^31 with their Barrett constants
@@ -1357,7 +1360,7 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     int64_t ktot, mtot;
     fill_geo_offsets(in, kofs, mofs, ktot, mtot);
     int hmax = D + 1;
-    size_t lds = pfa_ws_bytes(hmax);
+    size_t lds = pfa_ws_lds_bytes(hmax);
     if (lds > C->lds_max) {
         snprintf(errbuf, 256, "h11 = %d needs %zu B of LDS (device has %zu)", hmax, lds, C->lds_max);
         return 1;
@@ -1385,10 +1388,15 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipEventRecord(ev[0], s));
     if (n_p > 0) {
-        int grid = (int)std::min<int64_t>(n_p, 1 << 20);
+        // one workgroup per p-vector up to 16384 (each with its own cold
+        // workspace in global memory), looping beyond
+        const int grid = (int)std::min<int64_t>(n_p, 16384);
+        const size_t cold = pfa_ws_cold_bytes(hmax);
+        char *d_cold = (char *)dev_get(C, B_PREPCOLD, cold * grid, errbuf);
         HIPCHK(hipFuncSetAttribute((const void *)k_prep, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds));
-        hipLaunchKernelGGL(k_prep, dim3(grid), dim3(64), lds, s, G, d_ps, MH, d_pgeo, n_p, d_recs, L);
+        hipLaunchKernelGGL(k_prep, dim3(grid), dim3(64), lds, s, G, d_ps, MH, d_pgeo, n_p, d_recs, L,
+                           d_cold, cold);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(ev[1], s));
@@ -1643,7 +1651,8 @@ static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
             int64_t ip = next.fetch_add(1);
             if (ip >= n_p) break;
             int64_t *rec = recs.data() + ip * L.words;
-            prep_one(T, ws.data(), G, in->ps, MH, in->pgeo, ip, rec, L);
+            prep_one(T, ws.data(), ws.data() + pfa_ws_lds_bytes(D + 1), G, in->ps, MH, in->pgeo, ip,
+                     rec, L);
             if (rec_status(rec) != PFA_OK) continue;
             PfaProb P = rec_view(rec, L);
             HostPts &hp = tp[t];

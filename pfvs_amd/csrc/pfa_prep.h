@@ -32,52 +32,70 @@ struct PfaTeam {
     int lane, n;
 };
 
-// Working storage of one preparation (LDS on the device).  Sizes for h:
-//   Z, W1, W2, W3: h*h int64; Binter, ZB: h*(h-1); mat: (h-1)^2;
-//   GG: (h+1)^2 int128 (later the HNF); mu: h*h double; bb, mw, vec: h;
-//   flag: one int.
+// Working storage of one preparation.  The part the LLL and HNF loops
+// touch lives in LDS on the device ("hot"): W1 (h*h int64: the basis rows
+// being reduced), GG ((h+1)^2 int128: their exact Gram matrix, later the
+// HNF), mu (h*h double), bb, mw, vec (h), the HNF's row scratch hv, hw
+// (h int128) and pivots hpiv (h int), flag.  The part only O(h^3)
+// products touch ("cold") is per-workgroup global memory on the device:
+// Z, W2, W3 (h*h int64), Binter, ZB (h*(h-1)), mat ((h-1)^2).  On the
+// host both parts are one buffer.
 struct PfaWS {
     int h, d;
     int64_t *Z, *Bi, *ZB, *mat, *W1, *W2, *W3, *vec;
-    pfa_i128 *GG;
+    pfa_i128 *GG, *hv, *hw;
     double *mu, *bb, *mw;
-    int *flag;
+    int *flag, *hpiv;
 };
 
-// Bytes of a PfaWS for h (everything 16-byte aligned).
-PFA_HD static inline size_t pfa_ws_bytes(int h)
+#define PFA_WS_ROUND(x) (((size_t)(x) + 15) & ~(size_t)15)
+
+// Bytes of the hot (LDS) and cold (global) parts for h, 16-byte aligned.
+PFA_HD static inline size_t pfa_ws_lds_bytes(int h)
 {
-    size_t d = h - 1, w = 0;
-    w += (size_t)h * h * 8 * 4;          // Z, W1, W2, W3
-    w += (size_t)h * d * 8 * 2;          // Bi, ZB
-    w += d * d * 8;                      // mat
-    w += (size_t)(h + 1) * (h + 1) * 16; // GG / H
-    w += (size_t)h * h * 8;              // mu
-    w += (size_t)h * 8 * 3;              // bb, mw, vec
-    w += 64;                             // flag + padding
-    return (w + 15) & ~(size_t)15;
+    size_t w = 0;
+    w += PFA_WS_ROUND((size_t)(h + 1) * (h + 1) * 16);   // GG / H
+    w += PFA_WS_ROUND((size_t)h * h * 8) * 2;            // W1, mu
+    w += PFA_WS_ROUND((size_t)h * 8) * 3;                // bb, mw, vec
+    w += PFA_WS_ROUND((size_t)h * 16) * 2;               // hv, hw
+    w += PFA_WS_ROUND((size_t)h * 4);                    // hpiv
+    w += 16;                                             // flag
+    return w;
 }
 
-PFA_HD static inline void pfa_ws_carve(PfaWS &W, char *base, int h)
+PFA_HD static inline size_t pfa_ws_cold_bytes(int h)
+{
+    size_t d = h - 1, w = 0;
+    w += PFA_WS_ROUND((size_t)h * h * 8) * 3;            // Z, W2, W3
+    w += PFA_WS_ROUND((size_t)h * d * 8) * 2;            // Bi, ZB
+    w += PFA_WS_ROUND(d * d * 8);                        // mat
+    return w;
+}
+
+PFA_HD static inline size_t pfa_ws_bytes(int h) { return pfa_ws_lds_bytes(h) + pfa_ws_cold_bytes(h); }
+
+PFA_HD static inline void pfa_ws_carve(PfaWS &W, char *hot, char *cold, int h)
 {
     size_t d = h - 1;
-    char *p = base;
-    auto take = [&](size_t bytes) { char *r = p; p += (bytes + 15) & ~(size_t)15; return r; };
+    auto take = [](char *&p, size_t bytes) { char *r = p; p += PFA_WS_ROUND(bytes); return r; };
     W.h = h;
     W.d = (int)d;
-    W.GG = (pfa_i128 *)take((size_t)(h + 1) * (h + 1) * 16);
-    W.Z = (int64_t *)take((size_t)h * h * 8);
-    W.W1 = (int64_t *)take((size_t)h * h * 8);
-    W.W2 = (int64_t *)take((size_t)h * h * 8);
-    W.W3 = (int64_t *)take((size_t)h * h * 8);
-    W.Bi = (int64_t *)take((size_t)h * d * 8);
-    W.ZB = (int64_t *)take((size_t)h * d * 8);
-    W.mat = (int64_t *)take(d * d * 8);
-    W.mu = (double *)take((size_t)h * h * 8);
-    W.bb = (double *)take((size_t)h * 8);
-    W.mw = (double *)take((size_t)h * 8);
-    W.vec = (int64_t *)take((size_t)h * 8);
-    W.flag = (int *)take(16);
+    W.GG = (pfa_i128 *)take(hot, (size_t)(h + 1) * (h + 1) * 16);
+    W.W1 = (int64_t *)take(hot, (size_t)h * h * 8);
+    W.mu = (double *)take(hot, (size_t)h * h * 8);
+    W.bb = (double *)take(hot, (size_t)h * 8);
+    W.mw = (double *)take(hot, (size_t)h * 8);
+    W.vec = (int64_t *)take(hot, (size_t)h * 8);
+    W.hv = (pfa_i128 *)take(hot, (size_t)h * 16);
+    W.hw = (pfa_i128 *)take(hot, (size_t)h * 16);
+    W.hpiv = (int *)take(hot, (size_t)h * 4);
+    W.flag = (int *)take(hot, 16);
+    W.Z = (int64_t *)take(cold, (size_t)h * h * 8);
+    W.W2 = (int64_t *)take(cold, (size_t)h * h * 8);
+    W.W3 = (int64_t *)take(cold, (size_t)h * h * 8);
+    W.Bi = (int64_t *)take(cold, (size_t)h * d * 8);
+    W.ZB = (int64_t *)take(cold, (size_t)h * d * 8);
+    W.mat = (int64_t *)take(cold, d * d * 8);
 }
 
 // team-wide "any lane failed": lanes set *flag, then all read it
