@@ -715,13 +715,6 @@ PFA_HD static inline void post_one(const PostArgs &a, int64_t k)
     if (a.filter) a.nexp[k] = bad ? -1 : pfa_num_expansions(M, Kn, h, a.q[k], rec[2]);
 }
 
-__global__ void k_post(PostArgs a)
-{
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n;
-         k += (int64_t)gridDim.x * blockDim.x)
-        post_one(a, k);
-}
-
 // N = (kappa M)[1:, 1:] for a point (element e at N[e * stride]); 0, or 1 if
 // an entry leaves int64.  kappa is symmetric: N is too.
 PFA_HD static inline int kappa_M(const int64_t *kappa, int h, const int64_t *M, int64_t *N,
@@ -741,35 +734,52 @@ PFA_HD static inline int kappa_M(const int64_t *kappa, int h, const int64_t *M, 
     return 0;
 }
 
-// ------------------------------------------------------------------ k_det
-// det((kappa M)[1:, 1:]) != 0, exactly, for every point with n_exp > 0.
-// A wavefront holds R = 64 / nl points (nl: the largest n = h - 1 of the
-// launch); lane j of group r keeps column j of its point's N, rows
-// s < NMAX in registers.  k_det, per point:
+// ------------------------------------------------------------ k_post
+// One lane per point slot, single-wavefront blocks (64 consecutive slots):
+// Kn = ZB c and M = Binter c (only M[0] unless the rows are kept) in
+// int128, checked into int64 (a failure marks the p-vector for the exact
+// host path); K_gcd = gcd(Kn[1:]) and the number n_exp of (g, K0)
+// expansions (0: the K-integrality filter drops the point).  Kept rows are
+// staged in LDS and stored as one contiguous block per wavefront; points
+// with n_exp > 0 are appended to the list of the det test (one atomic per
+// wavefront).
+//
+// ------------------------------------------------------------ k_filter
+// det((kappa M)[1:, 1:]) != 0, exactly, for the listed points.  A
+// wavefront holds R = 64 / hl points (hl: the largest h of the launch);
+// lane i of a group computes M_i and owns column i of N (n = h - 1
+// columns, rows s < NMAX in registers).  Per point:
 //   1. N in double when max|kappa| * sum|M| < 2^52 (every product and
-//      partial sum is then an exact integer), from kappa as float (exact:
-//      |kappa| < 2^24 is required, else step 3), staged in LDS for the
-//      geometry of the block's points;
+//      partial sum is then an exact integer), from kappa as float (exact
+//      below 2^24, else step 3), staged in LDS for the geometry of the
+//      block's contiguous span of points;
 //   2. fraction-free elimination (Bareiss) in double, every product and
 //      difference checked below 2^53 so each value is the exact integer
 //      minor: a column without a pivot among the unused rows proves
-//      det = 0, n pivots prove det != 0.
-// Points that step 1 or 2 cannot settle go on a list for k_det_mod:
-//   3. N in int128 from the int64 kappa (an entry outside int64 leaves the
-//      point undecided, 2), then det modulo 31-bit primes with Montgomery
-//      products (rows are only ever scaled by units mod p, which keeps
-//      det != 0 invariant) until a residue is nonzero (det != 0) or the
-//      primes' product exceeds the columns' Hadamard bound (det = 0).
+//      det = 0, n pivots prove det != 0;
+//   3. points whose values reach 2^53 go on a second list (with their M)
+//      for k_det_mod: N in int128 from the int64 kappa (an entry outside
+//      int64 leaves the point undecided, 2), then det modulo 31-bit primes
+//      with Montgomery products (rows are only scaled by units mod p, which
+//      keeps det != 0 invariant) until a residue is nonzero (det != 0) or
+//      the primes' product exceeds the columns' Hadamard bound (det = 0).
 // Pivot rows are marked used instead of swapped (only the sign of det
-// changes).
-struct DetArgs {
-    int64_t n;
-    const int32_t *pidx;
-    const int64_t *M;
-    const int64_t *nexp;
-    int MH;
+// changes).  Totals (points, kept, expansions, nonsingular, undecided) go
+// to tot[0..4].
+struct FiltArgs {
+    int64_t n;                    // point slots
+    const int32_t *pidx;          // < 0: unused slot
+    const int64_t *q;
+    const int32_t *c;             // D int32 per point
+    const int64_t *Min;           // instead of c: M rows given (stride MH), all tested
+    int D, MH, hl;                // MH = row stride of M, Kn (= hl)
     const int64_t *recs;
     RecL L;
+    int filter;                   // count expansions and test det
+    int64_t *M, *Kn, *nexp;       // stored when non-null (points kept)
+    int8_t *nonsing;              // stored when non-null
+    int8_t *bad;                  // per p-vector: M or Kn outside int64
+    unsigned long long *tot;      // [0] points [1] kept [2] expansions [3] nonsingular [4] undecided
     GeoArgs G;                    // int64 kappa: step 3
     const float *kT;              // per distinct kappa: kT[(k n + i) n + j] = kappa[i+1][j+1][k]
     const int64_t *kTofs;         // per geometry (identical kappas share one copy)
@@ -777,10 +787,11 @@ struct DetArgs {
     int lds_floats;               // LDS floats per block (0: no staging)
     const PfaModP *primes;
     int np;
-    int nl;                       // lanes per point group
-    int64_t *modlist;             // points left to step 3
+    int64_t modcap;               // list capacity; beyond it a point stays undecided
+    int64_t *modkp, *modnexp, *modM;   // list: slot, n_exp, M row (stride MH)
     unsigned long long *modcount;
-    int8_t *nonsing;
+    int64_t *keepkp, *keepnexp;   // points with n_exp > 0 (k_post -> k_filter)
+    unsigned long long *keepcount;
 };
 
 #define PFA_TWO52 4503599627370496.0
@@ -806,27 +817,110 @@ __device__ static inline DetLanes det_lanes(int nl)
     return d;
 }
 
-// Geometry row of point kp: (h, geo), or h = 0 when the point needs no test.
-__device__ static inline int det_point(const DetArgs &a, int64_t kp, int &geo)
+__device__ static inline uint64_t shfl_u64(uint64_t v, int src)
 {
-    if (kp >= a.n || a.pidx[kp] < 0 || a.nexp[kp] <= 0) return 0;
-    const int32_t *ri = (const int32_t *)(a.recs + (int64_t)a.pidx[kp] * a.L.words);
-    geo = ri[26];
-    return ri[24];
+    const int lo = __shfl((int)(uint32_t)v, src), hi = __shfl((int)(uint32_t)(v >> 32), src);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
-// w[s] = N[s][j] = sum_k kappa[s+1][j+1][k] M[k] for the column j of a lane
-// (kt = that column's kT entries, LDS or global), msum = sum_k |M[k]|.
-// M[k] comes from lane src0 + k (mj) or, for k = n, from mlast.  Every
-// lane runs the shuffles.
-template <int NMAX>
-__device__ __forceinline__ void det_build(double *w, double &msum, const float *kt, double mj,
-                                          double mlast, int src0, int nl, int n, bool col)
+__device__ static inline uint64_t wave_sum_u64(uint64_t v)
 {
-    for (int k = 0; k <= nl; ++k) {
-        const double mk0 = __shfl(mj, src0 + (k < nl ? k : 0));
+    for (int o = 32; o >= 1; o >>= 1) v += shfl_u64(v, (threadIdx.x & 63) ^ o);
+    return v;
+}
+
+__global__ __launch_bounds__(64) void k_post(FiltArgs a)
+{
+    extern __shared__ int64_t stage[];             // 64 x MH when rows are kept
+    const int lane = threadIdx.x;
+    const bool store = a.M != nullptr;
+    unsigned long long t_pts = 0, t_kept = 0;
+    for (int64_t base = (int64_t)blockIdx.x * 64; base < a.n; base += (int64_t)gridDim.x * 64) {
+        const int64_t kp = base + lane;
+        const bool valid = kp < a.n && a.pidx[kp] >= 0;
+        const int64_t *rec = a.recs + (valid ? (int64_t)a.pidx[kp] * a.L.words : 0);
+        const int32_t *ri = (const int32_t *)rec;
+        const int d = valid ? ri[1] : 0, h = d + 1;
+        const int32_t *cc = a.c + (valid ? kp : 0) * a.D;
+        bool ok = true;
+        int64_t Kn0 = 0, M0 = 0;
+        uint64_t kg = 0;
+        // Kn (staged when kept)
+        for (int i = 0; i < a.MH; ++i) {
+            int64_t v = 0;
+            if (valid && i < h) {
+                const int64_t *zb = rec + a.L.ZB + (int64_t)i * d;
+                pfa_i128 sk = 0;
+                for (int t = 0; t < d; ++t) sk += (pfa_i128)zb[t] * cc[t];
+                if (!pfa_fits64(sk)) ok = false;
+                v = (int64_t)sk;
+                if (i == 0) Kn0 = v;
+                else kg = pfa_gcd64(kg, (uint64_t)(v < 0 ? -v : v));
+            }
+            if (store) stage[lane * a.MH + i] = v;
+        }
+        const int64_t rows = (a.n - base < 64 ? a.n - base : 64) * a.MH;
+        if (store) {
+            __syncthreads();
+            for (int64_t e = lane; e < rows; e += 64) a.Kn[base * a.MH + e] = stage[e];
+            __syncthreads();
+        }
+        // M (all entries when kept, else M[0])
+        for (int i = 0; i < (store ? a.MH : 1); ++i) {
+            int64_t v = 0;
+            if (valid && i < h) {
+                const int64_t *bi = rec + a.L.Bi + (int64_t)i * d;
+                pfa_i128 sm = 0;
+                for (int t = 0; t < d; ++t) sm += (pfa_i128)bi[t] * cc[t];
+                if (!pfa_fits64(sm)) ok = false;
+                v = (int64_t)sm;
+                if (i == 0) M0 = v;
+            }
+            if (store) stage[lane * a.MH + i] = v;
+        }
+        if (store) {
+            __syncthreads();
+            for (int64_t e = lane; e < rows; e += 64) a.M[base * a.MH + e] = stage[e];
+            __syncthreads();
+        }
+        if (valid && !ok) a.bad[a.pidx[kp]] = 1;
+        int64_t ne = 0;
+        if (a.filter && valid) ne = ok ? pfa_count_expansions(M0, Kn0, kg, a.q[kp], rec[2]) : -1;
+        if (a.nexp && valid) a.nexp[kp] = ne;
+        if (a.nonsing && valid) a.nonsing[kp] = 0;
+        // append n_exp > 0 to the list
+        const unsigned long long km = __ballot(ne > 0);
+        const int cnt = __popcll(km);
+        unsigned long long b0 = 0;
+        if (lane == 0 && cnt) b0 = atomicAdd(a.keepcount, (unsigned long long)cnt);
+        b0 = shfl_u64(b0, 0);
+        if (ne > 0) {
+            const int off = __popcll(km & ((1ull << lane) - 1));
+            a.keepkp[b0 + off] = kp;
+            a.keepnexp[b0 + off] = ne;
+        }
+        const unsigned long long vm = __ballot(valid);
+        if (lane == 0) {
+            t_pts += __popcll(vm);
+            t_kept += cnt;
+        }
+    }
+    if (a.tot && lane == 0) {
+        atomicAdd(&a.tot[0], t_pts);
+        atomicAdd(&a.tot[1], t_kept);
+    }
+}
+
+// w[s] = N[s][j] = sum_k kappa[s+1][j+1][k] M[k] for the column j of a
+// lane (kt: that column's kT entries, LDS or global), msum = sum_k |M[k]|;
+// M[k] comes from lane src0 + k.  Every lane runs every shuffle.
+template <int NMAX>
+__device__ __forceinline__ void det_build(double *w, double &msum, const float *kt, double md,
+                                          int src0, int hl, int n, bool col)
+{
+    for (int k = 0; k < hl; ++k) {
+        const double mk = __shfl(md, src0 + k);
         if (!col || k > n) continue;
-        const double mk = k < n ? mk0 : mlast;
         msum += fabs(mk);
         const float *kk = kt + (int64_t)k * n * n;
 #pragma unroll
@@ -836,36 +930,38 @@ __device__ __forceinline__ void det_build(double *w, double &msum, const float *
 }
 
 template <int NMAX>
-__global__ __launch_bounds__(256) void k_det(DetArgs a)
+__global__ __launch_bounds__(256) void k_filter(FiltArgs a)
 {
     extern __shared__ float lds_kT[];
     __shared__ int64_t lds_ofs;
     __shared__ int lds_cnt;
-    const DetLanes L = det_lanes(a.nl);
-    const int R = L.R, j = L.j;
+    const DetLanes L = det_lanes(a.hl);
+    const int R = L.R, i = L.j, nlmax = a.hl - 1;
     const int wv = threadIdx.x >> 6;
+    // the points: the list k_post made, or every slot when M is given
+    const int64_t cnt = a.keepkp ? (int64_t)*a.keepcount : a.n;
+    auto slot = [&](int64_t t) -> int64_t { return a.keepkp ? a.keepkp[t] : t; };
     // each block takes a contiguous span of rounds (4 wavefronts x R points)
-    const int64_t rounds = (a.n + 4 * R - 1) / (4 * R);
+    const int64_t rounds = (cnt + 4 * R - 1) / (4 * R);
     const int64_t span = (rounds + gridDim.x - 1) / gridDim.x;
     const int64_t rd0 = (int64_t)blockIdx.x * span;
     const int64_t rd1 = rd0 + span < rounds ? rd0 + span : rounds;
     if (rd0 >= rd1) return;
-    // stage kappa of the geometry of the span's first tested point
+    // stage kappa of the geometry of the span's first point
     int64_t cofs = -1;
     if (a.lds_floats > 0) {
         if (threadIdx.x == 0) {
             int64_t o = -1;
             int c = 0;
-            const int64_t end = rd1 * 4 * R < a.n ? rd1 * 4 * R : a.n;
-            for (int64_t kp = rd0 * 4 * R; kp < end && kp < rd0 * 4 * R + 4096; ++kp) {
-                int g = 0;
-                const int h = det_point(a, kp, g);
-                if (h <= 1) continue;
-                if ((int64_t)(h - 1) * (h - 1) * h <= a.lds_floats && a.kTmax[g] < PFA_TWO52) {
+            const int64_t t0 = rd0 * 4 * R;
+            if (t0 < cnt) {
+                const int64_t kp = slot(t0);
+                const int32_t *ri = (const int32_t *)(a.recs + (int64_t)a.pidx[kp] * a.L.words);
+                const int h = ri[24], g = ri[26];
+                if (h > 1 && (int64_t)(h - 1) * (h - 1) * h <= a.lds_floats && a.kTmax[g] < PFA_TWO52) {
                     o = a.kTofs[g];
                     c = (h - 1) * (h - 1) * h;
                 }
-                break;
             }
             lds_ofs = o;
             lds_cnt = c;
@@ -876,36 +972,55 @@ __global__ __launch_bounds__(256) void k_det(DetArgs a)
         for (int e = threadIdx.x; e < c; e += blockDim.x) lds_kT[e] = a.kT[cofs + e];
         __syncthreads();
     }
+    unsigned long long t_exp = 0, t_ns = 0, t_und = 0;
     for (int64_t rd = rd0; rd < rd1; ++rd) {
-        const int64_t kp = (rd * 4 + wv) * R + L.r;
-        int geo = 0;
-        const int h = L.inlane ? det_point(a, kp, geo) : 0;
-        const bool need = h > 0;
-        const int n = need ? h - 1 : 0;
-        const bool col = need && j < n;
-        const int64_t *Mrow = a.M + (need ? kp : 0) * a.MH;
+        const int64_t tl = (rd * 4 + wv) * R + L.r;
+        const bool need = L.inlane && tl < cnt;
+        const int64_t kp = need ? slot(tl) : 0;
+        const int64_t ne = need ? (a.keepkp ? a.keepnexp[tl] : 1) : 0;
+        const int64_t *rec = a.recs + (need ? (int64_t)a.pidx[kp] * a.L.words : 0);
+        const int32_t *ri = (const int32_t *)rec;
+        const int h = need ? ri[24] : 0, geo = need ? ri[26] : 0;
+        const int n = h > 0 ? h - 1 : 0;
+        const bool col = need && i < n;
         int res = (need && n == 0) ? 1 : 0;
+
+        // M_i (checked into int64 by k_post already)
+        int64_t Mi = 0;
+        if (need && i < h) {
+            if (a.Min) {
+                Mi = a.Min[kp * a.MH + i];
+            } else {
+                const int32_t *cc = a.c + kp * a.D;
+                const int64_t *bi = rec + a.L.Bi + (int64_t)i * n;
+                pfa_i128 sm = 0;
+                for (int t = 0; t < n; ++t) sm += (pfa_i128)bi[t] * cc[t];
+                Mi = (int64_t)sm;
+            }
+        }
 
         // 1. N in double
         double w[NMAX];
 #pragma unroll
         for (int s = 0; s < NMAX; ++s) w[s] = 0.0;
-        const double mj = col ? (double)Mrow[j] : 0.0;
-        const double mlast = need ? (double)Mrow[n] : 0.0;
-        const bool lds = col && a.kTofs[geo] == cofs;
-        double msum = 0.0;
-        if (!__ballot(col && !lds))
-            det_build<NMAX>(w, msum, lds_kT + j, mj, mlast, L.src0, a.nl, n, col);
-        else
-            det_build<NMAX>(w, msum, a.kT + (col ? a.kTofs[geo] : 0) + j, mj, mlast, L.src0, a.nl, n, col);
-        const bool exact = !col || a.kTmax[geo] * msum < PFA_TWO52;
+        bool exact = true;
+        {
+            const double md = (double)Mi;
+            const bool lds = col && a.kTofs[geo] == cofs;
+            double msum = 0.0;
+            if (!__ballot(col && !lds))
+                det_build<NMAX>(w, msum, lds_kT + i, md, L.src0, a.hl, n, col);
+            else
+                det_build<NMAX>(w, msum, a.kT + (col ? a.kTofs[geo] : 0) + i, md, L.src0, a.hl, n, col);
+            exact = !col || a.kTmax[geo] * msum < PFA_TWO52;
+        }
         const bool exact_g = !(__ballot(col && !exact) & L.gm);
 
         // 2. Bareiss in double
         bool run = need && n > 0 && exact_g, trip = false;
         uint64_t used = 0;
         double prev = 1.0;
-        for (int k = 0; k < a.nl; ++k) {
+        for (int k = 0; k < nlmax; ++k) {
             if (run && k >= n) { run = false; res = 1; }
             if (!__ballot(run)) break;
             int cand = -1;
@@ -921,55 +1036,82 @@ __global__ __launch_bounds__(256) void k_det(DetArgs a)
             for (int s = 0; s < NMAX; ++s)
                 if (s == p) pj = w[s];
             const double pk = __shfl(pj, srck);
-            bool ok = true;
+            bool okb = true;
 #pragma unroll
             for (int s = 0; s < NMAX; ++s) {
                 const double ik = __shfl(w[s], srck);
                 if (act && col && s < n && s != p && !((used >> s) & 1)) {
                     const double x1 = w[s] * pk, x2 = ik * pj, x = x1 - x2;
-                    ok = ok && fabs(x1) < PFA_TWO53 && fabs(x2) < PFA_TWO53 && fabs(x) < PFA_TWO53;
+                    okb = okb && fabs(x1) < PFA_TWO53 && fabs(x2) < PFA_TWO53 && fabs(x) < PFA_TWO53;
                     w[s] = k == 0 ? x : x / prev;
                 }
             }
-            if (act && (__ballot(!ok) & L.gm)) { run = false; act = false; trip = true; }
+            if (act && (__ballot(!okb) & L.gm)) { run = false; act = false; trip = true; }
             if (act) {
                 used |= 1ull << p;
                 prev = pk;
             }
         }
         if (run) res = 1;
-        if (L.inlane && j == 0 && kp < a.n) {
-            if (need && n > 0 && (!exact_g || trip)) {
-                a.nonsing[kp] = 2;                 // until k_det_mod decides
-                const unsigned long long t = atomicAdd(a.modcount, 1ull);
-                a.modlist[t] = kp;                 // capacity: one entry per point slot
-            } else {
-                a.nonsing[kp] = (int8_t)res;
+
+        // 3. the modular list
+        const bool modn = need && n > 0 && (!exact_g || trip);
+        int64_t t = -1;
+        if (modn && i == 0) {
+            t = (int64_t)atomicAdd(a.modcount, 1ull);
+            if (t >= a.modcap) t = -1;
+        }
+        t = (int64_t)shfl_u64((uint64_t)t, L.src0);
+        if (modn && t >= 0) {
+            if (i < a.MH) a.modM[t * a.MH + i] = Mi;
+            if (i == 0) {
+                a.modkp[t] = kp;
+                a.modnexp[t] = ne;
             }
+        }
+        if (modn && t < 0) res = 2;             // list full: left to the host
+        if (need && i == 0) {
+            if (a.nonsing) a.nonsing[kp] = (int8_t)(modn && t >= 0 ? 2 : res);
+            if (!(modn && t >= 0)) {
+                if (res == 1) { t_ns++; t_exp += (unsigned long long)ne; }
+                if (res == 2) t_und++;
+            }
+        }
+    }
+    if (a.tot) {
+        const unsigned long long v2 = wave_sum_u64(t_exp), v3 = wave_sum_u64(t_ns),
+                                 v4 = wave_sum_u64(t_und);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&a.tot[2], v2);
+            atomicAdd(&a.tot[3], v3);
+            atomicAdd(&a.tot[4], v4);
         }
     }
 }
 
-// Step 3 for the points k_det listed.
+// Step 3 for the points k_filter listed (M rows from the list).
 template <int NMAX>
-__global__ __launch_bounds__(256) void k_det_mod(DetArgs a)
+__global__ __launch_bounds__(256) void k_det_mod(FiltArgs a)
 {
-    const DetLanes L = det_lanes(a.nl);
-    const int R = L.R, j = L.j;
-    const int64_t cnt = (int64_t)*a.modcount;
+    const DetLanes L = det_lanes(a.hl);
+    const int R = L.R, j = L.j, nlmax = a.hl - 1;
+    const unsigned long long cntu = *a.modcount;
+    const int64_t cnt = (int64_t)cntu < a.modcap ? (int64_t)cntu : a.modcap;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    unsigned long long t_exp = 0, t_ns = 0, t_und = 0;
     for (int64_t base = wave * R; base < cnt; base += nwaves * R) {
         const int64_t t = base + L.r;
-        const int64_t kp = (L.inlane && t < cnt) ? a.modlist[t] : a.n;
-        int geo = 0;
-        const int h = L.inlane ? det_point(a, kp, geo) : 0;
-        const bool need = h > 1;
-        const int n = need ? h - 1 : 0;
+        const bool need = L.inlane && t < cnt;
+        const int64_t kp = need ? a.modkp[t] : 0;
+        const int64_t *rec = a.recs + (need ? (int64_t)a.pidx[kp] * a.L.words : 0);
+        const int32_t *ri = (const int32_t *)rec;
+        const int h = need ? ri[24] : 0, geo = need ? ri[26] : 0;
+        const int n = h > 0 ? h - 1 : 0;
         const bool col = need && j < n;
-        const int64_t *Mrow = a.M + (need ? kp : 0) * a.MH;
+        const int64_t *Mrow = a.modM + (need ? t : 0) * a.MH;
         int res = 2;
-        bool modn = need;
+        bool modn = need && n > 0;
         int64_t Ni[NMAX];
 #pragma unroll
         for (int s = 0; s < NMAX; ++s) Ni[s] = 0;
@@ -996,7 +1138,7 @@ __global__ __launch_bounds__(256) void k_det_mod(DetArgs a)
         if (modn && (__ballot(modn && col && cn == 0.0) & L.gm)) { modn = false; res = 0; }
         const double tb = (modn && col) ? 0.5 * log2(cn) + 1e-9 : 0.0;
         double need_bits = 1.0;
-        for (int q = 0; q < a.nl; ++q) need_bits += __shfl(tb, L.inlane ? L.src0 + q : L.lane);
+        for (int q = 0; q < a.hl; ++q) need_bits += __shfl(tb, L.inlane ? L.src0 + q : L.lane);
         double have = 0.0;
         for (int pi = 0; pi < a.np; ++pi) {
             if (!__ballot(modn)) break;
@@ -1007,7 +1149,7 @@ __global__ __launch_bounds__(256) void k_det_mod(DetArgs a)
                 wm[s] = (modn && col && s < n) ? pfa_mod_fast(Ni[s], P) : 0u;
             bool runm = modn, zero = false;
             uint64_t usedm = 0;
-            for (int k = 0; k < a.nl; ++k) {
+            for (int k = 0; k < nlmax; ++k) {
                 if (runm && k >= n) runm = false;
                 if (!__ballot(runm)) break;
                 int cand = -1;
@@ -1040,7 +1182,20 @@ __global__ __launch_bounds__(256) void k_det_mod(DetArgs a)
                 if (have > need_bits) { res = 0; modn = false; }
             }
         }
-        if (need && j == 0) a.nonsing[kp] = (int8_t)res;
+        if (need && j == 0) {
+            if (a.nonsing) a.nonsing[kp] = (int8_t)res;
+            if (res == 1) { t_ns++; t_exp += (unsigned long long)a.modnexp[t]; }
+            if (res == 2) t_und++;
+        }
+    }
+    if (a.tot) {
+        const unsigned long long v2 = wave_sum_u64(t_exp), v3 = wave_sum_u64(t_ns),
+                                 v4 = wave_sum_u64(t_und);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&a.tot[2], v2);
+            atomicAdd(&a.tot[3], v3);
+            atomicAdd(&a.tot[4], v4);
+        }
     }
 }
 
@@ -1113,7 +1268,8 @@ static void *dev_get(DevCtx *C, int slot, size_t bytes, char *errbuf)
 enum {
     B_GEO_H, B_GEO_Q, B_GEO_QMAX, B_GEO_LIN, B_KAPPA, B_MBASIS, B_KOFS, B_MOFS, B_PS, B_PGEO,
     B_RECS, B_RING, B_SEQ, B_STACKS, B_CTL, B_PT_PIDX, B_PT_Q, B_PT_C, B_CNT, B_M, B_KN, B_NEXP, B_BAD,
-    B_PRIMES, B_KT, B_KTOFS, B_KTMAX, B_MODCNT, B_MODLIST, B_NONSING, B_PREPCOLD, B_NBUF
+    B_PRIMES, B_KT, B_KTOFS, B_KTMAX, B_MODCNT, B_MODLIST, B_MODNEXP, B_MODM, B_NONSING, B_PREPCOLD,
+    B_TOT, B_KEEPCNT, B_KEEPKP, B_KEEPNEXP, B_NBUF
 };
 
 // the k largest primes below 2^31 with their Barrett constants
@@ -1229,12 +1385,12 @@ static void *upload(DevCtx *C, int slot, const void *src, size_t bytes, hipStrea
     return d;
 }
 
-// The exact filter over n point slots (k_det, then k_det_mod on the points
-// it could not settle); nonsing[] receives 0 / 1 / 2 on stream s.
-static void det_run(DevCtx *C, hipStream_t s, const pfa_run_input *gin,
-                    const std::vector<int64_t> &kofs, const GeoArgs &G, int64_t n,
-                    const int32_t *d_pidx, const int64_t *d_M, const int64_t *d_nexp, int MH,
-                    const int64_t *d_recs, RecL L, int D, int8_t *d_ns, char *errbuf)
+// Post-processing over n point slots: k_filter, then k_det_mod on the
+// points it listed.  Outputs go to the non-null pointers of `a` (filled in
+// by the caller: slots, inputs, outputs, bad, tot); this adds kappa in
+// float, the primes and the list.
+static void filter_run(DevCtx *C, hipStream_t s, const pfa_run_input *gin,
+                       const std::vector<int64_t> &kofs, FiltArgs a, char *errbuf)
 {
     static const std::vector<PfaModP> pr = primes31(96);
     std::vector<float> kT;
@@ -1246,40 +1402,58 @@ static void det_run(DevCtx *C, hipStream_t s, const pfa_run_input *gin,
         const int h = gin->h[g];
         lds_floats = std::max(lds_floats, (h - 1) * (h - 1) * h);
     }
-    const int lds_cap = 96 * 1024 / 4;
-    if (lds_floats > lds_cap) lds_floats = 0;
+    if (lds_floats > 96 * 1024 / 4) lds_floats = 0;
     unsigned long long *d_mc = (unsigned long long *)dev_get(C, B_MODCNT, 8, errbuf);
     HIPCHK(hipMemsetAsync(d_mc, 0, 8, s));
-    DetArgs a{n, d_pidx, d_M, d_nexp, MH, d_recs, L, G,
-              (const float *)upload(C, B_KT, kT.data(), 4 * kT.size(), s, errbuf),
-              (const int64_t *)upload(C, B_KTOFS, kTofs.data(), 8 * kTofs.size(), s, errbuf),
-              (const double *)upload(C, B_KTMAX, kTmax.data(), 8 * kTmax.size(), s, errbuf),
-              lds_floats,
-              (const PfaModP *)upload(C, B_PRIMES, pr.data(), sizeof(PfaModP) * pr.size(), s, errbuf),
-              (int)pr.size(), std::max(D, 1),
-              (int64_t *)dev_get(C, B_MODLIST, 8 * std::max<int64_t>(n, 1), errbuf), d_mc, d_ns};
-    if (n <= 0) return;
-    const int R = 64 / a.nl;
-    const int64_t rounds = (n + 4 * R - 1) / (4 * R);
+    a.kT = (const float *)upload(C, B_KT, kT.data(), 4 * kT.size(), s, errbuf);
+    a.kTofs = (const int64_t *)upload(C, B_KTOFS, kTofs.data(), 8 * kTofs.size(), s, errbuf);
+    a.kTmax = (const double *)upload(C, B_KTMAX, kTmax.data(), 8 * kTmax.size(), s, errbuf);
+    a.lds_floats = a.filter ? lds_floats : 0;
+    a.primes = (const PfaModP *)upload(C, B_PRIMES, pr.data(), sizeof(PfaModP) * pr.size(), s, errbuf);
+    a.np = (int)pr.size();
+    a.modcap = std::max<int64_t>(1, std::min<int64_t>(a.n, 1 << 20));
+    a.modkp = (int64_t *)dev_get(C, B_MODLIST, 8 * a.modcap, errbuf);
+    a.modnexp = (int64_t *)dev_get(C, B_MODNEXP, 8 * a.modcap, errbuf);
+    a.modM = (int64_t *)dev_get(C, B_MODM, 8 * a.modcap * a.MH, errbuf);
+    a.modcount = d_mc;
+    if (a.n <= 0) return;
+    if (!a.Min) {
+        unsigned long long *d_kc = (unsigned long long *)dev_get(C, B_KEEPCNT, 8, errbuf);
+        HIPCHK(hipMemsetAsync(d_kc, 0, 8, s));
+        a.keepkp = (int64_t *)dev_get(C, B_KEEPKP, 8 * a.n, errbuf);
+        a.keepnexp = (int64_t *)dev_get(C, B_KEEPNEXP, 8 * a.n, errbuf);
+        a.keepcount = d_kc;
+        const int pgrid = (int)std::min<int64_t>((a.n + 63) / 64, 65536);
+        const size_t plds = a.M ? (size_t)64 * a.MH * 8 : 0;
+        hipLaunchKernelGGL(k_post, dim3(pgrid), dim3(64), plds, s, a);
+        HIPCHK(hipGetLastError());
+        if (!a.filter) return;
+    }
+    const int R = 64 / a.hl;
+    // list length unknown on the host: size the grid by the slots
+    const int64_t rounds = (a.n + 4 * R - 1) / (4 * R);
     const int grid = (int)std::min<int64_t>(rounds, 16384);
-    const size_t lds = (size_t)lds_floats * 4;
-#define PFA_DET_LAUNCH(NM)                                                                     \
+    const size_t lds = (size_t)a.lds_floats * 4;
+    const int nl = a.hl - 1;
+#define PFA_FILTER_LAUNCH(NM)                                                                  \
     do {                                                                                        \
-        HIPCHK(hipFuncSetAttribute((const void *)k_det<NM>,                                    \
+        HIPCHK(hipFuncSetAttribute((const void *)k_filter<NM>,                                 \
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));      \
-        hipLaunchKernelGGL(k_det<NM>, dim3(grid), dim3(256), lds, s, a);                        \
+        hipLaunchKernelGGL(k_filter<NM>, dim3(grid), dim3(256), lds, s, a);                     \
         HIPCHK(hipGetLastError());                                                              \
-        hipLaunchKernelGGL(k_det_mod<NM>, dim3(512), dim3(256), 0, s, a);                       \
-        HIPCHK(hipGetLastError());                                                              \
+        if (a.filter) {                                                                         \
+            hipLaunchKernelGGL(k_det_mod<NM>, dim3(512), dim3(256), 0, s, a);                   \
+            HIPCHK(hipGetLastError());                                                          \
+        }                                                                                       \
     } while (0)
-    if (a.nl <= 8) PFA_DET_LAUNCH(8);
-    else if (a.nl <= 12) PFA_DET_LAUNCH(12);
-    else if (a.nl <= 16) PFA_DET_LAUNCH(16);
-    else if (a.nl <= 20) PFA_DET_LAUNCH(20);
-    else if (a.nl <= 24) PFA_DET_LAUNCH(24);
-    else if (a.nl <= 32) PFA_DET_LAUNCH(32);
-    else PFA_DET_LAUNCH(40);
-#undef PFA_DET_LAUNCH
+    if (nl <= 8) PFA_FILTER_LAUNCH(8);
+    else if (nl <= 12) PFA_FILTER_LAUNCH(12);
+    else if (nl <= 16) PFA_FILTER_LAUNCH(16);
+    else if (nl <= 20) PFA_FILTER_LAUNCH(20);
+    else if (nl <= 24) PFA_FILTER_LAUNCH(24);
+    else if (nl <= 32) PFA_FILTER_LAUNCH(32);
+    else PFA_FILTER_LAUNCH(40);
+#undef PFA_FILTER_LAUNCH
 }
 
 // Final assembly shared by the device and host runs: statuses, sort, copy.
@@ -1514,51 +1688,51 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
             return 1;
         }
     }
-    // post: M, Kn, filter
+    // post: M, Kn, the expansion count and the det filter (k_filter); the
+    // rows are stored only when the points are kept
     const int64_t n = (int64_t)npts;
     const int HS = D + 1;                    // row stride of M, Kn on the device
-    int64_t *d_M = (int64_t *)dev_get(C, B_M, 8 * std::max<int64_t>(n, 1) * HS, errbuf);
-    int64_t *d_Kn = (int64_t *)dev_get(C, B_KN, 8 * std::max<int64_t>(n, 1) * HS, errbuf);
-    int64_t *d_nexp = (int64_t *)dev_get(C, B_NEXP, 8 * std::max<int64_t>(n, 1), errbuf);
+    const bool keep = in->keep_points != 0;
+    int64_t *d_M = keep ? (int64_t *)dev_get(C, B_M, 8 * std::max<int64_t>(n, 1) * HS, errbuf) : nullptr;
+    int64_t *d_Kn = keep ? (int64_t *)dev_get(C, B_KN, 8 * std::max<int64_t>(n, 1) * HS, errbuf) : nullptr;
+    int64_t *d_nexp = keep && in->filter ? (int64_t *)dev_get(C, B_NEXP, 8 * std::max<int64_t>(n, 1), errbuf) : nullptr;
+    int8_t *d_ns = keep && in->filter ? (int8_t *)dev_get(C, B_NONSING, std::max<int64_t>(n, 1), errbuf) : nullptr;
     int8_t *d_bad = (int8_t *)dev_get(C, B_BAD, std::max<int64_t>(n_p, 1), errbuf);
-    int8_t *d_ns = (int8_t *)dev_get(C, B_NONSING, std::max<int64_t>(n, 1), errbuf);
+    unsigned long long *d_tot = (unsigned long long *)dev_get(C, B_TOT, 64, errbuf);
     HIPCHK(hipMemsetAsync(d_bad, 0, std::max<int64_t>(n_p, 1), s));
+    HIPCHK(hipMemsetAsync(d_tot, 0, 64, s));
     HIPCHK(hipEventRecord(ev[5], s));
-    if (n > 0) {
-        PostArgs Pa{n, d_pidx, d_q, d_c, D, HS, d_recs, L, d_M, d_Kn, d_nexp, d_bad, in->filter};
-        int grid = (int)std::min<int64_t>((n + 255) / 256, 65536);
-        hipLaunchKernelGGL(k_post, dim3(grid), dim3(256), 0, s, Pa);
-        HIPCHK(hipGetLastError());
-        if (in->filter) {
-            det_run(C, s, in, kofs, G, n, d_pidx, d_M, d_nexp, HS, d_recs, L, D, d_ns, errbuf);
-        }
+    {
+        FiltArgs Fa{};
+        Fa.n = n;
+        Fa.pidx = d_pidx;
+        Fa.q = d_q;
+        Fa.c = d_c;
+        Fa.Min = nullptr;
+        Fa.D = D;
+        Fa.MH = HS;
+        Fa.hl = HS;
+        Fa.recs = d_recs;
+        Fa.L = L;
+        Fa.filter = in->filter;
+        Fa.M = d_M;
+        Fa.Kn = d_Kn;
+        Fa.nexp = d_nexp;
+        Fa.nonsing = d_ns;
+        Fa.bad = d_bad;
+        Fa.tot = d_tot;
+        Fa.G = G;
+        filter_run(C, s, in, kofs, Fa, errbuf);
     }
     HIPCHK(hipEventRecord(ev[6], s));
     // results back
-    const int64_t nh = in->keep_points ? n : 0;     // points copied back
+    const int64_t nh = keep ? n : 0;          // points copied back
     std::vector<int32_t> h_pidx(nh), h_status(n_p), h_code(n_p);
     std::vector<int64_t> h_q(nh), h_M(nh * HS), h_Kn(nh * HS), h_nexp(in->filter ? nh : 0);
     std::vector<int8_t> h_ns(in->filter ? nh : 0), h_bad(n_p);
     std::vector<int64_t> hdr((size_t)n_p * 16);
-    if (in->filter && n > 0 && !in->keep_points) {
-        // counts only: reduce the filter results on the device side copy
-        std::vector<int64_t> tmp(n);
-        std::vector<int8_t> tns(n);
-        HIPCHK(hipMemcpyAsync(tmp.data(), d_nexp, 8 * n, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(tns.data(), d_ns, n, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        int64_t kept = 0, nex = 0, nns = 0, und = 0;
-        for (int64_t k = 0; k < n; ++k)
-            if (tmp[k] > 0) {
-                kept++;
-                if (tns[k] == 1) { nex += tmp[k]; nns++; }
-                if (tns[k] == 2) und++;
-            }
-        out->stats[PFA_ST_KEPT] = (double)kept;
-        out->stats[PFA_ST_EXPANSIONS] = (double)nex;
-        out->stats[PFA_ST_NONSING] = (double)nns;
-        out->stats[PFA_ST_UNDECIDED] = (double)und;
-    }
+    unsigned long long htot[8] = {0};
+    HIPCHK(hipMemcpyAsync(htot, d_tot, 64, hipMemcpyDeviceToHost, s));
     if (nh > 0) {
         HIPCHK(hipMemcpyAsync(h_pidx.data(), d_pidx, 4 * n, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(h_q.data(), d_q, 8 * n, hipMemcpyDeviceToHost, s));
@@ -1601,17 +1775,15 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     out->stats[PFA_ST_DONATED] = (double)hc[6];
     out->stats[PFA_ST_FETCHES] = (double)hc[7];
     for (auto &e : ev) (void)hipEventDestroy(e);
-    double st_keep[4] = {out->stats[PFA_ST_KEPT], out->stats[PFA_ST_EXPANSIONS],
-                         out->stats[PFA_ST_NONSING], out->stats[PFA_ST_UNDECIDED]};
+    // counts-only totals from the device (htot: points, kept, expansions,
+    // nonsingular, undecided)
+    double st_keep[5] = {(double)htot[1], (double)htot[2], (double)htot[3], (double)htot[4],
+                         (double)htot[0]};
     int rc = finish_run(in, out, nh, HS, h_pidx.data(), h_q.data(), h_M.data(), h_Kn.data(),
                         in->filter ? h_nexp.data() : nullptr, in->filter ? h_ns.data() : nullptr,
                         h_status.data(), h_code.data(), h_bad.data(), errbuf);
     if (!in->keep_points) {
-        std::vector<int32_t> pp(n);
-        if (n) HIPCHK(hipMemcpy(pp.data(), d_pidx, 4 * n, hipMemcpyDeviceToHost));
-        int64_t nv = 0;
-        for (int64_t k = 0; k < n; ++k) nv += pp[k] >= 0;
-        out->stats[PFA_ST_POINTS] = (double)nv;
+        out->stats[PFA_ST_POINTS] = st_keep[4];
         out->stats[PFA_ST_KEPT] = st_keep[0];
         out->stats[PFA_ST_EXPANSIONS] = st_keep[1];
         out->stats[PFA_ST_NONSING] = st_keep[2];
@@ -1982,16 +2154,28 @@ extern "C" int pfa_det_filter(int n_geo, const int32_t *h, const int64_t *kappa,
             ri[24] = h[g];
             ri[26] = g;
         }
-        std::vector<int64_t> one((size_t)std::max<int64_t>(count, 1), 1);
         GeoArgs G{};
         G.kappa = (const int64_t *)up(B_KAPPA, kappa, 8 * ktot);
         G.kofs = (const int64_t *)up(B_KOFS, kofs.data(), 8 * kofs.size());
         const int32_t *d_geo = (const int32_t *)up(B_PT_PIDX, geo, 4 * count);
         const int64_t *d_M = (const int64_t *)up(B_M, M, 8 * count * MH);
-        const int64_t *d_one = (const int64_t *)up(B_NEXP, one.data(), 8 * count);
         const int64_t *d_recs = (const int64_t *)up(B_RECS, recs.data(), 8 * recs.size());
         int8_t *d_ns = (int8_t *)dev_get(C, B_NONSING, std::max<int64_t>(count, 1), errbuf);
-        det_run(C, s, &gin, kofs, G, count, d_geo, d_M, d_one, MH, d_recs, L, D, d_ns, errbuf);
+        int8_t *d_bad = (int8_t *)dev_get(C, B_BAD, std::max(n_geo, 1), errbuf);
+        FiltArgs Fa{};
+        Fa.n = count;
+        Fa.pidx = d_geo;
+        Fa.Min = d_M;
+        Fa.D = D;
+        Fa.MH = MH;
+        Fa.hl = MH;
+        Fa.recs = d_recs;
+        Fa.L = L;
+        Fa.filter = 1;
+        Fa.nonsing = d_ns;
+        Fa.bad = d_bad;
+        Fa.G = G;
+        filter_run(C, s, &gin, kofs, Fa, errbuf);
         if (count > 0) HIPCHK(hipMemcpyAsync(out, d_ns, count, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         return 0;

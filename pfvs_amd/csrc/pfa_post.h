@@ -21,21 +21,21 @@ PFA_HD static inline pfa_i128 pfa_cdiv_pos(pfa_i128 a, pfa_i128 b)
     return -pfa_fdiv128(-a, b);
 }
 
-// Number of (g, K0) pairs of one point, or -1 if some value leaves int128.
-PFA_HD static inline int64_t pfa_num_expansions(const int64_t *M, const int64_t *Kn, int h,
-                                               int64_t q, int64_t Q)
+// Number of (g, K0) pairs of a point with M0 = M[0], Kn0 = Kn[0],
+// K_gcd = gcd(Kn[1:]) (kg; 0 means 1), form value q and Q; -1 if some
+// value leaves int128 (never on valid input).
+PFA_HD static inline int64_t pfa_count_expansions(int64_t M0v, int64_t Kn0, uint64_t kg, int64_t q,
+                                                 int64_t Q)
 {
-    pfa_u128 kg = 0;
-    for (int i = 1; i < h; ++i) kg = pfa_gcd128(kg, pfa_abs128((pfa_i128)Kn[i]));
     if (kg == 0) kg = 1;
     if ((kg >> 62) != 0) return -1;
     pfa_i128 Kgcd = (pfa_i128)kg;
     pfa_i128 QK = (pfa_i128)Q * Kgcd;
     if (!((pfa_i128)q < QK)) return 0;
     if (q <= 0) return -1;                    // never: M0 >= linmin > 0 forces q > 0
-    pfa_i128 M0 = M[0];
+    pfa_i128 M0 = M0v;
     if (M0 <= 0) return -1;
-    pfa_i128 num = (pfa_i128)q + M0 * (pfa_i128)Kn[0];
+    pfa_i128 num = (pfa_i128)q + M0 * (pfa_i128)Kn0;
     if (num % Kgcd != 0) return -1;          // exact by construction
     pfa_i128 raw = num / Kgcd;
     pfa_i128 gmax = (QK - 1) / q;
@@ -45,11 +45,21 @@ PFA_HD static inline int64_t pfa_num_expansions(const int64_t *M, const int64_t 
         pfa_i128 Qp = g * raw;
         pfa_i128 lo = pfa_cdiv_pos(Qp - Q, M0);
         pfa_i128 up = pfa_fdiv128(Qp - Q, M0);
-        pfa_i128 cap = pfa_cdiv_pos((pfa_i128)Kn[0] * g, Kgcd) - 1;
+        pfa_i128 cap = pfa_cdiv_pos((pfa_i128)Kn0 * g, Kgcd) - 1;
         if (cap < up) up = cap;
         if (up >= lo) cnt += (int64_t)(up - lo + 1);
     }
     return cnt;
+}
+
+// The same for rows M, Kn of length h.
+PFA_HD static inline int64_t pfa_num_expansions(const int64_t *M, const int64_t *Kn, int h,
+                                               int64_t q, int64_t Q)
+{
+    pfa_u128 kg = 0;
+    for (int i = 1; i < h; ++i) kg = pfa_gcd128(kg, pfa_abs128((pfa_i128)Kn[i]));
+    if ((kg >> 62) != 0) return -1;
+    return pfa_count_expansions(M[0], Kn[0], (uint64_t)kg, q, Q);
 }
 
 // Arithmetic modulo an odd prime p < 2^31: Barrett reduction with
