@@ -66,6 +66,7 @@ static RecL rec_layout(int D)
 // header (int32 view): [0] status [1] d [20] m0_level [21] leafg [22] nL
 // [23] strict [24] h [25] code [26] geo; (int64 view) [1] qmax [2] Q
 // [3] linmin [4..8] Qe QGmul phi slack Kerr (double) [9] hmask
+// [14] max |Binter| [15] max |ZB|
 PFA_HD static inline PfaProb rec_view(const int64_t *r, const RecL &L)
 {
     const int32_t *ri = (const int32_t *)r;
@@ -184,6 +185,17 @@ PFA_HD static inline void prep_one(const PfaTeam &T, char *ws_hot, char *ws_cold
     for (int e = T.lane; e < h * d; e += T.n) {
         rec[L.Bi + e] = W.Bi[e];
         rec[L.ZB + e] = W.ZB[e];
+    }
+    if (T.lane == 0) {
+        // entry bounds of Binter and ZB for the point transform
+        int64_t mb = 0, mz = 0;
+        for (int e = 0; e < h * d; ++e) {
+            const int64_t b = W.Bi[e] < 0 ? -W.Bi[e] : W.Bi[e], z = W.ZB[e] < 0 ? -W.ZB[e] : W.ZB[e];
+            mb = b > mb ? b : mb;
+            mz = z > mz ? z : mz;
+        }
+        rec[14] = mb;
+        rec[15] = mz;
     }
     PFA_SYNC();
     for (int e = T.lane; e < d * d; e += T.n) {
@@ -831,9 +843,10 @@ __device__ static inline uint64_t wave_sum_u64(uint64_t v)
 
 __global__ __launch_bounds__(64) void k_post(FiltArgs a)
 {
-    extern __shared__ int64_t stage[];             // 64 x MH when rows are kept
+    extern __shared__ int64_t stage[];             // rows 64 x MH (when kept), then c 64 x D
     const int lane = threadIdx.x;
     const bool store = a.M != nullptr;
+    int32_t *cst = (int32_t *)(stage + (store ? 64 * a.MH : 0));
     unsigned long long t_pts = 0, t_kept = 0;
     for (int64_t base = (int64_t)blockIdx.x * 64; base < a.n; base += (int64_t)gridDim.x * 64) {
         const int64_t kp = base + lane;
@@ -841,21 +854,26 @@ __global__ __launch_bounds__(64) void k_post(FiltArgs a)
         const int64_t *rec = a.recs + (valid ? (int64_t)a.pidx[kp] * a.L.words : 0);
         const int32_t *ri = (const int32_t *)rec;
         const int d = valid ? ri[1] : 0, h = d + 1;
-        const int32_t *cc = a.c + (valid ? kp : 0) * a.D;
-        bool ok = true;
-        int64_t Kn0 = 0, M0 = 0;
+        // the wavefront's 64 c vectors are one contiguous block: stage it
+        const int64_t nc = (a.n - base < 64 ? a.n - base : 64) * a.D;
+        __syncthreads();
+        for (int64_t e = lane; e < nc; e += 64) cst[e] = a.c[base * a.D + e];
+        __syncthreads();
+        const int32_t *cc = cst + lane * a.D;
+        int ok = 1;
+        int64_t Kn0 = 0, M0 = 0, cmax = 0;
         uint64_t kg = 0;
+        for (int t = 0; t < d; ++t) {
+            const int64_t x = cc[t] < 0 ? -(int64_t)cc[t] : cc[t];
+            cmax = x > cmax ? x : cmax;
+        }
         // Kn (staged when kept)
         for (int i = 0; i < a.MH; ++i) {
             int64_t v = 0;
             if (valid && i < h) {
-                const int64_t *zb = rec + a.L.ZB + (int64_t)i * d;
-                pfa_i128 sk = 0;
-                for (int t = 0; t < d; ++t) sk += (pfa_i128)zb[t] * cc[t];
-                if (!pfa_fits64(sk)) ok = false;
-                v = (int64_t)sk;
+                v = pfa_dot_c(rec + a.L.ZB + (int64_t)i * d, cc, d, rec[15], cmax, &ok);
                 if (i == 0) Kn0 = v;
-                else kg = pfa_gcd64(kg, (uint64_t)(v < 0 ? -v : v));
+                else if (kg != 1) kg = pfa_gcd64(kg, (uint64_t)(v < 0 ? -v : v));
             }
             if (store) stage[lane * a.MH + i] = v;
         }
@@ -869,11 +887,7 @@ __global__ __launch_bounds__(64) void k_post(FiltArgs a)
         for (int i = 0; i < (store ? a.MH : 1); ++i) {
             int64_t v = 0;
             if (valid && i < h) {
-                const int64_t *bi = rec + a.L.Bi + (int64_t)i * d;
-                pfa_i128 sm = 0;
-                for (int t = 0; t < d; ++t) sm += (pfa_i128)bi[t] * cc[t];
-                if (!pfa_fits64(sm)) ok = false;
-                v = (int64_t)sm;
+                v = pfa_dot_c(rec + a.L.Bi + (int64_t)i * d, cc, d, rec[14], cmax, &ok);
                 if (i == 0) M0 = v;
             }
             if (store) stage[lane * a.MH + i] = v;
@@ -885,7 +899,7 @@ __global__ __launch_bounds__(64) void k_post(FiltArgs a)
         }
         if (valid && !ok) a.bad[a.pidx[kp]] = 1;
         int64_t ne = 0;
-        if (a.filter && valid) ne = ok ? pfa_count_expansions(M0, Kn0, kg, a.q[kp], rec[2]) : -1;
+        if (a.filter && valid) ne = ok ? pfa_count_expansions_fast(M0, Kn0, kg, a.q[kp], rec[2]) : -1;
         if (a.nexp && valid) a.nexp[kp] = ne;
         if (a.nonsing && valid) a.nonsing[kp] = 0;
         // append n_exp > 0 to the list
@@ -987,16 +1001,19 @@ __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
 
         // M_i (checked into int64 by k_post already)
         int64_t Mi = 0;
-        if (need && i < h) {
-            if (a.Min) {
-                Mi = a.Min[kp * a.MH + i];
-            } else {
-                const int32_t *cc = a.c + kp * a.D;
-                const int64_t *bi = rec + a.L.Bi + (int64_t)i * n;
-                pfa_i128 sm = 0;
-                for (int t = 0; t < n; ++t) sm += (pfa_i128)bi[t] * cc[t];
-                Mi = (int64_t)sm;
+        if (a.Min) {
+            if (need && i < h) Mi = a.Min[kp * a.MH + i];
+        } else {
+            // lane t < n holds c[t]; every lane forms its entry with shuffles
+            const int32_t cv = (need && i < n) ? a.c[kp * a.D + i] : 0;
+            const int64_t *bi = rec + a.L.Bi + (int64_t)i * n;
+            const bool row = need && i < h;
+            pfa_i128 sm = 0;
+            for (int t = 0; t < nlmax; ++t) {
+                const int32_t ct = __shfl(cv, L.src0 + t);
+                if (row && t < n) sm += (pfa_i128)bi[t] * ct;
             }
+            Mi = row ? (int64_t)sm : 0;
         }
 
         // 1. N in double
@@ -1424,7 +1441,7 @@ static void filter_run(DevCtx *C, hipStream_t s, const pfa_run_input *gin,
         a.keepnexp = (int64_t *)dev_get(C, B_KEEPNEXP, 8 * a.n, errbuf);
         a.keepcount = d_kc;
         const int pgrid = (int)std::min<int64_t>((a.n + 63) / 64, 65536);
-        const size_t plds = a.M ? (size_t)64 * a.MH * 8 : 0;
+        const size_t plds = (a.M ? (size_t)64 * a.MH * 8 : 0) + (size_t)64 * a.D * 4;
         hipLaunchKernelGGL(k_post, dim3(pgrid), dim3(64), plds, s, a);
         HIPCHK(hipGetLastError());
         if (!a.filter) return;

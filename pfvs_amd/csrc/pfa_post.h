@@ -52,6 +52,73 @@ PFA_HD static inline int64_t pfa_count_expansions(int64_t M0v, int64_t Kn0, uint
     return cnt;
 }
 
+// floor(a / b) for b > 0 and |a|, b < 2^53: the double quotient is within 1
+// of a / b, and one remainder test corrects it.
+PFA_HD static inline int64_t pfa_fdiv_small(int64_t a, int64_t b)
+{
+    int64_t q = (int64_t)floor((double)a / (double)b);
+    const int64_t r = a - q * b;
+    if (r < 0) q--;
+    else if (r >= b) q++;
+    return q;
+}
+
+// pfa_count_expansions in int64 when every value stays below 2^52 (the
+// common case); otherwise the int128 form.  Same result.
+PFA_HD static inline int64_t pfa_count_expansions_fast(int64_t M0, int64_t Kn0, uint64_t kg,
+                                                      int64_t q, int64_t Q)
+{
+    const int64_t B31 = (int64_t)1 << 31, B52 = (int64_t)1 << 52;
+    const int64_t k = kg == 0 ? 1 : (int64_t)(kg < (uint64_t)B31 ? kg : (uint64_t)B31);
+    if (!(M0 > 0 && M0 < B31 && q > 0 && q < B31 && Q > 0 && Q < B31 && Kn0 > -B31 && Kn0 < B31 &&
+          k < B31))
+        return pfa_count_expansions(M0, Kn0, kg, q, Q);
+    const int64_t QK = Q * k;                       // < 2^62
+    if (!(q < QK)) return 0;
+    if (QK >= B52) return pfa_count_expansions(M0, Kn0, kg, q, Q);
+    const int64_t num = q + M0 * Kn0;               // |.| < 2^63
+    if (num % k != 0) return -1;                    // exact by construction
+    const int64_t raw = num / k;
+    const int64_t gmax = pfa_fdiv_small(QK - 1, q);
+    if (gmax > 1000000) return -1;
+    const double gd = (double)gmax;
+    if (!((raw < 0 ? -(double)raw : (double)raw) * gd < 4.0e15) ||
+        !((Kn0 < 0 ? -(double)Kn0 : (double)Kn0) * gd < 4.0e15))
+        return pfa_count_expansions(M0, Kn0, kg, q, Q);
+    int64_t cnt = 0;
+    for (int64_t g = 1; g <= gmax; ++g) {
+        const int64_t a = g * raw - Q;
+        const int64_t lo = -pfa_fdiv_small(-a, M0);
+        int64_t up = pfa_fdiv_small(a, M0);
+        const int64_t cap = -pfa_fdiv_small(-(Kn0 * g), k) - 1;
+        if (cap < up) up = cap;
+        if (up >= lo) cnt += up - lo + 1;
+    }
+    return cnt;
+}
+
+// sum_t row[t] c[t] for a row bounded by |row[t]| <= rmax and |c| <= cmax:
+// 32-bit products when rmax < 2^31, 64-bit when the bound keeps every
+// partial sum below 2^62, else int128 with *ok = 0 if the sum leaves int64.
+PFA_HD static inline int64_t pfa_dot_c(const int64_t *row, const int32_t *c, int d, int64_t rmax,
+                                       int64_t cmax, int *ok)
+{
+    const double bound = (double)rmax * (double)cmax * (double)d;
+    if (bound < 4.0e18) {
+        int64_t s = 0;
+        if (rmax < ((int64_t)1 << 31)) {
+            for (int t = 0; t < d; ++t) s += (int64_t)(int32_t)row[t] * (int64_t)c[t];
+        } else {
+            for (int t = 0; t < d; ++t) s += row[t] * (int64_t)c[t];
+        }
+        return s;
+    }
+    pfa_i128 s = 0;
+    for (int t = 0; t < d; ++t) s += (pfa_i128)row[t] * c[t];
+    if (!pfa_fits64(s)) *ok = 0;
+    return (int64_t)s;
+}
+
 // The same for rows M, Kn of length h.
 PFA_HD static inline int64_t pfa_num_expansions(const int64_t *M, const int64_t *Kn, int h,
                                                int64_t q, int64_t Q)
