@@ -761,10 +761,12 @@ PFA_HD static inline int kappa_M(const int64_t *kappa, int h, const int64_t *M, 
 // wavefront holds R = 64 / hl points (hl: the largest h of the launch);
 // lane i of a group computes M_i and owns column i of N (n = h - 1
 // columns, rows s < NMAX in registers).  Per point:
-//   1. N in double when max|kappa| * sum|M| < 2^52 (every product and
-//      partial sum is then an exact integer), from kappa as float (exact
-//      below 2^24, else step 3), staged in LDS for the geometry of the
-//      block's contiguous span of points;
+//   1. N from kappa as int32 (|kappa| < 2^31, else step 3), staged in LDS
+//      for the geometry of the block's contiguous span of points: with
+//      24-bit integer multiply-adds when sum|M| < 2^23 and
+//      max|kappa| sum|M| < 2^31 (every partial sum then fits int32), else
+//      in double when max|kappa| sum|M| < 2^52 (every product and partial
+//      sum is then an exact integer), else step 3;
 //   2. fraction-free elimination (Bareiss) in double, every product and
 //      difference checked below 2^53 so each value is the exact integer
 //      minor: a column without a pivot among the unused rows proves
@@ -793,9 +795,9 @@ struct FiltArgs {
     int8_t *bad;                  // per p-vector: M or Kn outside int64
     unsigned long long *tot;      // [0] points [1] kept [2] expansions [3] nonsingular [4] undecided
     GeoArgs G;                    // int64 kappa: step 3
-    const float *kT;              // per distinct kappa: kT[(k n + i) n + j] = kappa[i+1][j+1][k]
+    const int32_t *kT;            // per distinct kappa: kT[(k n + i) n + j] = kappa[i+1][j+1][k]
     const int64_t *kTofs;         // per geometry (identical kappas share one copy)
-    const double *kTmax;          // per geometry max |kappa| (inf when >= 2^24)
+    const double *kTmax;          // per geometry max |kappa| (inf when >= 2^31)
     int lds_floats;               // LDS floats per block (0: no staging)
     const PfaModP *primes;
     int np;
@@ -803,6 +805,8 @@ struct FiltArgs {
     int64_t *modkp, *modnexp, *modM;   // list: slot, n_exp, M row (stride MH)
     unsigned long long *modcount;
     int64_t *keepkp, *keepnexp;   // points with n_exp > 0 (k_post -> k_filter)
+    int64_t *keepmeta;            // their p-vector << 32 | geometry << 8 | h
+    int n_geo;
     unsigned long long *keepcount;
 };
 
@@ -912,6 +916,7 @@ __global__ __launch_bounds__(64) void k_post(FiltArgs a)
             const int off = __popcll(km & ((1ull << lane) - 1));
             a.keepkp[b0 + off] = kp;
             a.keepnexp[b0 + off] = ne;
+            a.keepmeta[b0 + off] = ((int64_t)a.pidx[kp] << 32) | ((int64_t)ri[26] << 8) | (int64_t)h;
         }
         const unsigned long long vm = __ballot(valid);
         if (lane == 0) {
@@ -926,17 +931,63 @@ __global__ __launch_bounds__(64) void k_post(FiltArgs a)
 }
 
 // w[s] = N[s][j] = sum_k kappa[s+1][j+1][k] M[k] for the column j of a
-// lane (kt: that column's kT entries, LDS or global), msum = sum_k |M[k]|;
-// M[k] comes from lane src0 + k.  Every lane runs every shuffle.
+// lane (kt: that column's kT entries, LDS or global); M[k] comes from lane
+// src0 + k.  Every lane runs every shuffle.  det_build_i24: exact when
+// |kappa|, |M| < 2^23 and every partial sum fits int32; det_build_f64:
+// exact when every partial sum stays below 2^53.
 template <int NMAX>
-__device__ __forceinline__ void det_build(double *w, double &msum, const float *kt, double md,
-                                          int src0, int hl, int n, bool col)
+__device__ __forceinline__ void det_build_i24(double *w, const int32_t *kt, int32_t mi, int src0,
+                                              int hl, int n, bool col)
+{
+    int32_t acc[NMAX];
+#pragma unroll
+    for (int s = 0; s < NMAX; ++s) acc[s] = 0;
+    for (int k = 0; k < hl; ++k) {
+        const int32_t mk = __shfl(mi, src0 + k);
+        if (!col || k > n) continue;
+        const int32_t *kk = kt + (int64_t)k * n * n;
+#pragma unroll
+        for (int s = 0; s < NMAX; ++s)
+            if (s < n) acc[s] += __mul24(kk[s * n], mk);
+    }
+#pragma unroll
+    for (int s = 0; s < NMAX; ++s) w[s] = (double)acc[s];
+}
+
+// The same from LDS in the zero-padded layout lds[(k NMAX + s) NMAX + j]
+// (entries with k > n, s >= n or j >= n are 0; every offset is a
+// compile-time constant from the lane's base j): all shuffles first, then
+// the fully unrolled multiply-adds with no per-entry predicate (padding
+// adds zeros).
+template <int NMAX>
+__device__ __forceinline__ void det_build_i24_lds(double *w, const int32_t *ktj, int32_t mi,
+                                                  int src0, int n)
+{
+    int32_t mk[NMAX + 1];
+#pragma unroll
+    for (int k = 0; k <= NMAX; ++k) {
+        const int32_t v = __shfl(mi, src0 + k);
+        mk[k] = k <= n ? v : 0;
+    }
+    int32_t acc[NMAX];
+#pragma unroll
+    for (int s = 0; s < NMAX; ++s) acc[s] = 0;
+#pragma unroll
+    for (int k = 0; k <= NMAX; ++k)
+#pragma unroll
+        for (int s = 0; s < NMAX; ++s) acc[s] += __mul24(ktj[(k * NMAX + s) * NMAX], mk[k]);
+#pragma unroll
+    for (int s = 0; s < NMAX; ++s) w[s] = (double)acc[s];
+}
+
+template <int NMAX>
+__device__ __forceinline__ void det_build_f64(double *w, const int32_t *kt, double md, int src0,
+                                              int hl, int n, bool col)
 {
     for (int k = 0; k < hl; ++k) {
         const double mk = __shfl(md, src0 + k);
         if (!col || k > n) continue;
-        msum += fabs(mk);
-        const float *kk = kt + (int64_t)k * n * n;
+        const int32_t *kk = kt + (int64_t)k * n * n;
 #pragma unroll
         for (int s = 0; s < NMAX; ++s)
             if (s < n) w[s] = fma((double)kk[s * n], mk, w[s]);
@@ -946,9 +997,11 @@ __device__ __forceinline__ void det_build(double *w, double &msum, const float *
 template <int NMAX>
 __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
 {
-    extern __shared__ float lds_kT[];
+    extern __shared__ int32_t lds_kT[];
     __shared__ int64_t lds_ofs;
     __shared__ int lds_cnt;
+    __shared__ int64_t s_kofs[256];                // kappa offset and bound of geometries < 256
+    __shared__ double s_kmax[256];
     const DetLanes L = det_lanes(a.hl);
     const int R = L.R, i = L.j, nlmax = a.hl - 1;
     const int wv = threadIdx.x >> 6;
@@ -972,9 +1025,10 @@ __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
                 const int64_t kp = slot(t0);
                 const int32_t *ri = (const int32_t *)(a.recs + (int64_t)a.pidx[kp] * a.L.words);
                 const int h = ri[24], g = ri[26];
-                if (h > 1 && (int64_t)(h - 1) * (h - 1) * h <= a.lds_floats && a.kTmax[g] < PFA_TWO52) {
+                if (h > 1 && h - 1 <= NMAX && (NMAX + 1) * NMAX * NMAX <= a.lds_floats &&
+                    a.kTmax[g] < PFA_TWO52) {
                     o = a.kTofs[g];
-                    c = (h - 1) * (h - 1) * h;
+                    c = h - 1;
                 }
             }
             lds_ofs = o;
@@ -982,19 +1036,51 @@ __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
         }
         __syncthreads();
         cofs = lds_ofs;
-        const int c = lds_cnt;
-        for (int e = threadIdx.x; e < c; e += blockDim.x) lds_kT[e] = a.kT[cofs + e];
+        // zero-padded copy: lds[(k NMAX + s) NMAX + j] = kT[(k n + s) n + j]
+        const int nn = lds_cnt, cnt = (nn + 1) * nn * nn;
+        if (cofs >= 0) {
+            for (int e = threadIdx.x; e < (NMAX + 1) * NMAX * NMAX; e += blockDim.x) lds_kT[e] = 0;
+            __syncthreads();
+            for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
+                const int k = e / (nn * nn), r = e - k * nn * nn, s2 = r / nn, j2 = r - s2 * nn;
+                lds_kT[(k * NMAX + s2) * NMAX + j2] = a.kT[cofs + e];
+            }
+        }
         __syncthreads();
     }
+    const int ng = a.n_geo < 256 ? a.n_geo : 256;
+    for (int g = threadIdx.x; g < ng; g += blockDim.x) {
+        s_kofs[g] = a.kTofs[g];
+        s_kmax[g] = a.kTmax[g];
+    }
+    __syncthreads();
+    auto kofs_of = [&](int g) -> int64_t { return g < 256 ? s_kofs[g] : a.kTofs[g]; };
+    auto kmax_of = [&](int g) -> double { return g < 256 ? s_kmax[g] : a.kTmax[g]; };
     unsigned long long t_exp = 0, t_ns = 0, t_und = 0;
     for (int64_t rd = rd0; rd < rd1; ++rd) {
         const int64_t tl = (rd * 4 + wv) * R + L.r;
         const bool need = L.inlane && tl < cnt;
-        const int64_t kp = need ? slot(tl) : 0;
-        const int64_t ne = need ? (a.keepkp ? a.keepnexp[tl] : 1) : 0;
-        const int64_t *rec = a.recs + (need ? (int64_t)a.pidx[kp] * a.L.words : 0);
-        const int32_t *ri = (const int32_t *)rec;
-        const int h = need ? ri[24] : 0, geo = need ? ri[26] : 0;
+        int64_t kp = 0, ne = 0, pi = 0;
+        int h = 0, geo = 0;
+        if (need) {
+            if (a.keepkp) {
+                // one level of loads: slot, n_exp and the point's metadata
+                kp = a.keepkp[tl];
+                ne = a.keepnexp[tl];
+                const int64_t m = a.keepmeta[tl];
+                pi = m >> 32;
+                geo = (int)((m >> 8) & 0xffffff);
+                h = (int)(m & 0xff);
+            } else {
+                kp = tl;
+                ne = 1;
+                pi = a.pidx[kp];
+                const int32_t *r0 = (const int32_t *)(a.recs + pi * a.L.words);
+                h = r0[24];
+                geo = r0[26];
+            }
+        }
+        const int64_t *rec = a.recs + pi * a.L.words;
         const int n = h > 0 ? h - 1 : 0;
         const bool col = need && i < n;
         int res = (need && n == 0) ? 1 : 0;
@@ -1016,20 +1102,45 @@ __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
             Mi = row ? (int64_t)sm : 0;
         }
 
-        // 1. N in double
+        // 1. N: group bound sum|M| (a shuffle tree), then 24-bit integer or
+        // double multiply-adds (the whole wavefront takes one form)
+        // (terms capped at 2^57 so the sum cannot wrap; a capped sum fails
+        // every test below)
+        uint64_t am = (need && i < h) ? (uint64_t)(Mi < 0 ? -Mi : Mi) : 0;
+        if (am > ((uint64_t)1 << 57)) am = (uint64_t)1 << 57;
+        for (int off = 1; off < a.hl; off <<= 1) {
+            const uint64_t o = shfl_u64(am, (L.inlane && i + off < a.hl) ? L.lane + off : L.lane);
+            am += o;
+        }
+        const double msum = (double)shfl_u64(am, L.src0);
+        const double kmax = need ? kmax_of(geo) : 0.0;
+        const bool exact = !col || kmax * msum < PFA_TWO52;
+        const bool small = !col || (msum < 8388608.0 && kmax < 8388608.0 && kmax * msum < 2147483648.0);
         double w[NMAX];
 #pragma unroll
         for (int s = 0; s < NMAX; ++s) w[s] = 0.0;
-        bool exact = true;
         {
-            const double md = (double)Mi;
-            const bool lds = col && a.kTofs[geo] == cofs;
-            double msum = 0.0;
-            if (!__ballot(col && !lds))
-                det_build<NMAX>(w, msum, lds_kT + i, md, L.src0, a.hl, n, col);
+            const bool lds = col && kofs_of(geo) == cofs;
+            const bool all_lds = !__ballot(col && !lds);
+            const int32_t *kt = a.kT + (col ? kofs_of(geo) : 0) + i;
+            bool done = false;
+            if constexpr (NMAX <= 24) {
+                // (larger NMAX never stage kappa: its padded slab exceeds 64 KB)
+                if (all_lds && !__ballot(!small)) {
+                    // lanes without a column read column 0 and keep w = 0
+                    det_build_i24_lds<NMAX>(w, lds_kT + (col ? i : 0), (int32_t)Mi, L.src0, n);
+                    if (!col) {
+#pragma unroll
+                        for (int s2 = 0; s2 < NMAX; ++s2) w[s2] = 0.0;
+                    }
+                    done = true;
+                }
+            }
+            if (done) {
+            } else if (!__ballot(!small))
+                det_build_i24<NMAX>(w, kt, (int32_t)Mi, L.src0, a.hl, n, col);
             else
-                det_build<NMAX>(w, msum, a.kT + (col ? a.kTofs[geo] : 0) + i, md, L.src0, a.hl, n, col);
-            exact = !col || a.kTmax[geo] * msum < PFA_TWO52;
+                det_build_f64<NMAX>(w, kt, (double)Mi, L.src0, a.hl, n, col);
         }
         const bool exact_g = !(__ballot(col && !exact) & L.gm);
 
@@ -1053,13 +1164,14 @@ __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
             for (int s = 0; s < NMAX; ++s)
                 if (s == p) pj = w[s];
             const double pk = __shfl(pj, srck);
-            bool okb = true;
+            int okb = 1;
 #pragma unroll
             for (int s = 0; s < NMAX; ++s) {
                 const double ik = __shfl(w[s], srck);
                 if (act && col && s < n && s != p && !((used >> s) & 1)) {
                     const double x1 = w[s] * pk, x2 = ik * pj, x = x1 - x2;
-                    okb = okb && fabs(x1) < PFA_TWO53 && fabs(x2) < PFA_TWO53 && fabs(x) < PFA_TWO53;
+                    okb &= (int)(fabs(x1) < PFA_TWO53) & (int)(fabs(x2) < PFA_TWO53) &
+                           (int)(fabs(x) < PFA_TWO53);
                     w[s] = k == 0 ? x : x / prev;
                 }
             }
@@ -1286,7 +1398,7 @@ enum {
     B_GEO_H, B_GEO_Q, B_GEO_QMAX, B_GEO_LIN, B_KAPPA, B_MBASIS, B_KOFS, B_MOFS, B_PS, B_PGEO,
     B_RECS, B_RING, B_SEQ, B_STACKS, B_CTL, B_PT_PIDX, B_PT_Q, B_PT_C, B_CNT, B_M, B_KN, B_NEXP, B_BAD,
     B_PRIMES, B_KT, B_KTOFS, B_KTMAX, B_MODCNT, B_MODLIST, B_MODNEXP, B_MODM, B_NONSING, B_PREPCOLD,
-    B_TOT, B_KEEPCNT, B_KEEPKP, B_KEEPNEXP, B_NBUF
+    B_TOT, B_KEEPCNT, B_KEEPKP, B_KEEPNEXP, B_KEEPMETA, B_NBUF
 };
 
 // the k largest primes below 2^31 with their Barrett constants
@@ -1350,12 +1462,12 @@ static void fill_geo_offsets(const pfa_run_input *in, std::vector<int64_t> &kofs
     }
 }
 
-// kappa[i+1][j+1][k] as float at kT[ofs + (k n + i) n + j] (n = h - 1; the
+// kappa[i+1][j+1][k] as int32 at kT[ofs + (k n + i) n + j] (n = h - 1; the
 // lanes of a point group read consecutive j), one copy per distinct kappa
 // (kTofs per geometry), and the largest |kappa| of each geometry
-// (infinity when one reaches 2^24, where float stops being exact).
+// (infinity when one reaches 2^31).
 static void build_kT(const pfa_run_input *in, const std::vector<int64_t> &kofs,
-                     std::vector<float> &kT, std::vector<int64_t> &kTofs,
+                     std::vector<int32_t> &kT, std::vector<int64_t> &kTofs,
                      std::vector<double> &kTmax)
 {
     kTofs.assign(in->n_geo, 0);
@@ -1387,12 +1499,12 @@ static void build_kT(const pfa_run_input *in, const std::vector<int64_t> &kofs,
             for (int64_t i = 0; i < n; ++i)
                 for (int64_t jj = 0; jj < n; ++jj) {
                     const int64_t v = kap[((i + 1) * h + (jj + 1)) * h + k];
-                    kT[kTofs[g] + (k * n + i) * n + jj] = (float)v;
+                    kT[kTofs[g] + (k * n + i) * n + jj] = (int32_t)v;      // exact when mx < 2^31
                     mx = std::max(mx, std::fabs((double)v));
                 }
-        kTmax[g] = mx < 16777216.0 ? mx : INFINITY;
+        kTmax[g] = mx < 2147483648.0 ? mx : INFINITY;
     }
-    if (kT.empty()) kT.push_back(0.f);
+    if (kT.empty()) kT.push_back(0);
 }
 
 static void *upload(DevCtx *C, int slot, const void *src, size_t bytes, hipStream_t s, char *errbuf)
@@ -1410,21 +1522,21 @@ static void filter_run(DevCtx *C, hipStream_t s, const pfa_run_input *gin,
                        const std::vector<int64_t> &kofs, FiltArgs a, char *errbuf)
 {
     static const std::vector<PfaModP> pr = primes31(96);
-    std::vector<float> kT;
+    std::vector<int32_t> kT;
     std::vector<double> kTmax;
     std::vector<int64_t> kTofs;
     build_kT(gin, kofs, kT, kTofs, kTmax);
-    int lds_floats = 0;                    // largest kappa slab of the launch, if it fits
-    for (int g = 0; g < gin->n_geo; ++g) {
-        const int h = gin->h[g];
-        lds_floats = std::max(lds_floats, (h - 1) * (h - 1) * h);
-    }
-    if (lds_floats > 96 * 1024 / 4) lds_floats = 0;
+    // LDS for kappa in the padded layout of the launch's NMAX, if it fits
+    const int nl = a.hl - 1;
+    const int nmax = nl <= 8 ? 8 : nl <= 12 ? 12 : nl <= 16 ? 16 : nl <= 20 ? 20 : nl <= 24 ? 24 : nl <= 32 ? 32 : 40;
+    int lds_floats = (nmax + 1) * nmax * nmax;
+    if (lds_floats > 64 * 1024 / 4) lds_floats = 0;
     unsigned long long *d_mc = (unsigned long long *)dev_get(C, B_MODCNT, 8, errbuf);
     HIPCHK(hipMemsetAsync(d_mc, 0, 8, s));
-    a.kT = (const float *)upload(C, B_KT, kT.data(), 4 * kT.size(), s, errbuf);
+    a.kT = (const int32_t *)upload(C, B_KT, kT.data(), 4 * kT.size(), s, errbuf);
     a.kTofs = (const int64_t *)upload(C, B_KTOFS, kTofs.data(), 8 * kTofs.size(), s, errbuf);
     a.kTmax = (const double *)upload(C, B_KTMAX, kTmax.data(), 8 * kTmax.size(), s, errbuf);
+    a.n_geo = gin->n_geo;
     a.lds_floats = a.filter ? lds_floats : 0;
     a.primes = (const PfaModP *)upload(C, B_PRIMES, pr.data(), sizeof(PfaModP) * pr.size(), s, errbuf);
     a.np = (int)pr.size();
@@ -1439,6 +1551,7 @@ static void filter_run(DevCtx *C, hipStream_t s, const pfa_run_input *gin,
         HIPCHK(hipMemsetAsync(d_kc, 0, 8, s));
         a.keepkp = (int64_t *)dev_get(C, B_KEEPKP, 8 * a.n, errbuf);
         a.keepnexp = (int64_t *)dev_get(C, B_KEEPNEXP, 8 * a.n, errbuf);
+        a.keepmeta = (int64_t *)dev_get(C, B_KEEPMETA, 8 * a.n, errbuf);
         a.keepcount = d_kc;
         const int pgrid = (int)std::min<int64_t>((a.n + 63) / 64, 65536);
         const size_t plds = (a.M ? (size_t)64 * a.MH * 8 : 0) + (size_t)64 * a.D * 4;
@@ -1451,7 +1564,6 @@ static void filter_run(DevCtx *C, hipStream_t s, const pfa_run_input *gin,
     const int64_t rounds = (a.n + 4 * R - 1) / (4 * R);
     const int grid = (int)std::min<int64_t>(rounds, 16384);
     const size_t lds = (size_t)a.lds_floats * 4;
-    const int nl = a.hl - 1;
 #define PFA_FILTER_LAUNCH(NM)                                                                  \
     do {                                                                                        \
         HIPCHK(hipFuncSetAttribute((const void *)k_filter<NM>,                                 \
@@ -1955,6 +2067,7 @@ static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
 static int validate_run(const pfa_run_input *in, char *errbuf)
 {
     if (in->n_geo < 0 || in->n_p < 0) { snprintf(errbuf, 256, "negative sizes"); return 1; }
+    if (in->n_geo >= (1 << 24)) { snprintf(errbuf, 256, "more than 2^24 geometries"); return 1; }
     for (int g = 0; g < in->n_geo; ++g)
         if (in->h[g] < 3 || in->h[g] > PFA_MAX_H11) {
             snprintf(errbuf, 256, "h11 = %d outside 3..%d", in->h[g], PFA_MAX_H11);

@@ -91,6 +91,15 @@ def cases(seed):
         kap = kappa_dependent(rng, h)
         M = rng.integers(-3, 4, size=(6, h))
         out.append((f"dependent h={h}", [kap], M, None))
+    # kappa near 2^20: N built in double (beyond the 24-bit integer form)
+    # and kappa near 2^28: N still exact, the elimination reaches 2^53 and
+    # the point moves to the modular step
+    for h in (7, 20):
+        for tag, sc in (("double build", 1 << 20), ("elimination overflow", 1 << 28)):
+            for name, kap in (("dense", kappa_dense(rng, h, amp=3, scale=sc)),
+                              ("dependent", kappa_dependent(rng, h, scale=sc))):
+                M = rng.integers(-3, 4, size=(4, h))
+                out.append((f"{tag} {name} h={h}", [kap], M, None))
     # entries of N near 2^40: exact double elimination overflows 2^53
     for h in (6, 12, 20):
         for name, kap in (("dense", kappa_dense(rng, h, scale=1 << 36)),
