@@ -111,6 +111,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the launch-parameter autotuner (library defaults)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only time a plain device copy of MIB MiB (the reference "
                          "kernel for HBM traffic counters) and exit")
@@ -129,7 +131,15 @@ def main():
         return
     inst, geoms, pf, pgeo, dil = workload(args.n_per)
     print(f"# workload: G(20,250,5) {len(pf)} p-vectors", file=sys.stderr, flush=True)
-    wall_b, st_b, r_b = time_mode(gpu, geoms, pf, pgeo, "batched", args.steps, args.warmup)
+    params = {}
+    if not args.no_tune:
+        from pfvs_amd import autotune
+        best = autotune.tune(geoms, pf, pgeo, use_cache=False)
+        params = {k: best[k] for k in ("waves_per_cu", "budget", "stack_cap")}
+        print(f"# autotuned: {params} ({best['ms_search']:.2f} ms search)", file=sys.stderr,
+              flush=True)
+    wall_b, st_b, r_b = time_mode(gpu, geoms, pf, pgeo, "batched", args.steps, args.warmup,
+                                  **params)
     nodes = st_b[-1]["nodes"]
     ms_dev = np.mean([s["ms_total"] for s in st_b])
     ms_search = np.mean([s["ms_search"] for s in st_b])
@@ -144,7 +154,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "int64+f64", "data": "synthetic",
         "config": {"workload": "G(20,250,5), 4096 p-vectors (cone coords <= 12, p_f >= 3), "
-                               "dilation uniform 1..4, M0min 13", "parallelism": "none"},
+                               "dilation uniform 1..4, M0min 13", "parallelism": "none",
+                   "search_params": params or "library defaults"},
         "detail": {"nodes": nodes, "points": st_b[-1]["points"], "kept": st_b[-1]["kept"],
                    "pfvs": st_b[-1]["expansions"], "search_ms": ms_search,
                    "prep_ms": float(np.mean([s["ms_prep"] for s in st_b])),

@@ -89,9 +89,9 @@ typedef struct {
     const int32_t *pgeo;
     int device;
     int mode;
-    int waves_per_cu;     /* batched: resident waves per CU */
-    int budget;           /* batched: steps per wave and pass */
-    int stack_cap;        /* batched: entries per wave stack */
+    int waves_per_cu;     /* batched: persistent wavefronts per CU (default 4) */
+    int budget;           /* batched: most nodes one wavefront donates at once (default 8) */
+    int stack_cap;        /* batched: entries per wavefront deque (default 64 (d + 2)) */
     int host_threads;     /* host mode */
     int filter;
     int sort;             /* 1: group by p and sort by (M, Kn) */

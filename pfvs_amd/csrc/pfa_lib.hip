@@ -1707,8 +1707,10 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     // points buffer: grow and rerun the search on overflow
     unsigned long long pcap = std::max<unsigned long long>(
         std::max<int64_t>(1 << 16, 16 * n_p), C->pcap_hint + C->pcap_hint / 8);
-    const int waves_per_cu = in->waves_per_cu > 0 ? in->waves_per_cu : 8;
-    const int donate_max = in->budget > 0 ? in->budget : 32;
+    // defaults measured on MI355X (h11 = 20 batch): fewer, deeper wavefronts
+    // keep the stacks' hot tops in L2
+    const int waves_per_cu = in->waves_per_cu > 0 ? in->waves_per_cu : 4;
+    const int donate_max = in->budget > 0 ? in->budget : 8;
     const int donate_min = 64;               // deque depth before a wavefront donates
     const int pt_chunk = 256;                // point slots reserved per atomic
     const int stack_cap = in->stack_cap > 0 ? std::max(in->stack_cap, 128) : 64 * (D + 2);
