@@ -41,6 +41,7 @@
 
 // ------------------------------------------------------------ record
 // Per-problem record in 8-byte words; offsets depend on the batch's max d.
+#define PFA_HDR 20                 // header words
 struct RecL {
     int D;
     int U, Uinv, mat, lin, H, L, Bi, ZB, words;
@@ -50,7 +51,7 @@ static RecL rec_layout(int D)
 {
     RecL r;
     r.D = D;
-    int o = 16;
+    int o = PFA_HDR;
     r.U = o; o += D * D;
     r.Uinv = o; o += D;
     r.mat = o; o += D * D;
@@ -66,7 +67,8 @@ static RecL rec_layout(int D)
 // header (int32 view): [0] status [1] d [20] m0_level [21] leafg [22] nL
 // [23] strict [24] h [25] code [26] geo; (int64 view) [1] qmax [2] Q
 // [3] linmin [4..8] Qe QGmul phi slack Kerr (double) [9] hmask
-// [14] max |Binter| [15] max |ZB|
+// [14] max |Binter| [15] max |ZB| [16] max |mat| [17] max |H| of the
+// pruning rows (int64 bounds for the search's exact prefix sums)
 PFA_HD static inline PfaProb rec_view(const int64_t *r, const RecL &L)
 {
     const int32_t *ri = (const int32_t *)r;
@@ -86,6 +88,8 @@ PFA_HD static inline PfaProb rec_view(const int64_t *r, const RecL &L)
     P.slack = rd[7];
     P.Kerr = rd[8];
     P.hmask = (uint64_t)r[9];
+    P.matmax = r[16];
+    P.hmax = r[17];
     P.U = (const double *)(r + L.U);
     P.Uinv = (const double *)(r + L.Uinv);
     P.mat = r + L.mat;
@@ -127,7 +131,7 @@ PFA_HD static inline void prep_one(const PfaTeam &T, char *ws_hot, char *ws_cold
     int32_t *ri = (int32_t *)rec;
     const int64_t *p = ps + ip * MH;
     if (T.lane == 0) {
-        for (int k = 0; k < 16; ++k) rec[k] = 0;
+        for (int k = 0; k < PFA_HDR; ++k) rec[k] = 0;
         ri[0] = PFA_HOST;
         ri[1] = d;
         rec[1] = G.qmax[geo];
@@ -222,6 +226,15 @@ PFA_HD static inline void prep_one(const PfaTeam &T, char *ws_hot, char *ws_cold
     PFA_SYNC();
     if (T.lane == 0) {
         rec[9] = (int64_t)hmask;
+        // bounds of the search's exact prefix sums
+        int64_t mm = 0, hm2 = 0;
+        for (int e = 0; e < d * d; ++e) {
+            const int64_t x = rM[e] < 0 ? -rM[e] : rM[e], y = rH[e] < 0 ? -rH[e] : rH[e];
+            mm = x > mm ? x : mm;
+            hm2 = y > hm2 ? y : hm2;
+        }
+        rec[16] = mm;
+        rec[17] = hm2;
         ri[20] = m0;
         ri[21] = leafg;
         ri[22] = leafg ? d : 0;
@@ -1744,11 +1757,11 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
             // the search may change statuses: keep (first attempt) or restore
             // (later attempts) the headers as the preparation left them
             if (attempt == 0) {
-                hdr0.resize((size_t)n_p * 16);
-                HIPCHK(hipMemcpy2DAsync(hdr0.data(), 16 * 8, d_recs, (size_t)L.words * 8, 16 * 8,
+                hdr0.resize((size_t)n_p * PFA_HDR);
+                HIPCHK(hipMemcpy2DAsync(hdr0.data(), PFA_HDR * 8, d_recs, (size_t)L.words * 8, PFA_HDR * 8,
                                         n_p, hipMemcpyDeviceToHost, s));
             } else {
-                HIPCHK(hipMemcpy2DAsync(d_recs, (size_t)L.words * 8, hdr0.data(), 16 * 8, 16 * 8,
+                HIPCHK(hipMemcpy2DAsync(d_recs, (size_t)L.words * 8, hdr0.data(), PFA_HDR * 8, PFA_HDR * 8,
                                         n_p, hipMemcpyHostToDevice, s));
             }
         }
@@ -1861,7 +1874,7 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     std::vector<int32_t> h_pidx(nh), h_status(n_p), h_code(n_p);
     std::vector<int64_t> h_q(nh), h_M(nh * HS), h_Kn(nh * HS), h_nexp(in->filter ? nh : 0);
     std::vector<int8_t> h_ns(in->filter ? nh : 0), h_bad(n_p);
-    std::vector<int64_t> hdr((size_t)n_p * 16);
+    std::vector<int64_t> hdr((size_t)n_p * PFA_HDR);
     unsigned long long htot[8] = {0};
     HIPCHK(hipMemcpyAsync(htot, d_tot, 64, hipMemcpyDeviceToHost, s));
     if (nh > 0) {
@@ -1875,7 +1888,7 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
         }
     }
     if (n_p > 0) {
-        HIPCHK(hipMemcpy2DAsync(hdr.data(), 16 * 8, d_recs, (size_t)L.words * 8, 16 * 8, n_p,
+        HIPCHK(hipMemcpy2DAsync(hdr.data(), PFA_HDR * 8, d_recs, (size_t)L.words * 8, PFA_HDR * 8, n_p,
                                 hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(h_bad.data(), d_bad, n_p, hipMemcpyDeviceToHost, s));
     }
@@ -1883,7 +1896,7 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     HIPCHK(hipMemcpyAsync(hc, d_cnt, 64, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     for (int64_t ip = 0; ip < n_p; ++ip) {
-        const int32_t *ri = (const int32_t *)&hdr[ip * 16];
+        const int32_t *ri = (const int32_t *)&hdr[ip * PFA_HDR];
         h_status[ip] = ri[0];
         h_code[ip] = ri[25];
     }
@@ -2370,6 +2383,15 @@ extern "C" int pfa_kernel_batch(const pfa_kernel_input *in, pfa_kernel_output *o
             }
             oh += (int64_t)in->nh[k] * d;
             r[9] = (int64_t)hm;
+            int64_t mm = 0, hm2 = 0;
+            for (int e = 0; e < d * d; ++e) {
+                const int64_t x = r[L.mat + e] < 0 ? -r[L.mat + e] : r[L.mat + e];
+                const int64_t y = r[L.H + e] < 0 ? -r[L.H + e] : r[L.H + e];
+                mm = x > mm ? x : mm;
+                hm2 = y > hm2 ? y : hm2;
+            }
+            r[16] = mm;
+            r[17] = hm2;
             ri[21] = in->nl[k] ? 2 : 0;
             ri[22] = in->nl[k];
             for (int e = 0; e < in->nl[k] * d * 2; ++e) r[L.L + e] = in->L[ol + e];

@@ -46,11 +46,13 @@ struct PfaProb {
     const int64_t *H;                 // (d, d): row of level i at H + i d
     const int64_t *L;                 // leaf gcd rows (nL, d) as int128 pairs, when leafg
     int nL;
+    int64_t matmax, hmax;             // max |mat|, max |H| over the pruning rows
 };
 
 // One node: level i with c[i+1:] fixed; c[i] runs over [cur, hi].
 struct PfaNode {
-    int32_t i, cur, hi, pad;
+    int32_t i, cur, hi;
+    int32_t cmax;    // max_{j>i} |c_j|
     double rem, err, off, aoff;
     pfa_i128 m0p;    // sum_{j>i} linvec_j c_j (exact)
     pfa_i128 pre;    // sum_{j>i} H_ij c_j for the row of level i
@@ -111,25 +113,43 @@ PFA_HD static inline int pfa_enter(const PfaProb &P, const int32_t *c, PfaNode &
 {
     const int d = P.d, i = nd.i;
     const double *Ui = P.U + (size_t)i * d;
+    const int64_t *Mi = P.mat + (size_t)i * d;
+    const int64_t *Hi = P.H + (size_t)i * d;
+    const int hrow = (int)((P.hmask >> i) & 1);
     double o = 0.0, a = 0.0;
-    for (int j = i + 1; j < d; ++j) {
-        double t = Ui[j] * (double)c[j];
-        o += t;
-        a += fabs(t);
+    // the exact prefixes in int64 when |mat|, |H| < 2^31 and every partial
+    // sum is bounded by max(|mat|, |H|) max|c_j| (d - 1 - i) < 2^62
+    const int64_t B31 = (int64_t)1 << 31, bm = P.matmax > P.hmax ? P.matmax : P.hmax;
+    if (bm < B31 && (double)bm * (double)nd.cmax * (double)(d - 1 - i) < 4.0e18) {
+        int64_t mr = 0, sp = 0;
+        for (int j = i + 1; j < d; ++j) {
+            const int32_t cj = c[j];
+            double t = Ui[j] * (double)cj;
+            o += t;
+            a += fabs(t);
+            mr += (int64_t)(int32_t)Mi[j] * cj;
+            sp += (int64_t)(int32_t)Hi[j] * cj;
+        }
+        nd.mrow = mr;
+        nd.pre = hrow ? sp : 0;
+    } else {
+        for (int j = i + 1; j < d; ++j) {
+            double t = Ui[j] * (double)c[j];
+            o += t;
+            a += fabs(t);
+        }
+        pfa_i128 mr = 0;
+        for (int j = i + 1; j < d; ++j) mr += (pfa_i128)Mi[j] * c[j];
+        nd.mrow = mr;
+        nd.pre = 0;
+        if (hrow) {
+            pfa_i128 s = 0;
+            for (int j = i + 1; j < d; ++j) s += (pfa_i128)Hi[j] * c[j];
+            nd.pre = s;
+        }
     }
     nd.off = o;
     nd.aoff = a;
-    const int64_t *Mi = P.mat + (size_t)i * d;
-    pfa_i128 mr = 0;
-    for (int j = i + 1; j < d; ++j) mr += (pfa_i128)Mi[j] * c[j];
-    nd.mrow = mr;
-    nd.pre = 0;
-    if ((P.hmask >> i) & 1) {
-        const int64_t *Hi = P.H + (size_t)i * d;
-        pfa_i128 s = 0;
-        for (int j = i + 1; j < d; ++j) s += (pfa_i128)Hi[j] * c[j];
-        nd.pre = s;
-    }
     const double slack = P.slack, Kerr = P.Kerr;
     double R = sqrt(fmax(nd.rem + slack + nd.err, 0.0));
     R += Kerr * (R + 2.0 * a) + 1e-300;
@@ -217,6 +237,8 @@ PFA_HD static inline int pfa_cand(const PfaProb &P, const PfaNode &nd, int32_t v
     ch.m0p = m0;
     ch.g = gn;
     ch.qp = q;
+    const int32_t av = v < 0 ? -v : v;
+    ch.cmax = av > nd.cmax ? av : nd.cmax;
     return 1;
 }
 
@@ -291,6 +313,7 @@ PFA_HD static inline void pfa_root(const PfaProb &P, PfaNode &nd)
     nd.mrow = 0;
     nd.cur = 1;
     nd.hi = 0;
+    nd.cmax = 0;
 }
 
 // 1 if the problem has no node at all (M0 identically 0 below linmin, or
@@ -347,6 +370,7 @@ PFA_HD static inline int pfa_dfs_t(const PfaProb &P, PfaNode *st, int32_t *c, Pf
             st[i - 1].m0p = ch.m0p;
             st[i - 1].g = ch.g;
             st[i - 1].qp = ch.qp;
+            st[i - 1].cmax = ch.cmax;
             i--;
             if (pfa_enter(P, c, st[i])) return PFA_I32;
             entered(i);
