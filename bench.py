@@ -6,7 +6,10 @@ sample of the same p-vectors.
 
 One step = one pass of the device pipeline (preparation, frontier roots,
 search, post-processing with the exact filter) over the whole batch, inputs
-resident in HBM.  Prints one JSON line.
+resident in HBM.  Prints one JSON line.  --workload mixed (all four
+instances of the mixed batch in one launch) and --workload large (deep
+h11 = 30 trees, reporting the exact-host fallback rate) print the same
+line for those batches.
 
 Roofline of the search kernel (k_batched): VALU issue.  Its algorithmic
 work W per launch is counted in lane-ops on the host path, which visits the
@@ -33,17 +36,46 @@ sys.path.insert(0, ROOT)
 P_VALU = None   # lane-ops/s, from the device's CU count and clock
 
 
-def workload(n_per=4096):
-    from pfvs_amd.instances import BATCH_MEMBERS, config_batch
-    members = [m for m in BATCH_MEMBERS if m[0] == 20]
-    (inst, ps, dil), = config_batch(n_per=n_per, members=members)
-    geoms, pgeo = [], np.zeros(len(ps), np.int32)
+def _dilation_geoms(inst, dil):
+    geoms, pgeo = [], np.zeros(len(dil), np.int32)
     kap, Mb = inst.data.kappa_cob, inst.data.M_lattice()
-    for k, D in enumerate([1, 2, 3, 4]):
+    for k, D in enumerate(sorted(set(int(x) for x in dil))):
         geoms.append(dict(kappa=kap, Mbasis=Mb, Q=inst.Q, dilation=D, M0min=13))
         pgeo[dil == D] = k
-    pf = np.hstack([np.zeros((len(ps), 1), np.int64), ps])
-    return inst, geoms, pf, pgeo, dil
+    return geoms, pgeo
+
+
+def workload(name="headline", n_per=4096):
+    """(description, geoms, pf (p-vectors with the coni entry), pgeo, cpu
+    items: per p-vector (kappa, Mbasis, Q, qmax))."""
+    from pfvs_amd.instances import (BATCH_MEMBERS, config_batch, config_large_tree,
+                                    geoms_for_qmax)
+    if name in ("headline", "mixed"):
+        members = [m for m in BATCH_MEMBERS if m[0] == 20] if name == "headline" else BATCH_MEMBERS
+        geoms, pfs, pgeos, items = [], [], [], []
+        for inst, ps, dil in config_batch(n_per=n_per, members=members):
+            g, pg = _dilation_geoms(inst, dil)
+            pgeos.append(pg + len(geoms))
+            geoms += g
+            pfs.append(np.hstack([np.zeros((len(ps), 1), np.int64), ps]))
+            kap, Mb = inst.data.kappa_cob, inst.data.M_lattice()
+            items += [(kap, Mb, inst.Q, int(np.floor(int(D) * inst.Q + 1e-7))) for D in dil]
+        desc = ("G(20,250,5), %d p-vectors (cone coords <= 12, p_f >= 3), dilation uniform 1..4, "
+                "M0min 13" % n_per if name == "headline" else
+                "G(6,90,2)+G(10,150,3)+G(14,200,4)+G(20,250,5), %d p-vectors each (cone coords "
+                "<= 12, p_f >= floor(4Q/338)), dilation uniform 1..4, M0min 13, one launch" % n_per)
+        w = max(p.shape[1] for p in pfs)
+        pf = np.vstack([np.pad(p, ((0, 0), (0, w - p.shape[1]))) for p in pfs])
+        return desc, geoms, pf, np.concatenate(pgeos), items
+    if name == "large":
+        inst, ps, qmax = config_large_tree()
+        geoms, pgeo = geoms_for_qmax(inst, qmax)
+        kap, Mb = inst.data.kappa_cob, inst.data.M_lattice()
+        items = [(kap, Mb, inst.Q, int(q)) for q in qmax]
+        desc = ("G(30,300,6), %d p-vectors (cone coords <= 20, p_f <= 3), per-p qmax = 338 p_f + b, "
+                "b uniform 8..14, M0min 13" % len(ps))
+        return desc, geoms, np.hstack([np.zeros((len(ps), 1), np.int64), ps]), pgeo, items
+    raise SystemExit(f"unknown workload {name}")
 
 
 def time_mode(gpu, geoms, pf, pgeo, mode, steps, warmup, **kw):
@@ -83,17 +115,16 @@ def measured_traffic(kernel="k_batched"):
     return None
 
 
-def cpu_baseline(inst, pf, dil, budget_s=15.0):
+def cpu_baseline(items, pf, budget_s=15.0):
     """The oracle (single thread) over p-vectors of the same batch, in a
     fixed seeded order, until budget_s seconds have elapsed."""
     from oracle import cpu
-    kap, Mb = inst.data.kappa_cob, inst.data.M_lattice()
     order = np.random.default_rng(7).permutation(len(pf))
     nodes = done = 0
     t0 = time.perf_counter()
     for i in order:
-        M, K, q, pidx, st, cnt = cpu.coni_batch(kap, Mb, pf[i:i + 1], inst.Q,
-                                                cpu.qmax_of(inst.Q, int(dil[i])))
+        kap, Mb, Q, qmax = items[i]
+        M, K, q, pidx, st, cnt = cpu.coni_batch(kap, Mb, pf[i:i + 1, :kap.shape[0]], Q, qmax)
         nodes += cnt[0]
         done += 1
         if time.perf_counter() - t0 > budget_s:
@@ -108,6 +139,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n-per", type=int, default=4096)
+    ap.add_argument("--workload", default="headline", choices=("headline", "mixed", "large"),
+                    help="headline: the h11 = 20 batch (the metric); mixed: all four "
+                         "instances in one launch; large: deep h11 = 30 trees")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -129,8 +163,9 @@ def main():
         print(json.dumps({"copy_bytes": nb, "copies": args.steps + 1, "ms_per_copy": ms,
                           "GB_per_s": 2 * nb / (ms * 1e-3) / 1e9}))
         return
-    inst, geoms, pf, pgeo, dil = workload(args.n_per)
-    print(f"# workload: G(20,250,5) {len(pf)} p-vectors", file=sys.stderr, flush=True)
+    desc, geoms, pf, pgeo, items = workload(args.workload, args.n_per)
+    print(f"# workload {args.workload}: {len(pf)} p-vectors, {len(geoms)} geometry entries",
+          file=sys.stderr, flush=True)
     params = {}
     if not args.no_tune:
         from pfvs_amd import autotune
@@ -145,16 +180,18 @@ def main():
     ms_search = np.mean([s["ms_search"] for s in st_b])
     print(f"# batched: {nodes:.0f} nodes, device {ms_dev:.2f} ms, search {ms_search:.2f} ms, "
           f"wall {wall_b*1e3:.1f} ms, passes {st_b[-1]['passes']:.0f}", file=sys.stderr, flush=True)
+    metric = {"headline": "FP nodes/s (batched persistent coniZpM, h11=20)",
+              "mixed": "FP nodes/s (batched persistent coniZpM, mixed h11 = 6/10/14/20)",
+              "large": "FP nodes/s (batched persistent coniZpM, deep h11 = 30 trees)"}[args.workload]
     out = {
-        "metric": "FP nodes/s (batched persistent coniZpM, h11=20)",
+        "metric": metric,
         "value": nodes / (ms_dev * 1e-3),
         "unit": "nodes/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_dev,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "int64+f64", "data": "synthetic",
-        "config": {"workload": "G(20,250,5), 4096 p-vectors (cone coords <= 12, p_f >= 3), "
-                               "dilation uniform 1..4, M0min 13", "parallelism": "none",
+        "config": {"workload": desc, "parallelism": "none",
                    "search_params": params or "library defaults"},
         "detail": {"nodes": nodes, "points": st_b[-1]["points"], "kept": st_b[-1]["kept"],
                    "pfvs": st_b[-1]["expansions"], "search_ms": ms_search,
@@ -165,6 +202,7 @@ def main():
                    "wait_frac": st_b[-1]["wait_cycles"] / max(st_b[-1]["wave_cycles"], 1),
                    "donated": st_b[-1]["donated"], "fetches": st_b[-1]["fetches"],
                    "coniPFVs_per_s": st_b[-1]["expansions"] / (ms_dev * 1e-3),
+                   "host_fallback_rate": st_b[-1]["host_p"] / max(len(pf), 1),
                    "wall_ms": wall_b * 1e3},
     }
     work, wnodes, wthreads = search_work(gpu, geoms, pf, pgeo)
@@ -174,7 +212,7 @@ def main():
     achieved = work / (ms_search * 1e-3)
     out["roofline"] = {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
                        "unit": "T lane-op/s", "frac": achieved / peak,
-                       "traffic": measured_traffic("k_batched"),
+                       "traffic": measured_traffic("k_batched") if args.workload == "headline" else None,
                        "kernel": "k_batched", "work_per_launch": work,
                        "ms_per_launch": ms_search,
                        "peak_basis": f"{cus} CUs x 64 lanes x {clk_khz / 1e6:.2f} GHz"}
@@ -186,7 +224,7 @@ def main():
         out["detail"]["batched_over_plain"] = out["value"] / out["detail"]["plain_nodes_per_s"]
         assert st_p[-1]["nodes"] == nodes, "plain and batched forms visited different nodes"
     if not args.no_cpu:
-        v, done, t = cpu_baseline(inst, pf, dil, args.cpu_seconds)
+        v, done, t = cpu_baseline(items, pf, args.cpu_seconds)
         out["cpu_baseline"] = {"value": v, "unit": "nodes/s", "cores": 1, "kind": "port",
                                "sample": f"{done} p-vectors of the same batch in {t:.1f} s "
                                          "(seeded order)"}

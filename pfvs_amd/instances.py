@@ -173,15 +173,18 @@ def pf_floor(Q: int, dmax: float) -> int:
     return max(1, int(math.floor(dmax * Q + 1e-7)) // 338)
 
 
-def random_pvecs(inst: Instance, n: int, E: int, seed: int, pf_min: int = 1) -> np.ndarray:
+def random_pvecs(inst: Instance, n: int, E: int, seed: int, pf_min: int = 1,
+                 pf_max: int | None = None, return_cone: bool = False):
     """n distinct primitive admissible p with cone coordinates uniform in
-    [1, E] and p_f in [pf_min, E] (seeded), in the data's basis without the
-    coni entry."""
+    [1, E] and p_f in [pf_min, pf_max] (pf_max default E) (seeded), in the
+    data's basis without the coni entry; with return_cone also the cone
+    coordinates (p_f, x_a, x_b, p_k...) of each."""
     h = inst.h11
     rng = np.random.default_rng(seed)
-    seen, out = set(), []
+    seen, out, cone = set(), [], []
     tries = 0
-    total = (E - pf_min + 1) * E ** (h - 2)
+    pf_hi = E if pf_max is None else pf_max
+    total = (pf_hi - pf_min + 1) * E ** (h - 2)
     if n > total:
         raise ValueError("fewer admissible p-vectors than requested")
     while len(out) < n:
@@ -189,7 +192,7 @@ def random_pvecs(inst: Instance, n: int, E: int, seed: int, pf_min: int = 1) -> 
         if tries > 50 * n + 1000:
             raise RuntimeError("could not draw enough distinct p-vectors")
         t = rng.integers(1, E + 1, h - 1)
-        t[0] = rng.integers(pf_min, E + 1)
+        t[0] = rng.integers(pf_min, pf_hi + 1)
         key = tuple(int(x) for x in t)
         if key in seen:
             continue
@@ -199,7 +202,9 @@ def random_pvecs(inst: Instance, n: int, E: int, seed: int, pf_min: int = 1) -> 
         if math.gcd(*[int(x) for x in p]) != 1:
             continue
         out.append(inst.UinvT @ p)
-    return np.array(out, dtype=np.int64)[:, 1:]
+        cone.append(t.astype(np.int64))
+    ps = np.array(out, dtype=np.int64)[:, 1:]
+    return (ps, np.array(cone, dtype=np.int64)) if return_cone else ps
 
 
 # The workloads (Q = h11 + h21 + 2, M0min = 13 throughout)
@@ -232,7 +237,29 @@ def config_batch(n_per=4096, E=12, members=BATCH_MEMBERS, seed_offset=0):
     return out
 
 
-def config_large_tree(n=2048, E=20):
-    """G(30, 300, 6): n p-vectors with cone coordinates <= E, dilation 3."""
+def config_large_tree(n=2048, E=20, pf_max=3, band=(8, 14), seed=1006):
+    """G(30, 300, 6): n p-vectors with cone coordinates <= E and p_f <= pf_max,
+    each with its own search bound qmax = 338 p_f + b, b uniform in band.
+
+    On G the tree of p is nonempty iff 338 p_f <= qmax, and at h11 = 30 it
+    grows very fast with the band qmax - 338 p_f: bands near 10 give trees
+    of 10^4 - 10^5 nodes (29 levels), bands above ~25 more than 10^6 and a
+    dilation of 3 (band >= 320 for every nonempty tree) far more than any
+    search can enumerate.  The per-p band keeps every tree large, deep,
+    uneven and finite.  Returns (inst, ps, qmax per p)."""
     inst = make_G(30, 300, 6)
-    return inst, random_pvecs(inst, n, E, seed=1006, pf_min=pf_floor(inst.Q, 3)), 3
+    ps, cone = random_pvecs(inst, n, E, seed=seed, pf_min=1, pf_max=pf_max, return_cone=True)
+    b = np.random.default_rng(seed + 1).integers(band[0], band[1] + 1, len(ps))
+    qmax = 338 * cone[:, 0].astype(np.int64) + b
+    return inst, ps, qmax
+
+def geoms_for_qmax(inst: Instance, qmax, M0min: int = 13):
+    """One geometry entry per distinct search bound (dilation = qmax / Q,
+    which reproduces qmax exactly) and the entry of each p: (geoms, pgeo)."""
+    qmax = np.asarray(qmax, dtype=np.int64)
+    kap, Mb = inst.data.kappa_cob, inst.data.M_lattice()
+    vals = sorted(set(int(x) for x in qmax))
+    index = {v: k for k, v in enumerate(vals)}
+    geoms = [dict(kappa=kap, Mbasis=Mb, Q=inst.Q, dilation=v / inst.Q, M0min=M0min) for v in vals]
+    pgeo = np.array([index[int(x)] for x in qmax], dtype=np.int32)
+    return geoms, pgeo

@@ -64,3 +64,54 @@ def test_gpu_batched_small_stacks_donate(need_gpu):
     assert canon_points(a["M"], a["Kn"], a["q"], a["pidx"], 10) == \
         canon_points(b["M"], b["Kn"], b["q"], b["pidx"], 10)
     assert a["stats"]["nodes"] == b["stats"]["nodes"]
+
+
+def test_gpu_large_tree_subset_matches_oracle(need_gpu):
+    """Deep h11 = 30 trees (29 levels) with per-p search bounds, batched in
+    one launch: identical points and node counts to the oracle."""
+    from pfvs_amd.instances import config_large_tree, geoms_for_qmax
+    inst, ps, qmax = config_large_tree(n=24, band=(8, 10))
+    geoms, pgeo = geoms_for_qmax(inst, qmax)
+    pf = np.hstack([np.zeros((len(ps), 1), np.int64), ps])
+    kap, Mb = inst.data.kappa_cob, inst.data.M_lattice()
+    exp, nodes = [], 0
+    for i in range(len(pf)):
+        M, K, q, pidx, st, cnt = cpu.coni_batch(kap, Mb, pf[i:i + 1], inst.Q, int(qmax[i]))
+        assert st[0] == 0
+        exp += canon_points(M, K, q, pidx + i, 30)
+        nodes += cnt[0]
+    exp.sort()
+    print(f"h=30: oracle {len(exp)} points, {nodes} nodes", flush=True)
+    r = need_gpu.run(geoms, pf, pgeo, mode="batched", filter=False)
+    hst = need_gpu.run(geoms, pf, pgeo, mode="host", filter=False)
+    assert np.all(r["pstat"] == 0)
+    assert canon_points(r["M"], r["Kn"], r["q"], r["pidx"], 30) == exp
+    # the product prunes with the FP64 Cholesky factor widened by its proven
+    # margin (the oracle with the exactly computed factor): at 29 levels a
+    # few more nodes survive; device and host code visit the same ones
+    assert int(r["stats"]["nodes"]) == int(hst["stats"]["nodes"])
+    assert nodes <= int(r["stats"]["nodes"]) <= nodes * 1.001
+
+
+def test_gpu_mixed_h_batch_filter_matches_host(need_gpu):
+    """All four instances of the mixed batch (h11 = 6, 10, 14, 20) in one
+    launch with the device filter on: the same rows, expansion counts and
+    det verdicts as the host code."""
+    from pfvs_amd.instances import config_batch
+    geoms, pfs, pgeos = [], [], []
+    for inst, ps, dil in config_batch(n_per=24):
+        kap, Mb = inst.data.kappa_cob, inst.data.M_lattice()
+        pg = np.zeros(len(ps), np.int32)
+        for D in sorted(set(int(x) for x in dil)):
+            pg[dil == D] = len(geoms)
+            geoms.append(dict(kappa=kap, Mbasis=Mb, Q=inst.Q, dilation=D, M0min=13))
+        pgeos.append(pg)
+        pfs.append(np.hstack([np.zeros((len(ps), 1), np.int64), ps]))
+    pf, pgeo = np.vstack([np.pad(p, ((0, 0), (0, 21 - p.shape[1]))) for p in pfs]), np.concatenate(pgeos)
+    d = need_gpu.run(geoms, pf, pgeo, mode="batched", filter=True)
+    h = need_gpu.run(geoms, pf, pgeo, mode="host", filter=True)
+    print(f"mixed: {len(d['q'])} points, kept {d['stats']['kept']:.0f}", flush=True)
+    assert (d["pstat"] == h["pstat"]).all()
+    for k in ("M", "Kn", "q", "pidx", "nexp", "nonsing"):
+        assert (d[k] == h[k]).all(), k
+    assert d["stats"]["nodes"] == h["stats"]["nodes"]
