@@ -133,15 +133,69 @@ def cpu_baseline(items, pf, budget_s=15.0):
     return nodes / t, done, t
 
 
+def serving_bench(gpu, args):
+    """Requests r = 0 .. R-1 of the serving stream (h11 uniform in 5..40,
+    log-uniform sizes up to 4096 p-vectors, band-limited bounds) arrive as a
+    seeded Poisson process at args.rate per second; every response is
+    checked against the same request run alone."""
+    from pfvs_amd import coni
+    from pfvs_amd.instances import geoms_for_qmax, serving_request
+    from pfvs_amd.serve import Server
+    reqs = [serving_request(r) for r in range(args.requests)]
+    data = [(inst.data.kappa_cob, inst.data.M_lattice(), inst.Q, ps, qmax)
+            for inst, ps, qmax in reqs]
+    gaps = np.random.default_rng(11).exponential(1.0 / args.rate, len(reqs))
+    srv = Server().start()
+    srv.submit(*data[0]).wait()                  # warm: code and buffers
+    srv.jobs.clear()
+    srv.iterations.clear()
+    jobs = []
+    t0 = time.perf_counter()
+    for k, dt in enumerate(gaps):
+        t0 += dt
+        while time.perf_counter() < t0:
+            time.sleep(min(0.001, max(0.0, t0 - time.perf_counter())))
+        jobs.append(srv.submit(*data[k]))
+    for j in jobs:
+        j.wait(timeout=600)
+    srv.stop(timeout=60)
+    st = srv.stats()
+    exact = 0
+    for (inst, ps, qmax), (kap, Mb, Q, _, _), j in zip(reqs, data, jobs):
+        geoms, pgeo = geoms_for_qmax(inst, qmax)
+        pf = np.hstack([np.zeros((len(ps), 1), np.int64), ps])
+        r = gpu.run(geoms, pf, pgeo, mode="batched", filter=True, sort=True)
+        Ks, Ms = coni.assemble(r, ps, pf, kap, Mb, Q, qmax, 13)
+        exact += int(np.array_equal(Ks, j.result[0]) and np.array_equal(Ms, j.result[1]))
+    hs = [inst.h11 for inst, _, _ in reqs]
+    return {
+        "metric": "coniZpM serving: aggregate FP nodes/s under a Poisson stream",
+        "value": st["nodes_per_s"], "unit": "nodes/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": st["span_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int64+f64", "data": "synthetic",
+        "config": {"workload": f"{len(reqs)} requests G(h, 10h, r), h uniform 5..40 (here "
+                               f"{min(hs)}..{max(hs)}), N log-uniform 1..4096, cone coords <= 16, "
+                               f"p_f <= 3, qmax = 338 p_f + b (b <= band(h)); Poisson "
+                               f"{args.rate}/s", "parallelism": "none"},
+        "detail": dict(st, p_vectors=int(sum(len(ps) for _, ps, _ in reqs)),
+                       bit_exact_alone_vs_mix=f"{exact}/{len(reqs)}"),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n-per", type=int, default=4096)
-    ap.add_argument("--workload", default="headline", choices=("headline", "mixed", "large"),
+    ap.add_argument("--workload", default="headline",
+                    choices=("headline", "mixed", "large", "serving"),
                     help="headline: the h11 = 20 batch (the metric); mixed: all four "
-                         "instances in one launch; large: deep h11 = 30 trees")
+                         "instances in one launch; large: deep h11 = 30 trees; serving: a "
+                         "Poisson stream of coniZpM requests through the continuous-batching "
+                         "server")
+    ap.add_argument("--requests", type=int, default=64, help="serving: number of requests")
+    ap.add_argument("--rate", type=float, default=40.0, help="serving: arrivals per second")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -162,6 +216,9 @@ def main():
         ms = gpu.copy_probe(nb, reps=args.steps)
         print(json.dumps({"copy_bytes": nb, "copies": args.steps + 1, "ms_per_copy": ms,
                           "GB_per_s": 2 * nb / (ms * 1e-3) / 1e9}))
+        return
+    if args.workload == "serving":
+        print(json.dumps(serving_bench(gpu, args)))
         return
     desc, geoms, pf, pgeo, items = workload(args.workload, args.n_per)
     print(f"# workload {args.workload}: {len(pf)} p-vectors, {len(geoms)} geometry entries",

@@ -186,6 +186,7 @@ def _exact_host_path(kappa, Mbasis, p_full, Q, qmax, M0min, max_N):
         Tm = np.array([list(r) for r in K] + [w], dtype=object).T
         B, ZB = B @ Tm, ZB @ Tm
         mat = Tm.T @ mat @ Tm
+        l = [int(x) for x in B[0]]                 # M0 = B[0] c in the new basis: (0, .., 0, g)
     if any(abs(int(x)) >= 2**63 for x in mat.ravel()):
         raise IncompleteSearchError(
             f"p={list(map(int, p_full[1:]))}: ellipsoid matrix entries exceed int64")
@@ -244,6 +245,21 @@ def coniZpM(data, ps, Q=None, M0min=13, ellipsoid_dilation=1, use_gcd_lattice=Fa
         warnings.warn(f"GPU path failed ({e}); using the CPU", RuntimeWarning, stacklevel=2)
         r = gpu.run([geo], p_full, np.zeros(len(ps), np.int32), mode="host", filter=True,
                     sort=True, host_threads=threads)
+    Ks, Ms = assemble(r, ps, p_full, kappa, Mbasis, Q, np.full(len(ps), qmax), M0min,
+                      max_N_pfvs, verbosity)
+    if return_formal_pfvs:
+        raise NotImplementedError("PFV objects are not provided; use (Ks, Ms)")
+    return Ks, Ms
+
+
+def assemble(r, ps, p_full, kappa, Mbasis, Q, qmax, M0min, max_N_pfvs=1_000_000_000,
+             verbosity=0):
+    """(Ks, Ms) of one geometry's p-vectors from a pipeline result r (rows
+    M, Kn, q, nonsing grouped by the p-vectors' indices pidx into ps, and
+    pstat per p-vector): the status errors, the exact host path for every
+    p-vector with pstat 1 (qmax per p-vector) and the post-processing of
+    pfvs/coni.py:275-412, in the reference's order."""
+    h = np.asarray(kappa).shape[0]
     pstat = r["pstat"]
     counts = np.bincount(r["pidx"], minlength=len(ps)) if len(r["pidx"]) else np.zeros(len(ps), int)
     for ip in np.flatnonzero(counts > max_N_pfvs):
@@ -253,18 +269,15 @@ def coniZpM(data, ps, Q=None, M0min=13, ellipsoid_dilation=1, use_gcd_lattice=Fa
     if verbosity >= 1:
         print(f"found {len(r['q'])} lattice points for {len(ps)} p-vectors "
               f"({int(np.sum(pstat == 1))} handled by the exact host path)...")
-    h = h11
     pieces = [_pfvs_from_points(r["M"][:, :h].T, r["Kn"][:, :h].T, r["q"], r["pidx"], kappa, h,
                                 Q, M0min, nonsing=r["nonsing"], verbosity=verbosity)]
     for ip in np.flatnonzero(pstat == 1):
-        Mv, Kv, qv = _exact_host_path(kappa, Mbasis, p_full[ip], Q, qmax, M0min, max_N_pfvs)
+        Mv, Kv, qv = _exact_host_path(kappa, Mbasis, p_full[ip], Q, int(qmax[ip]), M0min,
+                                      max_N_pfvs)
         pieces.append(_pfvs_from_points(Mv, Kv, qv, np.full(len(qv), ip), kappa, h, Q, M0min,
                                         verbosity=verbosity))
     Ks = np.vstack([pc[0] for pc in pieces])
     Ms = np.vstack([pc[1] for pc in pieces])
     keys = np.concatenate([pc[2] for pc in pieces])
     order = np.argsort(keys, kind="stable")
-    Ks, Ms = Ks[order], Ms[order]
-    if return_formal_pfvs:
-        raise NotImplementedError("PFV objects are not provided; use (Ks, Ms)")
-    return Ks, Ms
+    return Ks[order], Ms[order]

@@ -263,3 +263,24 @@ def geoms_for_qmax(inst: Instance, qmax, M0min: int = 13):
     geoms = [dict(kappa=kap, Mbasis=Mb, Q=inst.Q, dilation=v / inst.Q, M0min=M0min) for v in vals]
     pgeo = np.array([index[int(x)] for x in qmax], dtype=np.int32)
     return geoms, pgeo
+
+
+def serving_band_max(h: int) -> int:
+    """Largest band qmax - 338 p_f of the serving stream at h11 = h (trees of
+    at most ~10^5 - 10^6 nodes; the band's effect grows with h11)."""
+    return 30 if h < 15 else 12 if h <= 25 else 10 if h <= 32 else 8
+
+
+def serving_request(r: int, n_max: int = 4096, E: int = 16):
+    """Request r of the serving stream: G(h, 10 h, seed = r) with h uniform in
+    5..40, N log-uniform in 1..n_max p-vectors (cone coordinates <= E, p_f
+    <= 3) and per-p bounds qmax = 338 p_f + b, b uniform in 4..band_max(h)
+    (all seeded by r).  Returns (inst, ps, qmax)."""
+    rng = np.random.default_rng(5000 + r)
+    h = int(rng.integers(5, 41))
+    n = int(round(math.exp(rng.uniform(0.0, math.log(n_max)))))
+    inst = make_G(h, 10 * h, r)
+    n = min(n, (3 * E ** (h - 2)) // 2)
+    ps, cone = random_pvecs(inst, n, E, seed=7000 + r, pf_min=1, pf_max=3, return_cone=True)
+    b = rng.integers(4, serving_band_max(h) + 1, len(ps))
+    return inst, ps, 338 * cone[:, 0].astype(np.int64) + b

@@ -40,3 +40,30 @@ def test_host_pipeline_matches_oracle(h, h21, seed, E, dil, n):
     assert canon_points(r["M"], r["Kn"], r["q"], r["pidx"], h) == canon_points(M, K, q, pidx, h)
     # same basis and pruning as the oracle: identical node counts
     assert int(r["stats"]["nodes"]) == cnt[0]
+
+
+@pytest.mark.parametrize("h,h21,seed,E,dil", [(6, 90, 2, 12, 4), (8, 120, 1, 6, 3),
+                                              (10, 150, 3, 12, 4)])
+def test_exact_host_path_matches_pipeline(h, h21, seed, E, dil):
+    """The exact fallback for p-vectors the device hands back (Python-int
+    lattice, cut-aware basis, host search) gives the pipeline's points on
+    p-vectors where both apply; it shares no lattice code with it."""
+    from pfvs_amd import coni
+    inst = make_G(h, h21, seed)
+    lo = pf_floor(inst.Q, dil) if h > 6 else 1
+    ps = random_pvecs(inst, 40, E, seed=3000 + seed, pf_min=lo, pf_max=lo)     # nonempty trees
+    pf = np.hstack([np.zeros((len(ps), 1), np.int64), ps])
+    kap, Mb = inst.data.kappa_cob, inst.data.M_lattice()
+    qmax = cpu.qmax_of(inst.Q, dil)
+    geo = dict(kappa=kap, Mbasis=Mb, Q=inst.Q, dilation=dil, M0min=13)
+    r = gpu.run([geo], pf, np.zeros(len(pf), np.int32), mode="host", filter=False)
+    assert np.all(r["pstat"] == 0)
+    got, n_pts = [], 0
+    for ip in range(len(pf)):
+        Mv, Kv, qv = coni._exact_host_path(kap, Mb, pf[ip], inst.Q, qmax, 13, 10**9)
+        Mv = np.asarray(Mv.T, dtype=np.int64).reshape(-1, h)
+        Kv = np.asarray(Kv.T, dtype=np.int64).reshape(-1, h)
+        got += canon_points(Mv, Kv, qv, np.full(len(qv), ip), h)
+        n_pts += len(qv)
+    assert n_pts > 0
+    assert sorted(got) == canon_points(r["M"], r["Kn"], r["q"], r["pidx"], h)
