@@ -140,12 +140,12 @@ def serving_bench(gpu, args):
     checked against the same request run alone."""
     from pfvs_amd import coni
     from pfvs_amd.instances import geoms_for_qmax, serving_request
-    from pfvs_amd.serve import Server
+    from pfvs_amd.serve import Server, points_checksum
     reqs = [serving_request(r) for r in range(args.requests)]
     data = [(inst.data.kappa_cob, inst.data.M_lattice(), inst.Q, ps, qmax)
             for inst, ps, qmax in reqs]
     gaps = np.random.default_rng(11).exponential(1.0 / args.rate, len(reqs))
-    srv = Server().start()
+    srv = Server(keep_points="checksum").start()
     srv.submit(*data[0]).wait()                  # warm: code and buffers
     srv.jobs.clear()
     srv.iterations.clear()
@@ -160,13 +160,15 @@ def serving_bench(gpu, args):
         j.wait(timeout=600)
     srv.stop(timeout=60)
     st = srv.stats()
-    exact = 0
+    exact = exact_pts = n_pts = 0
     for (inst, ps, qmax), (kap, Mb, Q, _, _), j in zip(reqs, data, jobs):
         geoms, pgeo = geoms_for_qmax(inst, qmax)
         pf = np.hstack([np.zeros((len(ps), 1), np.int64), ps])
         r = gpu.run(geoms, pf, pgeo, mode="batched", filter=True, sort=True)
         Ks, Ms = coni.assemble(r, ps, pf, kap, Mb, Q, qmax, 13)
         exact += int(np.array_equal(Ks, j.result[0]) and np.array_equal(Ms, j.result[1]))
+        exact_pts += int(points_checksum(r, inst.h11) == j.points_sum)
+        n_pts += len(r["q"])
     hs = [inst.h11 for inst, _, _ in reqs]
     return {
         "metric": "coniZpM serving: aggregate FP nodes/s under a Poisson stream",
@@ -178,7 +180,9 @@ def serving_bench(gpu, args):
                                f"p_f <= 3, qmax = 338 p_f + b (b <= band(h)); Poisson "
                                f"{args.rate}/s", "parallelism": "none"},
         "detail": dict(st, p_vectors=int(sum(len(ps) for _, ps, _ in reqs)),
-                       bit_exact_alone_vs_mix=f"{exact}/{len(reqs)}"),
+                       bit_exact_alone_vs_mix=f"{exact}/{len(reqs)}",
+                       points_checksum_alone_vs_mix=f"{exact_pts}/{len(reqs)}",
+                       lattice_points=n_pts),
     }
 
 

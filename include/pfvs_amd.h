@@ -108,7 +108,12 @@ enum {
     PFA_ST_STEPS, PFA_ST_WAIT_CYC, PFA_ST_WAVE_CYC, PFA_ST_DONATED, PFA_ST_FETCHES,
     /* host mode only: terms of the algorithmic work (lane-ops) of the search */
     PFA_ST_SUM_K, PFA_ST_SUM_K_M0, PFA_ST_SUM_K_R, PFA_ST_SUM_R, PFA_ST_SUM_S, PFA_ST_SUM_LEAF,
-    PFA_ST_WORK, PFA_ST_N
+    PFA_ST_WORK,
+    PFA_ST_MS_FINISH,     /* host ms of the final assembly: statuses, canonical sort, copy-out */
+    PFA_ST_MS_ALLOC,      /* device: host ms spent growing device and pinned buffers */
+    PFA_ST_MS_D2H,        /* device: host ms of copying the results back */
+    PFA_ST_ATTEMPTS,      /* device: searches run (more than 1 after a point-buffer overflow) */
+    PFA_ST_N
 };
 
 typedef struct {

@@ -74,6 +74,30 @@ def _in_kahler_cone(H, ps):
     return bool(np.all(util.exact_matmul(H, np.asarray(ps).T) >= 1))
 
 
+def _lex_increasing(key, Ms, Kns):
+    """True when the columns (key, M, Kn) are strictly increasing in
+    lexicographic order -- the canonical order the library's sorted runs
+    already return -- so the sort can be skipped.  Each block (key, M, Kn)
+    is compared row against row in its (points, h) layout: the first
+    differing entry of every still-undecided adjacent pair decides it."""
+    n = len(key)
+    if n < 2:
+        return True
+    undecided = np.ones(n - 1, dtype=bool)
+    rows = np.arange(n - 1)
+    for X in (np.asarray(key)[:, None], np.asarray(Ms).T, np.asarray(Kns).T):
+        a, b = X[:-1], X[1:]
+        diff = a != b
+        first = diff.argmax(axis=1)
+        r = np.flatnonzero(undecided & diff[rows, first])
+        if np.any(a[r, first[r]] > b[r, first[r]]):
+            return False
+        undecided[r] = False
+        if not undecided.any():
+            return True
+    return False                                  # equal neighbours: let the sort decide
+
+
 def _pfvs_from_points(Ms, Kns, Qs, key, kappa, h11, Q, M0min, nonsing=None, verbosity=0):
     """Lattice points -> coni PFVs (pfvs/coni.py:275-412): keep q < Q K_gcd,
     canonical (p, M, Kn) order, expand each point by g = 1..(Q K_gcd - 1)//q
@@ -99,11 +123,15 @@ def _pfvs_from_points(Ms, Kns, Qs, key, kappa, h11, Q, M0min, nonsing=None, verb
         np.array([math.gcd(*[int(x) for x in col]) for col in Kperps.T], dtype=object)
     K_gcds[K_gcds < 1] = 1
     mask = Qs < Q * K_gcds
-    Ms, Kns, Qs, K_gcds, key, ns = Ms[:, mask], Kns[:, mask], Qs[mask], K_gcds[mask], key[mask], ns[mask]
+    if not mask.all():
+        Ms, Kns, Qs, K_gcds, key, ns = (Ms[:, mask], Kns[:, mask], Qs[mask], K_gcds[mask],
+                                        key[mask], ns[mask])
     if Qs.shape[0] == 0:
         return empty
-    order = np.lexsort(tuple(Kns[::-1]) + tuple(Ms[::-1]) + (key,))
-    Ms, Kns, Qs, K_gcds, key, ns = Ms[:, order], Kns[:, order], Qs[order], K_gcds[order], key[order], ns[order]
+    if not _lex_increasing(key, Ms, Kns):
+        order = np.lexsort(tuple(Kns[::-1]) + tuple(Ms[::-1]) + (key,))
+        Ms, Kns, Qs, K_gcds, key, ns = (Ms[:, order], Kns[:, order], Qs[order], K_gcds[order],
+                                        key[order], ns[order])
     gcap = (Q * K_gcds - 1) // np.maximum(Qs, 1)
     gmax = int(np.max(gcap))
     Qperp_b = gmax * (Qb + Mb * Kb)

@@ -245,14 +245,17 @@ PFA_HD static inline void prep_one(const PfaTeam &T, char *ws_hot, char *ws_cold
     PFA_SYNC();
 }
 
+// order (optional): the p-vectors this launch prepares, by index (a size
+// class of a mixed batch, its LDS sized for the class's largest h11).
 __global__ __launch_bounds__(64) void k_prep(GeoArgs G, const int64_t *ps, int MH,
-                                             const int32_t *pgeo, int64_t n, int64_t *recs, RecL L,
-                                             char *cold, size_t cold_bytes)
+                                             const int32_t *pgeo, int64_t n, const int32_t *order,
+                                             int64_t *recs, RecL L, char *cold, size_t cold_bytes)
 {
     extern __shared__ __align__(16) char smem[];
     PfaTeam T{(int)threadIdx.x, 64};
     char *mine = cold + (size_t)blockIdx.x * cold_bytes;
-    for (int64_t ip = blockIdx.x; ip < n; ip += gridDim.x) {
+    for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const int64_t ip = order ? (int64_t)order[i] : i;
         prep_one(T, smem, mine, G, ps, MH, pgeo, ip, recs + ip * L.words, L);
         __syncthreads();
     }
@@ -1362,9 +1365,11 @@ struct DevCtx {
     std::mutex mu;
     hipStream_t stream = nullptr;
     std::vector<DevBuf> bufs;
+    std::vector<DevBuf> hbufs;             // pinned host staging of results
     int cus = 0;
     size_t lds_max = 0;
     unsigned long long pcap_hint = 0;      // largest point count seen
+    double ms_alloc = 0;                   // this run's buffer growth (host ms)
 };
 
 static std::mutex g_ctx_mu;
@@ -1395,12 +1400,35 @@ static void *dev_get(DevCtx *C, int slot, size_t bytes, char *errbuf)
     if ((int)C->bufs.size() <= slot) C->bufs.resize(slot + 1);
     DevBuf &b = C->bufs[slot];
     if (b.n < bytes) {
+        auto t0 = std::chrono::steady_clock::now();
         if (b.p) (void)hipFree(b.p);
         b.p = nullptr;
         b.n = 0;
         size_t want = bytes + bytes / 4 + 4096;
         HIPCHK(hipMalloc(&b.p, want));
         b.n = want;
+        C->ms_alloc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return b.p;
+}
+
+// Grow-only pinned host buffers, per device, for the results' copy back
+// (no zero fill, no page faults after the first use, full-rate DMA).
+enum { H_PIDX, H_Q, H_M, H_KN, H_NEXP, H_NS, H_NBUF };
+
+static void *host_get(DevCtx *C, int slot, size_t bytes, char *errbuf)
+{
+    if ((int)C->hbufs.size() <= slot) C->hbufs.resize(slot + 1);
+    DevBuf &b = C->hbufs[slot];
+    if (b.n < bytes) {
+        auto t0 = std::chrono::steady_clock::now();
+        if (b.p) (void)hipHostFree(b.p);
+        b.p = nullptr;
+        b.n = 0;
+        size_t want = bytes + bytes / 4 + 4096;
+        HIPCHK(hipHostMalloc(&b.p, want, hipHostMallocDefault));
+        b.n = want;
+        C->ms_alloc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     return b.p;
 }
@@ -1411,7 +1439,7 @@ enum {
     B_GEO_H, B_GEO_Q, B_GEO_QMAX, B_GEO_LIN, B_KAPPA, B_MBASIS, B_KOFS, B_MOFS, B_PS, B_PGEO,
     B_RECS, B_RING, B_SEQ, B_STACKS, B_CTL, B_PT_PIDX, B_PT_Q, B_PT_C, B_CNT, B_M, B_KN, B_NEXP, B_BAD,
     B_PRIMES, B_KT, B_KTOFS, B_KTMAX, B_MODCNT, B_MODLIST, B_MODNEXP, B_MODM, B_NONSING, B_PREPCOLD,
-    B_TOT, B_KEEPCNT, B_KEEPKP, B_KEEPNEXP, B_KEEPMETA, B_NBUF
+    B_TOT, B_KEEPCNT, B_KEEPKP, B_KEEPNEXP, B_KEEPMETA, B_PREPORDER, B_NBUF
 };
 
 // the k largest primes below 2^31 with their Barrett constants
@@ -1446,18 +1474,96 @@ static int batch_dmax(const pfa_run_input *in)
     return D;
 }
 
-// canonical order: by p-vector, then (M, Kn) lexicographically
-static void canonical_order(std::vector<int64_t> &idx, const int64_t *pidx, const int64_t *M,
-                            const int64_t *Kn, int MH)
+// Size classes of the preparation: p-vectors grouped by the h11 of their
+// geometry into classes of largest h11 at most 12, 20, 28 and the maximum;
+// within a class by descending h11 (long problems first).  One class
+// without an index list when every p-vector has the same h11.
+struct PrepClass {
+    int h;                        // largest h11 in the class
+    int64_t n;
+    std::vector<int32_t> idx;
+};
+
+static std::vector<PrepClass> prep_classes(const pfa_run_input *in)
 {
-    std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) {
-        if (pidx[a] != pidx[b]) return pidx[a] < pidx[b];
+    int hlo = PFA_MAX_H11, hhi = 0;
+    for (int g = 0; g < in->n_geo; ++g) {
+        hlo = std::min<int>(hlo, in->h[g]);
+        hhi = std::max<int>(hhi, in->h[g]);
+    }
+    if (hlo == hhi || in->n_p < 2) return {PrepClass{hhi, in->n_p, {}}};
+    static const int edges[] = {12, 20, 28, PFA_MAX_H11};
+    std::vector<std::vector<int32_t>> by_h(PFA_MAX_H11 + 1);
+    for (int64_t i = 0; i < in->n_p; ++i) by_h[in->h[in->pgeo[i]]].push_back((int32_t)i);
+    std::vector<PrepClass> out;
+    int lo = 0;
+    for (int e : edges) {
+        PrepClass c{0, 0, {}};
+        for (int h = e; h > lo; --h)
+            if (!by_h[h].empty()) {
+                if (!c.h) c.h = h;
+                c.idx.insert(c.idx.end(), by_h[h].begin(), by_h[h].end());
+            }
+        c.n = (int64_t)c.idx.size();
+        if (c.n) out.push_back(std::move(c));
+        lo = e;
+    }
+    std::reverse(out.begin(), out.end());    // largest class first
+    return out;
+}
+
+// Host threads for the final assembly: the caller's count, else the
+// hardware's, at most 16.
+static int finish_threads(const pfa_run_input *in)
+{
+    int n = in->host_threads > 0 ? in->host_threads : (int)std::thread::hardware_concurrency();
+    return std::max(1, std::min(n, 16));
+}
+
+// fn(lo, hi) over [0, n) in chunks of `chunk`, on up to nth threads.
+template <class F>
+static void parallel_chunks(int64_t n, int64_t chunk, int nth, F fn)
+{
+    const int64_t nchunks = (n + chunk - 1) / chunk;
+    nth = (int)std::max<int64_t>(1, std::min<int64_t>(nth, nchunks));
+    if (nth == 1) {
+        if (n > 0) fn((int64_t)0, n);
+        return;
+    }
+    std::atomic<int64_t> next{0};
+    auto work = [&]() {
+        for (int64_t c; (c = next.fetch_add(1)) < nchunks;)
+            fn(c * chunk, std::min(n, (c + 1) * chunk));
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nth; ++t) th.emplace_back(work);
+    work();
+    for (auto &t : th) t.join();
+}
+
+// canonical order: by p-vector, then (M, Kn) lexicographically.  A stable
+// counting sort by p-vector, then every p-vector's points sorted on their
+// own (small, cache-resident groups) on nth threads.
+static void canonical_order(std::vector<int64_t> &idx, const int64_t *pidx, const int64_t *M,
+                            const int64_t *Kn, int MH, int64_t n_p, int nth)
+{
+    std::vector<int64_t> start(n_p + 1, 0);
+    for (int64_t k : idx) start[pidx[k] + 1]++;
+    for (int64_t i = 0; i < n_p; ++i) start[i + 1] += start[i];
+    std::vector<int64_t> by_p(idx.size()), pos(start.begin(), start.end() - 1);
+    for (int64_t k : idx) by_p[pos[pidx[k]]++] = k;
+    auto less = [&](int64_t a, int64_t b) {
         const int64_t *ma = M + a * MH, *mb = M + b * MH;
         for (int i = 0; i < MH; ++i) if (ma[i] != mb[i]) return ma[i] < mb[i];
         const int64_t *ka = Kn + a * MH, *kb = Kn + b * MH;
         for (int i = 0; i < MH; ++i) if (ka[i] != kb[i]) return ka[i] < kb[i];
         return false;
+    };
+    parallel_chunks(n_p, 64, nth, [&](int64_t lo, int64_t hi) {
+        for (int64_t ip = lo; ip < hi; ++ip)
+            std::sort(by_p.begin() + start[ip], by_p.begin() + start[ip + 1], less);
     });
+    idx.swap(by_p);
 }
 
 static void fill_geo_offsets(const pfa_run_input *in, std::vector<int64_t> &kofs,
@@ -1606,6 +1712,8 @@ static int finish_run(const pfa_run_input *in, pfa_run_output *out, int64_t npts
 {
     const int MH = PFA_MAX_H11;
     const int64_t n_p = in->n_p;
+    const int nth = finish_threads(in);
+    auto t0 = std::chrono::steady_clock::now();
     out->pstat = (int8_t *)malloc(std::max<int64_t>(n_p, 1));
     out->pcode = (int32_t *)malloc(sizeof(int32_t) * std::max<int64_t>(n_p, 1));
     for (int64_t i = 0; i < n_p; ++i) {
@@ -1622,7 +1730,7 @@ static int finish_run(const pfa_run_input *in, pfa_run_output *out, int64_t npts
     for (int64_t k = 0; k < npts; ++k) pidx64[k] = pidx32[k];
     int64_t n_valid = 0;
     for (int64_t k = 0; k < npts; ++k) n_valid += pidx32[k] >= 0;
-    if (in->sort) canonical_order(idx, pidx64.data(), M, Kn, HS);
+    if (in->sort) canonical_order(idx, pidx64.data(), M, Kn, HS, n_p, nth);
     const int64_t n = (int64_t)idx.size();
     out->n = n;
     size_t nb = std::max<int64_t>(n, 1);
@@ -1632,23 +1740,29 @@ static int finish_run(const pfa_run_input *in, pfa_run_output *out, int64_t npts
     out->pidx = (int64_t *)malloc(sizeof(int64_t) * nb);
     out->nexp = (int64_t *)malloc(sizeof(int64_t) * nb);
     out->nonsing = (int8_t *)malloc(nb);
+    parallel_chunks(n, 1 << 14, nth, [&](int64_t lo, int64_t hi) {
+        for (int64_t t = lo; t < hi; ++t) {
+            int64_t k = idx[t];
+            for (int i = 0; i < MH; ++i) {
+                out->M[t * MH + i] = i < HS ? M[k * HS + i] : 0;
+                out->Kn[t * MH + i] = i < HS ? Kn[k * HS + i] : 0;
+            }
+            out->q[t] = q[k];
+            out->pidx[t] = pidx64[k];
+            out->nexp[t] = nexp ? nexp[k] : -1;
+            out->nonsing[t] = nonsing ? nonsing[k] : 2;
+        }
+    });
     int64_t kept = 0, nex = 0, nns = 0, und = 0, nhost = 0;
-    for (int64_t t = 0; t < n; ++t) {
-        int64_t k = idx[t];
-        for (int i = 0; i < MH; ++i) {
-            out->M[t * MH + i] = i < HS ? M[k * HS + i] : 0;
-            out->Kn[t * MH + i] = i < HS ? Kn[k * HS + i] : 0;
+    if (nexp)
+        for (int64_t t = 0; t < n; ++t) {
+            int64_t k = idx[t];
+            if (nexp[k] > 0) {
+                kept++;
+                if (nonsing && nonsing[k] == 1) { nex += nexp[k]; nns++; }
+                if (nonsing && nonsing[k] == 2) und++;
+            }
         }
-        out->q[t] = q[k];
-        out->pidx[t] = pidx64[k];
-        out->nexp[t] = nexp ? nexp[k] : -1;
-        out->nonsing[t] = nonsing ? nonsing[k] : 2;
-        if (nexp && nexp[k] > 0) {
-            kept++;
-            if (nonsing && nonsing[k] == 1) { nex += nexp[k]; nns++; }
-            if (nonsing && nonsing[k] == 2) und++;
-        }
-    }
     for (int64_t i = 0; i < n_p; ++i) nhost += out->pstat[i] == 1;
     out->stats[PFA_ST_POINTS] = (double)n_valid;
     out->stats[PFA_ST_KEPT] = (double)kept;
@@ -1656,6 +1770,8 @@ static int finish_run(const pfa_run_input *in, pfa_run_output *out, int64_t npts
     out->stats[PFA_ST_NONSING] = (double)nns;
     out->stats[PFA_ST_UNDECIDED] = (double)und;
     out->stats[PFA_ST_HOST_P] = (double)nhost;
+    out->stats[PFA_ST_MS_FINISH] =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     (void)errbuf;
     return 0;
 }
@@ -1667,6 +1783,7 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     DevCtx *C = ctx_for(in->device);
     std::lock_guard<std::mutex> lk(C->mu);
     ctx_init(C, in->device, errbuf);
+    C->ms_alloc = 0;
     hipStream_t s = C->stream;
     const int MH = PFA_MAX_H11, D = batch_dmax(in);
     const int64_t n_p = in->n_p;
@@ -1705,15 +1822,34 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     HIPCHK(hipEventRecord(ev[0], s));
     if (n_p > 0) {
         // one workgroup per p-vector up to 16384 (each with its own cold
-        // workspace in global memory), looping beyond
-        const int grid = (int)std::min<int64_t>(n_p, 16384);
-        const size_t cold = pfa_ws_cold_bytes(hmax);
-        char *d_cold = (char *)dev_get(C, B_PREPCOLD, cold * grid, errbuf);
+        // workspace in global memory), looping beyond.  A batch of mixed
+        // h11 runs one launch per size class, each with the LDS of its
+        // largest h11 (a uniform h11 = 40 workspace would cut the
+        // occupancy of small-h11 p-vectors threefold), largest first.
+        std::vector<PrepClass> cls = prep_classes(in);
+        int32_t *d_order = nullptr;
+        size_t cold_max = 0;
+        for (const PrepClass &c : cls)
+            cold_max = std::max(cold_max, pfa_ws_cold_bytes(c.h) * (size_t)std::min<int64_t>(c.n, 16384));
+        char *d_cold = (char *)dev_get(C, B_PREPCOLD, cold_max, errbuf);
+        if (cls.size() > 1) {
+            std::vector<int32_t> ord;
+            ord.reserve(n_p);
+            for (const PrepClass &c : cls) ord.insert(ord.end(), c.idx.begin(), c.idx.end());
+            d_order = (int32_t *)up(B_PREPORDER, ord.data(), 4 * ord.size());
+        }
         HIPCHK(hipFuncSetAttribute((const void *)k_prep, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds));
-        hipLaunchKernelGGL(k_prep, dim3(grid), dim3(64), lds, s, G, d_ps, MH, d_pgeo, n_p, d_recs, L,
-                           d_cold, cold);
-        HIPCHK(hipGetLastError());
+        int64_t off = 0;
+        for (const PrepClass &c : cls) {
+            const int grid = (int)std::min<int64_t>(c.n, 16384);
+            const size_t cold = pfa_ws_cold_bytes(c.h);
+            hipLaunchKernelGGL(k_prep, dim3(grid), dim3(64), pfa_ws_lds_bytes(c.h), s, G, d_ps, MH,
+                               d_pgeo, c.n, d_order ? d_order + off : nullptr, d_recs, L, d_cold,
+                               cold);
+            HIPCHK(hipGetLastError());
+            off += c.n;
+        }
     }
     HIPCHK(hipEventRecord(ev[1], s));
 
@@ -1745,7 +1881,9 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     int32_t *d_c = nullptr;
     int *d_ovf = nullptr;
     std::vector<int64_t> hdr0;
+    int attempts = 0;
     for (int attempt = 0;; ++attempt) {
+        attempts = attempt + 1;
         d_pidx = (int32_t *)dev_get(C, B_PT_PIDX, 4 * pcap, errbuf);
         d_q = (int64_t *)dev_get(C, B_PT_Q, 8 * pcap, errbuf);
         d_c = (int32_t *)dev_get(C, B_PT_C, 4 * pcap * D, errbuf);
@@ -1869,22 +2007,30 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
         filter_run(C, s, in, kofs, Fa, errbuf);
     }
     HIPCHK(hipEventRecord(ev[6], s));
-    // results back
+    // results back, into the pinned staging buffers
+    HIPCHK(hipEventSynchronize(ev[6]));
+    auto t_d2h = std::chrono::steady_clock::now();
     const int64_t nh = keep ? n : 0;          // points copied back
-    std::vector<int32_t> h_pidx(nh), h_status(n_p), h_code(n_p);
-    std::vector<int64_t> h_q(nh), h_M(nh * HS), h_Kn(nh * HS), h_nexp(in->filter ? nh : 0);
-    std::vector<int8_t> h_ns(in->filter ? nh : 0), h_bad(n_p);
+    const size_t nb1 = (size_t)std::max<int64_t>(nh, 1);
+    int32_t *h_pidx = (int32_t *)host_get(C, H_PIDX, 4 * nb1, errbuf);
+    int64_t *h_q = (int64_t *)host_get(C, H_Q, 8 * nb1, errbuf);
+    int64_t *h_M = (int64_t *)host_get(C, H_M, 8 * nb1 * HS, errbuf);
+    int64_t *h_Kn = (int64_t *)host_get(C, H_KN, 8 * nb1 * HS, errbuf);
+    int64_t *h_nexp = (int64_t *)host_get(C, H_NEXP, 8 * nb1, errbuf);
+    int8_t *h_ns = (int8_t *)host_get(C, H_NS, nb1, errbuf);
+    std::vector<int32_t> h_status(n_p), h_code(n_p);
+    std::vector<int8_t> h_bad(n_p);
     std::vector<int64_t> hdr((size_t)n_p * PFA_HDR);
     unsigned long long htot[8] = {0};
     HIPCHK(hipMemcpyAsync(htot, d_tot, 64, hipMemcpyDeviceToHost, s));
     if (nh > 0) {
-        HIPCHK(hipMemcpyAsync(h_pidx.data(), d_pidx, 4 * n, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(h_q.data(), d_q, 8 * n, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(h_M.data(), d_M, 8 * n * HS, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(h_Kn.data(), d_Kn, 8 * n * HS, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h_pidx, d_pidx, 4 * n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h_q, d_q, 8 * n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h_M, d_M, 8 * n * HS, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h_Kn, d_Kn, 8 * n * HS, hipMemcpyDeviceToHost, s));
         if (in->filter) {
-            HIPCHK(hipMemcpyAsync(h_nexp.data(), d_nexp, 8 * n, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipMemcpyAsync(h_ns.data(), d_ns, n, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(h_nexp, d_nexp, 8 * n, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(h_ns, d_ns, n, hipMemcpyDeviceToHost, s));
         }
     }
     if (n_p > 0) {
@@ -1913,6 +2059,10 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     out->stats[PFA_ST_MS_TOTAL] = out->stats[PFA_ST_MS_PREP] + ms_roots + ms_search +
                                   out->stats[PFA_ST_MS_POST] + attempts_wasted_ms;
     out->stats[PFA_ST_QUEUE_PEAK] = (double)queue_peak;
+    out->stats[PFA_ST_MS_D2H] =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_d2h).count();
+    out->stats[PFA_ST_ATTEMPTS] = (double)attempts;
+    out->stats[PFA_ST_MS_ALLOC] = C->ms_alloc;
     out->stats[PFA_ST_STEPS] = (double)hc[3];
     out->stats[PFA_ST_WAIT_CYC] = (double)hc[4];
     out->stats[PFA_ST_WAVE_CYC] = (double)hc[5];
@@ -1923,8 +2073,8 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     // nonsingular, undecided)
     double st_keep[5] = {(double)htot[1], (double)htot[2], (double)htot[3], (double)htot[4],
                          (double)htot[0]};
-    int rc = finish_run(in, out, nh, HS, h_pidx.data(), h_q.data(), h_M.data(), h_Kn.data(),
-                        in->filter ? h_nexp.data() : nullptr, in->filter ? h_ns.data() : nullptr,
+    int rc = finish_run(in, out, nh, HS, h_pidx, h_q, h_M, h_Kn,
+                        in->filter ? h_nexp : nullptr, in->filter ? h_ns : nullptr,
                         h_status.data(), h_code.data(), h_bad.data(), errbuf);
     if (!in->keep_points) {
         out->stats[PFA_ST_POINTS] = st_keep[4];
@@ -2176,6 +2326,11 @@ extern "C" void pfg_release(int device)
     if (hipSetDevice(device) != hipSuccess) return;
     for (auto &b : C->bufs) {
         if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.n = 0;
+    }
+    for (auto &b : C->hbufs) {
+        if (b.p) (void)hipHostFree(b.p);
         b.p = nullptr;
         b.n = 0;
     }

@@ -156,6 +156,106 @@ PFA_HD static inline int pfa_unimodular(const PfaTeam &T, const int64_t *v, int 
     return pfa_team_any(T, flag, bad) ? -1 : 0;
 }
 
+// nearest integer to a / b (b != 0), ties up; |a| < 2^126
+PFA_HD static inline pfa_i128 pfa_round_div128(pfa_i128 a, pfa_i128 b)
+{
+    if (b < 0) { a = -a; b = -b; }
+    return pfa_fdiv128(2 * a + b, 2 * b);
+}
+
+PFA_HD static inline int pfa_mag_below(pfa_i128 a, int bits)
+{
+    const pfa_i128 lim = (pfa_i128)1 << bits;
+    return a < lim && a > -lim;
+}
+
+// r -= q * u over n entries with |r| kept below 2^125 (0 on overflow)
+PFA_HD static inline int pfa_axpy128(pfa_i128 *r, pfa_i128 q, const int64_t *u, int n)
+{
+    if (q == 0) return 1;
+    const double qd = q < 0 ? -(double)q : (double)q;
+    for (int c = 0; c < n; ++c) {
+        if (u[c] == 0) continue;
+        const double ud = u[c] < 0 ? -(double)u[c] : (double)u[c];
+        if (qd * ud >= 0x1p124) return 0;
+        r[c] -= q * (pfa_i128)u[c];
+        if (!pfa_mag_below(r[c], 125)) return 0;
+    }
+    return 1;
+}
+
+// The same U as pfa_unimodular (U v = (g, 0, ..., 0), |det U| = 1) for
+// vectors whose plain Euclid chain overflows int64: there the Bezout row
+// (entries up to max|v|) times v[k] becomes kernel row k.  Here the chain
+// starts at the first nonzero entry i0 of v, and every new kernel row and
+// the Bezout row are formed in int128, then reduced against the earlier
+// kernel rows in reverse order of processing (row m is nonzero only at the
+// indices processed before it and at its own index, where its entry is
+// nonzero), which keeps their entries near max|v|.  One lane computes (a
+// rare path); x, r (int128, n) and ord (int, n) are scratch.  The chain
+// meets the same gcd steps as pfa_unimodular's, so g is the same.  Returns
+// 0, or -1 when an entry would still leave its range.
+PFA_HD static inline int pfa_unimodular_reduced(const PfaTeam &T, const int64_t *v, int n,
+                                                int64_t *U, pfa_i128 *x, pfa_i128 *r, int *ord,
+                                                int *flag)
+{
+    pfa_team_clear(T, flag);
+    int bad = 0;
+    if (T.lane == 0) {
+        int i0 = -1;
+        for (int i = 0; i < n && i0 < 0; ++i)
+            if (v[i] != 0) i0 = i;
+        int no = 0;
+        if (i0 < 0) {                           // v = 0: the identity, as the plain chain
+            for (int e = 0; e < n * n; ++e) U[e] = (e / n) == (e % n);
+        } else {
+            ord[no++] = i0;
+            for (int i = 0; i < n; ++i)
+                if (i != i0) ord[no++] = i;
+            for (int e = 0; e < n * n; ++e) U[e] = 0;
+            for (int c = 0; c < n; ++c) x[c] = c == i0;
+        }
+        int64_t w = i0 < 0 ? 0 : v[i0];
+        for (int m = 1; m < n && i0 >= 0 && !bad; ++m) {
+            const int k = ord[m];
+            const int64_t vk = v[k];
+            if (vk == 0) {
+                for (int c = 0; c < n; ++c) r[c] = c == k;
+            } else {
+                int64_t s, t, g = pfa_xgcd(w, vk, &s, &t);
+                if (g == 0) { bad = 1; break; }
+                const int64_t m10 = -pfa_fdiv64(vk, g), m11 = pfa_fdiv64(w, g);
+                if (!pfa_mag_below(m10, 62) || !pfa_mag_below(s, 62)) { bad = 1; break; }
+                for (int c = 0; c < n; ++c) {
+                    r[c] = (pfa_i128)m10 * x[c] + (c == k ? (pfa_i128)m11 : 0);
+                    x[c] = (pfa_i128)s * x[c] + (c == k ? (pfa_i128)t : 0);
+                }
+                w = g;
+            }
+            // reduce r against kernel rows m-1 .. 1, then store it as row m
+            for (int j = m - 1; j >= 1 && !bad; --j) {
+                const int64_t *uj = U + (size_t)j * n;
+                if (!pfa_axpy128(r, pfa_round_div128(r[ord[j]], uj[ord[j]]), uj, n)) bad = 1;
+            }
+            for (int c = 0; c < n && !bad; ++c) {
+                if (!pfa_fits64(r[c])) bad = 1;
+                else U[(size_t)m * n + c] = (int64_t)r[c];
+            }
+            // reduce the Bezout row against rows m .. 1; it must stay small
+            // for the next step's products
+            for (int j = m; j >= 1 && !bad; --j) {
+                const int64_t *uj = U + (size_t)j * n;
+                if (!pfa_axpy128(x, pfa_round_div128(x[ord[j]], uj[ord[j]]), uj, n)) bad = 1;
+            }
+            for (int c = 0; c < n && !bad; ++c)
+                if (!pfa_mag_below(x[c], 62)) bad = 1;
+        }
+        if (!bad && i0 >= 0)
+            for (int c = 0; c < n; ++c) U[c] = (int64_t)x[c];
+    }
+    return pfa_team_any(T, flag, bad) ? -1 : 0;
+}
+
 // Gram-Schmidt row kk (mu[kk][0:kk], bb[kk]) from the exact Gram matrix:
 //   mu[kk][j] = (G[kk][j] - sum_{l<j} mu[j][l] mu[kk][l] bb[l]) / bb[j],
 //   bb[kk]    =  G[kk][kk] - sum_{l<kk} mu[kk][l] mu[kk][l] bb[l].
@@ -571,8 +671,11 @@ PFA_HD static inline int pfa_build_lattice(const PfaTeam &T, const PfaWS &W, con
         W.vec[j] = (int64_t)s;
     }
     if (pfa_team_any(T, W.flag, bad)) return -1;
-    // orthogonal lattice: rows 1.. of the unimodular U (W1)
-    if ((rc = pfa_unimodular(T, W.vec, h, W.W1, W.flag))) return rc * 10 - 1;
+    // orthogonal lattice: rows 1.. of the unimodular U (W1); the reduced
+    // chain when the plain one overflows (hv, hw, hpiv are free until the HNF)
+    rc = pfa_unimodular(T, W.vec, h, W.W1, W.flag);
+    if (rc == -1) rc = pfa_unimodular_reduced(T, W.vec, h, W.W1, W.hv, W.hw, W.hpiv, W.flag);
+    if (rc) return rc * 10 - 1;
     int64_t *O = W.W1 + h;                            // (d, h) rows
     PFA_SYNC();
     if ((rc = pfa_lll_euclid(T, W, O, d, h))) return rc * 10 - 2;
@@ -620,7 +723,9 @@ PFA_HD static inline int pfa_build_lattice(const PfaTeam &T, const PfaWS &W, con
     for (int a = T.lane; a < d; a += T.n) l[a] = W.Bi[a];
     PFA_SYNC();
     int64_t *V = W.W1;                                 // (d, d); row 0 = w, rows 1.. = K
-    if ((rc = pfa_unimodular(T, l, d, V, W.flag))) return rc * 10 - 4;
+    rc = pfa_unimodular(T, l, d, V, W.flag);
+    if (rc == -1) rc = pfa_unimodular_reduced(T, l, d, V, W.hv, W.hw, W.hpiv, W.flag);
+    if (rc) return rc * 10 - 4;
     PFA_SYNC();
     if ((rc = pfa_lll_form(T, W, V + d, d - 1, d, W.mat, V))) return rc * 10 - 4;
     int64_t *Tm = W.W2;                                // T[i][a]: column a = K_a, last = w

@@ -31,7 +31,8 @@ STAT_NAMES = ["nodes", "cand", "leaf", "points", "passes", "ms_prep", "ms_roots"
               "ms_post", "ms_total", "search_launches", "host_p", "queue_peak", "ms_wall",
               "kept", "expansions", "nonsingular", "undecided", "steps", "wait_cycles",
               "wave_cycles", "donated", "fetches", "sum_k", "sum_k_m0", "sum_k_r", "sum_r",
-              "sum_s", "sum_leaf", "work"]
+              "sum_s", "sum_leaf", "work", "ms_finish",
+              "ms_alloc", "ms_d2h", "attempts"]
 
 
 class _PfgInput(ctypes.Structure):
@@ -281,6 +282,23 @@ def coni_batch(kappa, Mbasis, ps, Q, dilation, M0min, max_N_out, device=0):
     return M[:, :h], Kn[:, :h], q, pidx, pstat
 
 
+_libc_free = ctypes.CDLL(None).free
+_libc_free.argtypes = [ctypes.c_void_p]
+
+
+class _Malloced:
+    """Owner of a buffer the library allocated with malloc: exposes it as
+    bytes to numpy and frees it when garbage collected."""
+
+    def __init__(self, addr, nbytes):
+        self.addr = addr
+        self.__array_interface__ = {"shape": (nbytes,), "typestr": "|u1",
+                                    "data": (addr, False), "version": 3}
+
+    def __del__(self):
+        _libc_free(self.addr)
+
+
 def run(geoms, ps, pgeo, mode="batched", device=0, waves_per_cu=0, budget=0, stack_cap=0,
         host_threads=0, filter=True, sort=True, keep_points=True, verbose=False):
     """This library's run call.  Returns a dict with M, Kn (n x max_h11()),
@@ -307,7 +325,17 @@ def run(geoms, ps, pgeo, mode="batched", device=0, waves_per_cu=0, budget=0, sta
             if n == 0 or not ptr:
                 return np.zeros(shape, dtype=dt)
             return np.ctypeslib.as_array(ptr, shape=shape).copy()
-        res = dict(M=arr(out.M, (n, MH)), Kn=arr(out.Kn, (n, MH)), q=arr(out.q, (n,)),
+
+        def own(name):
+            """The library's malloc'd (n, MH) buffer as an array, without a
+            copy; freed with the last array over it."""
+            ptr = getattr(out, name)
+            if n == 0 or not ptr:
+                return np.zeros((n, MH), dtype=np.int64)
+            a = np.asarray(_Malloced(ctypes.addressof(ptr.contents), 8 * n * MH))
+            setattr(out, name, None)
+            return a.view(np.int64).reshape(n, MH)
+        res = dict(M=own("M"), Kn=own("Kn"), q=arr(out.q, (n,)),
                    pidx=arr(out.pidx, (n,)), nexp=arr(out.nexp, (n,)),
                    nonsing=arr(out.nonsing, (n,), np.int8))
         res["pstat"] = np.ctypeslib.as_array(out.pstat, shape=(n_p,)).copy() if n_p \

@@ -51,6 +51,8 @@ class Job:
         self.t_start = None
         self.t_done = None
         self.result = None
+        self.points = None
+        self.points_sum = None
         self.error = None
         self.done = threading.Event()
 
@@ -68,12 +70,35 @@ def record_bytes(h):
     return 8 * (20 + 3 * d * d + 2 * d + 2 * (d + 1) * d + 2 * (d + 1) * d)
 
 
+def points_checksum(r, h):
+    """Order-sensitive 64-bit checksum of lattice-point rows (M, Kn, q,
+    pidx, nonsing; the first h columns of M and Kn): per row a weighted sum
+    of its entries, then the rows weighted by position, all mod 2^64."""
+    n = len(r["q"])
+    if n == 0:
+        return 0
+    cols = np.hstack([np.asarray(r["M"])[:, :h], np.asarray(r["Kn"])[:, :h],
+                      np.asarray(r["q"]).reshape(n, 1), np.asarray(r["pidx"]).reshape(n, 1),
+                      np.asarray(r["nonsing"]).reshape(n, 1).astype(np.int64)]).astype(np.uint64)
+    w = np.arange(1, cols.shape[1] + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15) | \
+        np.uint64(1)
+    with np.errstate(over="ignore"):
+        rows = (cols * w).sum(axis=1, dtype=np.uint64)
+        pos = np.arange(n, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+        return int((rows * pos).sum(dtype=np.uint64))
+
+
 class Server:
     def __init__(self, device=0, max_batch_p=16384, hbm_budget=32 << 30, points_per_p=2000,
-                 params=None, runner=None, max_N_pfvs=1_000_000_000, finish_workers=4):
+                 params=None, runner=None, max_N_pfvs=1_000_000_000, finish_workers=4,
+                 keep_points=False):
         """runner(geoms, ps, pgeo) -> pipeline result dict (default: gpu.run,
         batched, filter on, sorted rows).  Completed jobs are assembled on
-        `finish_workers` host threads while the next iteration runs."""
+        `finish_workers` host threads while the next iteration runs.  With
+        keep_points=True a job also keeps its lattice-point rows (job.points:
+        M, Kn, q, pidx into its own p-vectors, nonsing); with "checksum" it
+        keeps only points_checksum of them (job.points_sum), computed after
+        the job's completion time is taken."""
         self.device = device
         self.max_batch_p = int(max_batch_p)
         self.hbm_budget = int(hbm_budget)
@@ -87,9 +112,10 @@ class Server:
         self._ids = 0
         self._thread = None
         self.jobs = []
-        self.iterations = []                  # (t0, t1, n_p, n_jobs, nodes)
+        self.iterations = []                  # (t0, t1, n_p, n_jobs, nodes, stage ms, h11s)
         self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=finish_workers)
         self.finish_s = 0.0
+        self.keep_points = keep_points
 
     # ------------------------------------------------------------ client side
     def submit(self, kappa, Mbasis, Q, ps, qmax, M0min=13):
@@ -179,7 +205,13 @@ class Server:
                     job.done.set()
                 continue
             t1 = time.perf_counter()
-            self.iterations.append((t0, t1, n, len(owner), float(r["stats"]["nodes"])))
+            st = r["stats"]
+            self.iterations.append((t0, t1, n, len(owner), float(st["nodes"]),
+                                    {k: round(float(st.get(k, 0.0)), 1) for k in
+                                     ("ms_prep", "ms_roots", "ms_search", "ms_post", "ms_total",
+                                      "ms_alloc", "ms_d2h", "ms_finish", "ms_wall",
+                                      "attempts")},
+                                    sorted({job.h for job, _, _ in pieces})))
             pidx = np.asarray(r["pidx"])
             self.points_per_p = 0.5 * self.points_per_p + 0.5 * max(len(pidx) / n, 1.0)
             for job, lo, b0, k in owner:
@@ -193,6 +225,7 @@ class Server:
 
     def _finish(self, job):
         t0 = time.perf_counter()
+        rr = None
         try:
             rr = {k: np.concatenate([p[k] for p in job.rows]) for k in
                   ("q", "pidx", "nonsing")}
@@ -206,6 +239,10 @@ class Server:
         job.rows = []
         job.t_done = time.perf_counter()
         self.finish_s += job.t_done - t0
+        if job.error is None and self.keep_points == "checksum":
+            job.points_sum = points_checksum(rr, job.h)
+        elif job.error is None and self.keep_points:
+            job.points = rr
         job.done.set()
 
     # ---------------------------------------------------------------- report
@@ -225,6 +262,7 @@ class Server:
             if self.iterations else 0.0,
             "nodes": nodes, "busy_s": busy, "finish_s": self.finish_s, "span_s": span,
             "nodes_per_s": nodes / span if span > 0 else 0.0,
-            "iteration_ms_p_jobs": [(round((t1 - t0) * 1e3, 1), n_p, n_j)
-                                    for t0, t1, n_p, n_j, _ in self.iterations],
+            "iterations_detail": [dict(ms=round((t1 - t0) * 1e3, 1), p=n_p, jobs=n_j,
+                                       h=list(hs), nodes=nd, **ms)
+                                  for t0, t1, n_p, n_j, nd, ms, hs in self.iterations],
         }

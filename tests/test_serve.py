@@ -60,26 +60,79 @@ def test_server_fails_jobs_of_a_failed_iteration():
     srv.stop(timeout=10)
 
 
-def run_alone(inst, ps, qmax):
+def test_points_checksum_sees_order_and_entries():
+    from pfvs_amd.serve import points_checksum
+    rng = np.random.default_rng(0)
+    r = dict(M=rng.integers(-5, 5, (50, 41)), Kn=rng.integers(-5, 5, (50, 41)),
+             q=rng.integers(0, 9, 50), pidx=np.arange(50),
+             nonsing=rng.integers(0, 3, 50).astype(np.int8))
+    c = points_checksum(r, 20)
+    for key, edit in (("M", lambda a: a.__setitem__([3, 4], a[[4, 3]])),
+                      ("Kn", lambda a: a.__setitem__((7, 5), a[7, 5] + 1)),
+                      ("q", lambda a: a.__setitem__(0, a[0] + 1)),
+                      ("nonsing", lambda a: a.__setitem__(9, (a[9] + 1) % 3))):
+        r2 = {k: v.copy() for k, v in r.items()}
+        edit(r2[key])
+        assert points_checksum(r2, 20) != c, key
+    r2 = {k: v.copy() for k, v in r.items()}
+    r2["M"][:, 30] += 1                                  # beyond h: padding, ignored
+    assert points_checksum(r2, 20) == c
+    assert points_checksum(dict(r, q=r["q"][:0]), 20) == 0
+
+
+def run_alone(kap, Mb, Q, ps, qmax):
     """One job on its own through the same pipeline and post-processing."""
-    from pfvs_amd.instances import geoms_for_qmax
-    geoms, pgeo = geoms_for_qmax(inst, qmax)
+    geoms, pgeo = [], np.zeros(len(ps), np.int32)
+    for v in np.unique(qmax):
+        pgeo[np.flatnonzero(qmax == v)] = len(geoms)
+        geoms.append(dict(kappa=kap, Mbasis=Mb, Q=Q, dilation=int(v) / Q, M0min=13))
     pf = np.hstack([np.zeros((len(ps), 1), np.int64), ps])
     r = gpu.run(geoms, pf, pgeo, mode="batched", filter=True, sort=True)
-    kap, Mb = inst.data.kappa_cob, inst.data.M_lattice()
-    return coni.assemble(r, ps, pf, kap, Mb, inst.Q, qmax, 13)
+    return r, coni.assemble(r, ps, pf, kap, Mb, Q, qmax, 13)
+
+
+def _dataset_jobs(n):
+    """Dataset geometries whose PFVs are known: (args, geometry record)."""
+    from helpers import cydata_of, geometries
+    from pfvs_amd.pvectors import box_enum
+    out = []
+    for g in geometries():
+        if not (3 <= g["h11"] <= 11) or not g["K"]:
+            continue
+        data = cydata_of(g)
+        ps = box_enum(g["B"], data.H_cob, 1, True)
+        if not 0 < len(ps) <= 2000:
+            continue
+        Q = data.h11 + data.h21 + 4
+        qmax = np.full(len(ps), int(np.floor(g["D"] * Q + 1e-7)))
+        out.append(((data.kappa_cob, data.M_lattice(), Q, ps, qmax), g))
+        if len(out) == n:
+            break
+    return out
 
 
 @pytest.mark.gpu
 def test_gpu_serving_results_same_alone_and_in_mix(need_gpu):
+    """Synthetic requests of h11 5..40 interleaved with dataset geometries:
+    every job's lattice points and (Ks, Ms) equal the job run alone, and
+    the dataset jobs reproduce the dataset's PFVs."""
+    from helpers import as_set
     from pfvs_amd.instances import serving_request
-    reqs = [serving_request(r, n_max=96) for r in range(10)]
-    srv = Server(max_batch_p=256).start()
+    reqs = []
+    for r in range(10):
+        inst, ps, qmax = serving_request(r, n_max=96)
+        qmax = np.broadcast_to(np.asarray(qmax, np.int64), (len(ps),)).copy()
+        reqs.append(((inst.data.kappa_cob, inst.data.M_lattice(), inst.Q, ps, qmax), None))
+    ds = _dataset_jobs(4)
+    assert len(ds) == 4
+    for k, job in enumerate(ds):
+        reqs.insert(2 * k + 1, job)
+    srv = Server(max_batch_p=256, keep_points=True).start()
     jobs = []
 
     def client():
-        for inst, ps, qmax in reqs:
-            jobs.append(srv.submit(inst.data.kappa_cob, inst.data.M_lattice(), inst.Q, ps, qmax))
+        for args, _ in reqs:
+            jobs.append(srv.submit(*args))
             time.sleep(0.002)
     th = threading.Thread(target=client)
     th.start()
@@ -90,6 +143,18 @@ def test_gpu_serving_results_same_alone_and_in_mix(need_gpu):
     st = srv.stats()
     print(f"serving: {st}", flush=True)
     assert st["errors"] == 0 and st["jobs"] == len(reqs)
-    for (inst, ps, qmax), j in zip(reqs, jobs):
-        Ks, Ms = run_alone(inst, ps, qmax)
+    n_pfvs = 0
+    for (args, g), j in zip(reqs, jobs):
+        r, (Ks, Ms) = run_alone(*args)
+        h = j.h
         assert np.array_equal(Ks, j.result[0]) and np.array_equal(Ms, j.result[1]), j.id
+        P = j.points
+        assert np.array_equal(P["pidx"], r["pidx"]) and np.array_equal(P["q"], r["q"]), j.id
+        assert np.array_equal(P["M"][:, :h], r["M"][:, :h]), j.id
+        assert np.array_equal(P["Kn"][:, :h], r["Kn"][:, :h]), j.id
+        assert np.array_equal(P["nonsing"], r["nonsing"]), j.id
+        if g is not None:
+            got, exp = as_set(Ks, Ms), as_set(g["K"], g["M"])
+            assert exp <= got, j.id
+            n_pfvs += len(Ks)
+    assert n_pfvs > 0
