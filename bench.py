@@ -205,6 +205,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the launch-parameter autotuner (library defaults)")
+    ap.add_argument("--waves-per-cu", type=int, default=0,
+                    help="fixed search launch setting instead of the autotuner's (with --budget)")
+    ap.add_argument("--budget", type=int, default=0,
+                    help="fixed donation size instead of the autotuner's (with --waves-per-cu)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only time a plain device copy of MIB MiB (the reference "
                          "kernel for HBM traffic counters) and exit")
@@ -228,7 +232,9 @@ def main():
     print(f"# workload {args.workload}: {len(pf)} p-vectors, {len(geoms)} geometry entries",
           file=sys.stderr, flush=True)
     params = {}
-    if not args.no_tune:
+    if args.waves_per_cu > 0 or args.budget > 0:
+        params = {"waves_per_cu": args.waves_per_cu, "budget": args.budget, "stack_cap": 0}
+    elif not args.no_tune:
         from pfvs_amd import autotune
         best = autotune.tune(geoms, pf, pgeo, use_cache=False)
         params = {k: best[k] for k in ("waves_per_cu", "budget", "stack_cap")}
