@@ -145,7 +145,7 @@ def serving_bench(gpu, args):
     data = [(inst.data.kappa_cob, inst.data.M_lattice(), inst.Q, ps, qmax)
             for inst, ps, qmax in reqs]
     gaps = np.random.default_rng(11).exponential(1.0 / args.rate, len(reqs))
-    srv = Server(keep_points="checksum").start()
+    srv = Server(keep_points="checksum", reserve=(16384, 4_000_000, 41)).start()
     srv.submit(*data[0]).wait()                  # warm: code and buffers
     srv.jobs.clear()
     srv.iterations.clear()
@@ -178,7 +178,9 @@ def serving_bench(gpu, args):
         "config": {"workload": f"{len(reqs)} requests G(h, 10h, r), h uniform 5..40 (here "
                                f"{min(hs)}..{max(hs)}), N log-uniform 1..4096, cone coords <= 16, "
                                f"p_f <= 3, qmax = 338 p_f + b (b <= band(h)); Poisson "
-                               f"{args.rate}/s", "parallelism": "none"},
+                               f"{args.rate}/s", "parallelism": "none",
+                   "server": {"max_batch_p": srv.max_batch_p, "pipeline": srv.pipeline,
+                              "reserve": list(srv.reserve)}},
         "detail": dict(st, p_vectors=int(sum(len(ps) for _, ps, _ in reqs)),
                        bit_exact_alone_vs_mix=f"{exact}/{len(reqs)}",
                        points_checksum_alone_vs_mix=f"{exact_pts}/{len(reqs)}",

@@ -177,6 +177,15 @@ int pfa_det_filter(int n_geo, const int32_t *h, const int64_t *kappa, int64_t co
  * counters.  No counterpart in the reference.  Returns 0 or 1. */
 int pfa_copy_probe(int device, int64_t bytes, int reps, double *ms_per_copy);
 
+/* Grows the per-device buffers (device memory and both pinned result
+ * staging sets) to hold a batch of n_p p-vectors and n_points points at
+ * h11 <= h_max, so that later calls of that size allocate nothing; also
+ * sizes the first point buffer of later searches.  For long-running
+ * callers (a serving loop) to pay allocation once, at start.  No
+ * counterpart in the reference.  Returns 0, or 1 with a message in err
+ * (256 bytes). */
+int pfa_reserve(int device, int64_t n_p, int64_t n_points, int h_max, char *err);
+
 /* Library facts: number of compute units and peak engine clock (kHz) of
  * `device`, and the build's architecture string. */
 int pfa_device_cus(int device);

@@ -1361,14 +1361,24 @@ struct DevBuf {
     size_t n = 0;
 };
 
+// Pinned host staging of one call's results.  Two sets per device: a call
+// copies its results into one while holding the device, then releases the
+// device and assembles from the set, so the next call's kernels overlap it.
+struct Stage {
+    std::mutex mu;
+    std::vector<DevBuf> bufs;
+};
+
 struct DevCtx {
     std::mutex mu;
     hipStream_t stream = nullptr;
     std::vector<DevBuf> bufs;
-    std::vector<DevBuf> hbufs;             // pinned host staging of results
+    Stage stage[2];
+    int next_stage = 0;                    // (under mu)
     int cus = 0;
     size_t lds_max = 0;
     unsigned long long pcap_hint = 0;      // largest point count seen
+    double ppp_hint = 0;                   // largest points per p-vector seen (n_p >= 64)
     double ms_alloc = 0;                   // this run's buffer growth (host ms)
 };
 
@@ -1416,10 +1426,10 @@ static void *dev_get(DevCtx *C, int slot, size_t bytes, char *errbuf)
 // (no zero fill, no page faults after the first use, full-rate DMA).
 enum { H_PIDX, H_Q, H_M, H_KN, H_NEXP, H_NS, H_NBUF };
 
-static void *host_get(DevCtx *C, int slot, size_t bytes, char *errbuf)
+static void *host_get(DevCtx *C, Stage &S, int slot, size_t bytes, char *errbuf)
 {
-    if ((int)C->hbufs.size() <= slot) C->hbufs.resize(slot + 1);
-    DevBuf &b = C->hbufs[slot];
+    if ((int)S.bufs.size() <= slot) S.bufs.resize(slot + 1);
+    DevBuf &b = S.bufs[slot];
     if (b.n < bytes) {
         auto t0 = std::chrono::steady_clock::now();
         if (b.p) (void)hipHostFree(b.p);
@@ -1781,7 +1791,7 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
 {
     auto t_wall = std::chrono::steady_clock::now();
     DevCtx *C = ctx_for(in->device);
-    std::lock_guard<std::mutex> lk(C->mu);
+    std::unique_lock<std::mutex> lk(C->mu);
     ctx_init(C, in->device, errbuf);
     C->ms_alloc = 0;
     hipStream_t s = C->stream;
@@ -1856,6 +1866,8 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     // points buffer: grow and rerun the search on overflow
     unsigned long long pcap = std::max<unsigned long long>(
         std::max<int64_t>(1 << 16, 16 * n_p), C->pcap_hint + C->pcap_hint / 8);
+    pcap = std::max<unsigned long long>(
+        pcap, (unsigned long long)std::min(1.25 * C->ppp_hint * (double)n_p, (double)(1 << 26)));
     // defaults measured on MI355X (h11 = 20 batch): fewer, deeper wavefronts
     // keep the stacks' hot tops in L2
     const int waves_per_cu = in->waves_per_cu > 0 ? in->waves_per_cu : 4;
@@ -2012,12 +2024,15 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     auto t_d2h = std::chrono::steady_clock::now();
     const int64_t nh = keep ? n : 0;          // points copied back
     const size_t nb1 = (size_t)std::max<int64_t>(nh, 1);
-    int32_t *h_pidx = (int32_t *)host_get(C, H_PIDX, 4 * nb1, errbuf);
-    int64_t *h_q = (int64_t *)host_get(C, H_Q, 8 * nb1, errbuf);
-    int64_t *h_M = (int64_t *)host_get(C, H_M, 8 * nb1 * HS, errbuf);
-    int64_t *h_Kn = (int64_t *)host_get(C, H_KN, 8 * nb1 * HS, errbuf);
-    int64_t *h_nexp = (int64_t *)host_get(C, H_NEXP, 8 * nb1, errbuf);
-    int8_t *h_ns = (int8_t *)host_get(C, H_NS, nb1, errbuf);
+    Stage &S = C->stage[C->next_stage];
+    C->next_stage ^= 1;
+    std::unique_lock<std::mutex> sl(S.mu);    // waits for the call two back to finish with it
+    int32_t *h_pidx = (int32_t *)host_get(C, S, H_PIDX, 4 * nb1, errbuf);
+    int64_t *h_q = (int64_t *)host_get(C, S, H_Q, 8 * nb1, errbuf);
+    int64_t *h_M = (int64_t *)host_get(C, S, H_M, 8 * nb1 * HS, errbuf);
+    int64_t *h_Kn = (int64_t *)host_get(C, S, H_KN, 8 * nb1 * HS, errbuf);
+    int64_t *h_nexp = (int64_t *)host_get(C, S, H_NEXP, 8 * nb1, errbuf);
+    int8_t *h_ns = (int8_t *)host_get(C, S, H_NS, nb1, errbuf);
     std::vector<int32_t> h_status(n_p), h_code(n_p);
     std::vector<int8_t> h_bad(n_p);
     std::vector<int64_t> hdr((size_t)n_p * PFA_HDR);
@@ -2069,6 +2084,8 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     out->stats[PFA_ST_DONATED] = (double)hc[6];
     out->stats[PFA_ST_FETCHES] = (double)hc[7];
     for (auto &e : ev) (void)hipEventDestroy(e);
+    if (n_p >= 64) C->ppp_hint = std::max(C->ppp_hint, (double)npts / (double)n_p);
+    lk.unlock();                              // the device is free; assemble from the stage
     // counts-only totals from the device (htot: points, kept, expansions,
     // nonsingular, undecided)
     double st_keep[5] = {(double)htot[1], (double)htot[2], (double)htot[3], (double)htot[4],
@@ -2329,10 +2346,65 @@ extern "C" void pfg_release(int device)
         b.p = nullptr;
         b.n = 0;
     }
-    for (auto &b : C->hbufs) {
-        if (b.p) (void)hipHostFree(b.p);
-        b.p = nullptr;
-        b.n = 0;
+    for (Stage &S : C->stage) {
+        std::lock_guard<std::mutex> sl(S.mu);
+        for (auto &b : S.bufs) {
+            if (b.p) (void)hipHostFree(b.p);
+            b.p = nullptr;
+            b.n = 0;
+        }
+    }
+}
+
+extern "C" int pfa_reserve(int device, int64_t n_p, int64_t n_points, int h_max, char *err)
+{
+    char *errbuf = err;
+    try {
+        if (device < 0 || n_p < 0 || n_points < 0 || h_max < 3 || h_max > PFA_MAX_H11) {
+            snprintf(errbuf, 256, "pfa_reserve: bad arguments (n_p %lld, n_points %lld, h_max %d)",
+                     (long long)n_p, (long long)n_points, h_max);
+            return 1;
+        }
+        DevCtx *C = ctx_for(device);
+        std::lock_guard<std::mutex> lk(C->mu);
+        ctx_init(C, device, errbuf);
+        const int D = h_max - 1, HS = D + 1;
+        const RecL L = rec_layout(D);
+        const EntL E = ent_layout(D);
+        const size_t np1 = (size_t)std::max<int64_t>(n_p, 1), n1 = (size_t)std::max<int64_t>(n_points, 1);
+        dev_get(C, B_RECS, 8 * (size_t)L.words * np1, errbuf);
+        dev_get(C, B_PREPCOLD, pfa_ws_cold_bytes(h_max) * std::min<size_t>(np1, 16384), errbuf);
+        // a search sizes its point buffer at 9/8 of the largest count seen
+        const size_t pc = n1 + n1 / 8 + 1;
+        dev_get(C, B_PT_PIDX, 4 * pc, errbuf);
+        dev_get(C, B_PT_Q, 8 * pc, errbuf);
+        dev_get(C, B_PT_C, 4 * pc * D, errbuf);
+        const unsigned long long ring_cap =
+            (unsigned long long)std::max<int64_t>(2 * n_p + 64, 1 << 20) + 2 * 64ULL * 4096ULL;
+        dev_get(C, B_RING, (size_t)ring_cap * E.bytes, errbuf);
+        dev_get(C, B_SEQ, 8 * ring_cap, errbuf);
+        dev_get(C, B_STACKS, (size_t)std::max(1, C->cus * 4) * 64 * (D + 2) * E.bytes, errbuf);
+        dev_get(C, B_M, 8 * n1 * HS, errbuf);
+        dev_get(C, B_KN, 8 * n1 * HS, errbuf);
+        dev_get(C, B_NEXP, 8 * n1, errbuf);
+        dev_get(C, B_NONSING, n1, errbuf);
+        dev_get(C, B_BAD, np1, errbuf);
+        dev_get(C, B_KEEPKP, 8 * n1, errbuf);
+        dev_get(C, B_KEEPNEXP, 8 * n1, errbuf);
+        dev_get(C, B_KEEPMETA, 8 * n1, errbuf);
+        for (Stage &S : C->stage) {
+            std::lock_guard<std::mutex> sl(S.mu);
+            host_get(C, S, H_PIDX, 4 * n1, errbuf);
+            host_get(C, S, H_Q, 8 * n1, errbuf);
+            host_get(C, S, H_M, 8 * n1 * HS, errbuf);
+            host_get(C, S, H_KN, 8 * n1 * HS, errbuf);
+            host_get(C, S, H_NEXP, 8 * n1, errbuf);
+            host_get(C, S, H_NS, n1, errbuf);
+        }
+        C->pcap_hint = std::max<unsigned long long>(C->pcap_hint, (unsigned long long)n_points);
+        return 0;
+    } catch (int) {
+        return 1;
     }
 }
 

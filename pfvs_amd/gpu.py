@@ -135,6 +135,9 @@ def _load():
     lib.pfa_det_filter.argtypes = [ctypes.c_int, _i32p, _i64p, ctypes.c_int64, _i32p, _i64p,
                                    ctypes.c_int, ctypes.c_void_p, ctypes.c_char_p]
     lib.pfa_det_filter.restype = ctypes.c_int
+    lib.pfa_reserve.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
+                                ctypes.c_char_p]
+    lib.pfa_reserve.restype = ctypes.c_int
     _lib, _lib_path = lib, path
     return lib
 
@@ -175,6 +178,16 @@ def release(device: int = 0) -> None:
     lib = _load()
     if lib is not None:
         lib.pfg_release(int(device))
+
+
+def reserve(n_p, n_points, h_max=None, device=0):
+    """Grow the library's buffers on `device` for batches of n_p p-vectors
+    and n_points points at h11 <= h_max (default max_h11())."""
+    lib = require()
+    err = ctypes.create_string_buffer(256)
+    h_max = max_h11() if h_max is None else int(h_max)
+    if lib.pfa_reserve(int(device), int(n_p), int(n_points), h_max, err) != 0:
+        raise RuntimeError(f"reserve failed: {err.value.decode(errors='replace')}")
 
 
 def device_facts(device: int = 0):

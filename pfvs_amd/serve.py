@@ -14,6 +14,11 @@ iterations; every iteration
      (pfa_coni_run with one geometry entry per job and bound),
   3. hands each job the rows of its p-vectors.
 
+Two iterations can be in flight: the next one is admitted while the
+current one runs, waits for the device, and runs its kernels while the
+library assembles the current one's results on the host (the library
+releases the device before that assembly).
+
 A job completes when its last p-vector has come back; its (Ks, Ms) are
 then assembled by coniZpM's own host post-processing (pfvs_amd.coni.
 assemble), so a job's result does not depend on what it was batched with.
@@ -45,7 +50,7 @@ class Job:
         self.qmax = np.broadcast_to(np.asarray(qmax, dtype=np.int64), (len(self.ps),)).copy()
         self.next = 0                     # first p-vector not yet admitted
         self.returned = 0
-        self.rows = []                    # (M, Kn, q, pidx, nonsing) pieces
+        self.rows = []                    # (first p-vector, {M, Kn, q, pidx, nonsing}) pieces
         self.pstat = np.zeros(len(self.ps), np.int8)
         self.t_arrival = time.perf_counter()
         self.t_start = None
@@ -91,14 +96,20 @@ def points_checksum(r, h):
 class Server:
     def __init__(self, device=0, max_batch_p=16384, hbm_budget=32 << 30, points_per_p=2000,
                  params=None, runner=None, max_N_pfvs=1_000_000_000, finish_workers=4,
-                 keep_points=False):
+                 keep_points=False, pipeline=2, reserve=None):
         """runner(geoms, ps, pgeo) -> pipeline result dict (default: gpu.run,
         batched, filter on, sorted rows).  Completed jobs are assembled on
         `finish_workers` host threads while the next iteration runs.  With
         keep_points=True a job also keeps its lattice-point rows (job.points:
         M, Kn, q, pidx into its own p-vectors, nonsing); with "checksum" it
         keeps only points_checksum of them (job.points_sum), computed after
-        the job's completion time is taken."""
+        the job's completion time is taken.
+
+        Up to `pipeline` iterations are in flight: the library runs their
+        device work in order on the device and releases the device before
+        its host-side assembly of results, so that assembly overlaps the
+        next iteration's kernels.  reserve = (n_p, n_points, h_max) grows
+        the library's buffers at start (gpu.reserve)."""
         self.device = device
         self.max_batch_p = int(max_batch_p)
         self.hbm_budget = int(hbm_budget)
@@ -114,6 +125,10 @@ class Server:
         self.jobs = []
         self.iterations = []                  # (t0, t1, n_p, n_jobs, nodes, stage ms, h11s)
         self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=finish_workers)
+        self.pipeline = max(1, int(pipeline))
+        self._slots = threading.Semaphore(self.pipeline)
+        self._runpool = concurrent.futures.ThreadPoolExecutor(max_workers=self.pipeline)
+        self.reserve = reserve
         self.finish_s = 0.0
         self.keep_points = keep_points
 
@@ -133,6 +148,8 @@ class Server:
         return job
 
     def start(self):
+        if self.reserve is not None:
+            gpu.reserve(*self.reserve, device=self.device)
         self._thread = threading.Thread(target=self._loop, daemon=True)
         self._thread.start()
         return self
@@ -143,6 +160,7 @@ class Server:
             self._cv.notify()
         if self._thread:
             self._thread.join(timeout)
+        self._runpool.shutdown(wait=True)
         self._pool.shutdown(wait=True)
 
     # ------------------------------------------------------------- scheduler
@@ -174,63 +192,81 @@ class Server:
         return pieces
 
     def _loop(self):
-        MH = gpu.max_h11()
         while True:
+            self._slots.acquire()
             pieces = self._admit()
             if not pieces:
+                self._slots.release()
                 if self._stop:
                     return
                 continue
-            n = sum(hi - lo for _, lo, hi in pieces)
-            pf = np.zeros((n, MH), np.int64)
-            pgeo = np.zeros(n, np.int32)
-            geoms, owner, base = [], [], 0
-            for job, lo, hi in pieces:
-                k = hi - lo
-                pf[base:base + k, :job.h] = job.p_full[lo:hi]
-                qs = job.qmax[lo:hi]
-                for v in np.unique(qs):               # one geometry entry per bound
-                    pgeo[base + np.flatnonzero(qs == v)] = len(geoms)
-                    geoms.append(dict(kappa=job.kappa, Mbasis=job.Mbasis, Q=job.Q,
-                                      dilation=int(v) / job.Q, M0min=job.M0min))
-                owner.append((job, lo, base, k))
-                base += k
-            t0 = time.perf_counter()
-            try:
-                r = self.runner(geoms, pf, pgeo)
-            except Exception as e:              # the whole iteration failed: fail its jobs
+            self._runpool.submit(self._iterate, pieces)
+
+    def _iterate(self, pieces):
+        try:
+            self._run_iteration(pieces)
+        finally:
+            self._slots.release()
+
+    def _run_iteration(self, pieces):
+        MH = gpu.max_h11()
+        n = sum(hi - lo for _, lo, hi in pieces)
+        pf = np.zeros((n, MH), np.int64)
+        pgeo = np.zeros(n, np.int32)
+        geoms, owner, base = [], [], 0
+        for job, lo, hi in pieces:
+            k = hi - lo
+            pf[base:base + k, :job.h] = job.p_full[lo:hi]
+            qs = job.qmax[lo:hi]
+            for v in np.unique(qs):               # one geometry entry per bound
+                pgeo[base + np.flatnonzero(qs == v)] = len(geoms)
+                geoms.append(dict(kappa=job.kappa, Mbasis=job.Mbasis, Q=job.Q,
+                                  dilation=int(v) / job.Q, M0min=job.M0min))
+            owner.append((job, lo, base, k))
+            base += k
+        t0 = time.perf_counter()
+        try:
+            r = self.runner(geoms, pf, pgeo)
+        except Exception as e:              # the whole iteration failed: fail its jobs
+            with self._cv:
                 for job, lo, b0, k in owner:
-                    job.error = e
-                    job.t_done = time.perf_counter()
-                    job.done.set()
-                continue
-            t1 = time.perf_counter()
-            st = r["stats"]
-            self.iterations.append((t0, t1, n, len(owner), float(st["nodes"]),
-                                    {k: round(float(st.get(k, 0.0)), 1) for k in
-                                     ("ms_prep", "ms_roots", "ms_search", "ms_post", "ms_total",
-                                      "ms_alloc", "ms_d2h", "ms_finish", "ms_wall",
-                                      "attempts")},
-                                    sorted({job.h for job, _, _ in pieces})))
-            pidx = np.asarray(r["pidx"])
-            self.points_per_p = 0.5 * self.points_per_p + 0.5 * max(len(pidx) / n, 1.0)
+                    if job.error is None:
+                        job.error = e
+                        job.t_done = time.perf_counter()
+                        job.done.set()
+            return
+        t1 = time.perf_counter()
+        st = r["stats"]
+        self.iterations.append((t0, t1, n, len(owner), float(st["nodes"]),
+                                {k: round(float(st.get(k, 0.0)), 1) for k in
+                                 ("ms_prep", "ms_roots", "ms_search", "ms_post", "ms_total",
+                                  "ms_alloc", "ms_d2h", "ms_finish", "ms_wall",
+                                  "attempts")},
+                                sorted({job.h for job, _, _ in pieces})))
+        pidx = np.asarray(r["pidx"])
+        self.points_per_p = 0.5 * self.points_per_p + 0.5 * max(len(pidx) / n, 1.0)
+        done = []
+        with self._cv:
             for job, lo, b0, k in owner:
                 a, b = np.searchsorted(pidx, [b0, b0 + k])
-                job.rows.append(dict(M=r["M"][a:b], Kn=r["Kn"][a:b], q=r["q"][a:b],
-                                     pidx=pidx[a:b] - b0 + lo, nonsing=r["nonsing"][a:b]))
+                job.rows.append((lo, dict(M=r["M"][a:b], Kn=r["Kn"][a:b], q=r["q"][a:b],
+                                          pidx=pidx[a:b] - b0 + lo,
+                                          nonsing=r["nonsing"][a:b])))
                 job.pstat[lo:lo + k] = r["pstat"][b0:b0 + k]
                 job.returned += k
-                if job.returned == len(job.ps):
-                    self._pool.submit(self._finish, job)
+                if job.returned == len(job.ps) and job.error is None:
+                    done.append(job)
+        for job in done:
+            self._pool.submit(self._finish, job)
 
     def _finish(self, job):
         t0 = time.perf_counter()
         rr = None
         try:
-            rr = {k: np.concatenate([p[k] for p in job.rows]) for k in
-                  ("q", "pidx", "nonsing")}
+            rows = [p for _, p in sorted(job.rows, key=lambda t: t[0])]
+            rr = {k: np.concatenate([p[k] for p in rows]) for k in ("q", "pidx", "nonsing")}
             for k in ("M", "Kn"):
-                rr[k] = np.vstack([p[k] for p in job.rows])
+                rr[k] = np.vstack([p[k] for p in rows])
             rr["pstat"] = job.pstat
             job.result = coni.assemble(rr, job.ps, job.p_full, job.kappa, job.Mbasis, job.Q,
                                        job.qmax, job.M0min, self.max_N)

@@ -33,7 +33,7 @@ def _job_args(h, n, seed):
 def test_server_splits_and_completes_in_order():
     gpu.require()
     log = []
-    srv = Server(max_batch_p=10, runner=_empty_runner(log))
+    srv = Server(max_batch_p=10, runner=_empty_runner(log), pipeline=1)
     jobs = [srv.submit(*_job_args(6, n, s)) for s, n in enumerate((7, 25, 3, 0, 12))]
     srv.start()
     for j in jobs:
@@ -46,6 +46,44 @@ def test_server_splits_and_completes_in_order():
     assert done == sorted(done)                                 # FIFO completion
     st = srv.stats()
     assert st["jobs"] == 5 and st["errors"] == 0 and st["p99_ms"] >= st["p50_ms"]
+
+
+def test_pipelined_iterations_return_every_row_once():
+    """Two iterations in flight, the later one finishing first: every
+    p-vector runs once, every job completes once, and a job split over
+    iterations gets its rows back in p-vector order."""
+    gpu.require()
+    calls = []
+
+    def run(geoms, ps, pgeo):
+        n = len(ps)
+        calls.append(n)
+        time.sleep(0.05 if len(calls) % 2 else 0.0)       # odd calls finish last
+        MH = gpu.max_h11()
+        M = np.zeros((n, MH), np.int64)
+        M[:, 0] = 13                                       # M0 >= M0min, as every point has
+        M[:, 1] = ps[:, 1]                                 # tag each row with its p-vector
+        return dict(M=M, Kn=M.copy(), q=np.ones(n, np.int64), pidx=np.arange(n),
+                    nonsing=np.zeros(n, np.int8), pstat=np.zeros(n, np.int8),
+                    stats={"nodes": float(n)})
+    srv = Server(max_batch_p=8, runner=run, pipeline=2, keep_points=True)
+    args = []
+    for s_, n in enumerate((13, 5, 21, 2)):
+        kap, Mb, Q, ps, qmax = _job_args(6, n, s_)
+        ps = ps.copy()
+        ps[:, 0] = np.arange(n) + 1                        # p-vector tags 1..n
+        args.append((kap, Mb, Q, ps, qmax))
+    jobs = [srv.submit(*a) for a in args]
+    srv.start()
+    for j in jobs:
+        j.wait(timeout=30)
+    srv.stop(timeout=10)
+    assert sum(calls) == 13 + 5 + 21 + 2 and len(calls) >= 6
+    for (kap, Mb, Q, ps, qmax), j in zip(args, jobs):
+        P = j.points
+        assert P["pidx"].tolist() == list(range(len(ps)))
+        assert P["M"][:, 1].tolist() == list(range(1, len(ps) + 1))
+    assert srv.stats()["jobs"] == 4
 
 
 def test_server_fails_jobs_of_a_failed_iteration():
