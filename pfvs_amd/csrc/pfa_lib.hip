@@ -464,15 +464,44 @@ __device__ static inline int donate(const BatchArgs &a, char *stk, int bot, int 
     return k;
 }
 
+// The top of each wavefront's deque is cached in LDS: entries with index
+// >= cb live in a circular buffer of PFA_SL entries (slot = index mod
+// PFA_SL) and are authoritative there; entries below cb are in the deque's
+// global array at their own index.  Every step's node reads and child
+// pushes touch the top only, so they stay in LDS; entries move to and from
+// global memory only when the cached part outgrows the buffer, when the
+// top 64 are not all cached, and for donation and compaction.
+#define PFA_SL 128
+
+// copy deque entries [from, to) between the LDS cache and the global array
+// (to_global: LDS -> global), wave-cooperatively in 16-byte words
+__device__ static inline void cache_move(char *cache, char *stk, int from, int to, int ES,
+                                         int lane, bool to_global)
+{
+    __syncthreads();                       // earlier LDS writes of any lane first
+    const int w = ES / 16, total = (to - from) * w;
+    for (int t = lane; t < total; t += 64) {
+        const int idx = from + t / w, k = t % w;
+        int4 *g = (int4 *)(stk + (size_t)idx * ES) + k;
+        int4 *l = (int4 *)(cache + (size_t)(idx & (PFA_SL - 1)) * ES) + k;
+        if (to_global) *g = *l;
+        else *l = *g;
+    }
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
 {
     const int lane = threadIdx.x;
     const int ES = a.E.bytes, D = a.E.D;
     char *stk = a.stacks + (size_t)blockIdx.x * a.stack_cap * ES;
+    extern __shared__ __align__(16) char s_cache[];     // PFA_SL entries
     __shared__ int s_incl[64];
     __shared__ unsigned long long s_u64[2];
     __shared__ int32_t s_c[64 * PFA_CSTRIDE];
     int32_t *c = s_c + lane * PFA_CSTRIDE;
+    auto ent = [&](int idx) -> char * { return s_cache + (size_t)(idx & (PFA_SL - 1)) * ES; };
+    int cb = 0;                                         // cache base
     const unsigned long long t0 = wall_clock64();
     const unsigned long long c0 = clock64();
     unsigned long long n_nodes = 0, n_cand = 0, n_leaf = 0, n_wait = 0, n_fetch = 0, n_don = 0;
@@ -511,12 +540,13 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
             if (h == ~0ULL) break;
             __threadfence();                               // acquire the entry
             const int4 *src = (const int4 *)(a.Q.ring + (h % a.Q.cap) * ES);
-            int4 *dst = (int4 *)stk;
+            int4 *dst = (int4 *)ent(0);
             for (int k = lane; k < ES / 16; k += 64) dst[k] = src[k];
             __syncthreads();
             if (lane == 0) atomicAdd(&a.Q.ctl[64], 1ULL);
             bot = 0;
             top = 1;
+            cb = 0;
             have_unit = 1;
         }
         if ((++steps & 255) == 0 && a.deadline) {
@@ -529,12 +559,17 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
                 break;
             }
         }
-        // gather up to 64 candidates from the top nodes (lane l: node top-1-l)
+        // gather up to 64 candidates from the top nodes (lane l: node top-1-l),
+        // all of them cached
         const int live = top - bot;
         const int ntop = live < 64 ? live : 64;
+        if (cb > top - ntop) {
+            cache_move(s_cache, stk, top - ntop, cb, ES, lane, false);
+            cb = top - ntop;
+        }
         int cur = 0, hi = -1;
         if (lane < ntop) {
-            const PfaNode *nd = (const PfaNode *)(stk + (size_t)(top - 1 - lane) * ES);
+            const PfaNode *nd = (const PfaNode *)ent(top - 1 - lane);
             cur = nd->cur;
             hi = nd->hi;
         }
@@ -559,7 +594,7 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
         PfaNode nd;
         int32_t pidx = 0, v = 0;
         if (j >= 0) {
-            const char *e = stk + (size_t)(top - 1 - j) * ES;
+            const char *e = ent(top - 1 - j);
             nd = *(const PfaNode *)e;
             pidx = ((const int32_t *)(e + 128))[0];
             const int32_t *ec = (const int32_t *)(e + 144);
@@ -569,13 +604,13 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
         }
         int part_new_cur = 0;
         if (F < ntop && lane == 0) {
-            const PfaNode *pn = (const PfaNode *)(stk + (size_t)(top - 1 - F) * ES);
+            const PfaNode *pn = (const PfaNode *)ent(top - 1 - F);
             part_new_cur = pn->cur + (64 - (F > 0 ? s_incl[F - 1] : 0));
         }
         __syncthreads();                     // every parent read before any write
         int newtop = top - F;
         if (F < ntop && lane == 0)
-            ((PfaNode *)(stk + (size_t)(top - 1 - F) * ES))->cur = part_new_cur;
+            ((PfaNode *)ent(top - 1 - F))->cur = part_new_cur;
         int has_child = 0, emit = 0;
         int64_t q = 0;
         PfaNode ch;
@@ -630,6 +665,8 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
         const int nch = __popcll(cm);
         if (newtop + nch > a.stack_cap) {
             __syncthreads();
+            // the whole deque to the global array, then compact or donate there
+            cache_move(s_cache, stk, cb, newtop, ES, lane, true);
             if (bot > 0) {
                 // compact [bot, newtop) to 0 in chunks of bot entries
                 for (int base = bot; base < newtop; base += bot) {
@@ -652,10 +689,16 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
                 }
                 newtop -= k;
             }
+            cb = newtop;                     // nothing cached
+        }
+        if (newtop + nch - cb > PFA_SL) {    // room for the children: oldest cached out
+            const int nb = newtop + nch - PFA_SL;
+            cache_move(s_cache, stk, cb, nb, ES, lane, true);
+            cb = nb;
         }
         if (has_child) {
             int slot = newtop + __popcll(cm & ((1ULL << lane) - 1));
-            char *e = stk + (size_t)slot * ES;
+            char *e = ent(slot);
             *(PfaNode *)e = ch;
             ((int32_t *)(e + 128))[0] = pidx;
             int32_t *ec = (int32_t *)(e + 144);
@@ -679,6 +722,10 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
                 if ((unsigned long long)k > waiting) k = (int)waiting;
                 if (k > a.donate_max) k = a.donate_max;
                 waiting -= k;
+                if (cb < bot + k) {          // donated entries come from the global array
+                    cache_move(s_cache, stk, cb, bot + k, ES, lane, true);
+                    cb = bot + k;
+                }
                 int got = donate(a, stk, bot, k, 0, lane, s_u64);
                 bot += got;
                 n_don += got;
@@ -1376,7 +1423,8 @@ struct DevCtx {
     Stage stage[2];
     int next_stage = 0;                    // (under mu)
     int cus = 0;
-    size_t lds_max = 0;
+    size_t lds_max = 0;                    // per workgroup
+    size_t lds_cu = 0;                     // per CU
     unsigned long long pcap_hint = 0;      // largest point count seen
     double ppp_hint = 0;                   // largest points per p-vector seen (n_p >= 64)
     double ms_alloc = 0;                   // this run's buffer growth (host ms)
@@ -1403,6 +1451,8 @@ static void ctx_init(DevCtx *C, int device, char *errbuf)
     HIPCHK(hipGetDeviceProperties(&prop, device));
     C->cus = prop.multiProcessorCount;
     C->lds_max = prop.sharedMemPerBlock;
+    C->lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
+                                                      : prop.sharedMemPerBlock;
 }
 
 static void *dev_get(DevCtx *C, int slot, size_t bytes, char *errbuf)
@@ -1870,7 +1920,12 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
         pcap, (unsigned long long)std::min(1.25 * C->ppp_hint * (double)n_p, (double)(1 << 26)));
     // defaults measured on MI355X (h11 = 20 batch): fewer, deeper wavefronts
     // keep the stacks' hot tops in L2
-    const int waves_per_cu = in->waves_per_cu > 0 ? in->waves_per_cu : 4;
+    int waves_per_cu = in->waves_per_cu > 0 ? in->waves_per_cu : 4;
+    // the batched search keeps PFA_SL deque entries per wavefront in LDS:
+    // launch no more wavefronts per CU than LDS holds resident
+    const size_t bat_lds = (size_t)PFA_SL * E.bytes;
+    const size_t bat_lds_all = bat_lds + 64 * 4 + 16 + 64 * PFA_CSTRIDE * 4;
+    waves_per_cu = std::max(1, std::min<int>(waves_per_cu, (int)(C->lds_cu / bat_lds_all)));
     const int donate_max = in->budget > 0 ? in->budget : 8;
     const int donate_min = 64;               // deque depth before a wavefront donates
     const int pt_chunk = 256;                // point slots reserved per atomic
@@ -1943,7 +1998,9 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
             HIPCHK(hipEventRecord(ev[3], s));
             BatchArgs A{d_recs, L, E, Qa, stacks, stack_cap, donate_max, donate_min, pt_chunk,
                         Pt, Cn, deadline};
-            hipLaunchKernelGGL(k_batched, dim3(nwaves), dim3(64), 0, s, A);
+            HIPCHK(hipFuncSetAttribute((const void *)k_batched,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bat_lds));
+            hipLaunchKernelGGL(k_batched, dim3(nwaves), dim3(64), bat_lds, s, A);
             HIPCHK(hipGetLastError());
             launches = passes = 1;
             HIPCHK(hipEventRecord(ev[4], s));
