@@ -495,11 +495,45 @@ PFA_HD static inline int pfa_lll_form(const PfaTeam &T, const PfaWS &W, int64_t 
     return pfa_lll_core(T, W, B, m, n, W.GG, last);
 }
 
+// max |a_e| over n entries, the same value on every lane of the team
+PFA_HD static inline uint64_t pfa_team_absmax(const PfaTeam &T, const int64_t *a, int n)
+{
+    uint64_t m = 0;
+    for (int e = T.lane; e < n; e += T.n) {
+        const uint64_t x = a[e] < 0 ? 0 - (uint64_t)a[e] : (uint64_t)a[e];
+        m = x > m ? x : m;
+    }
+#if defined(__HIP_DEVICE_COMPILE__)
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t y = __shfl_xor(m, o, 64);
+        m = y > m ? y : m;
+    }
+#endif
+    return m;
+}
+
+// Sums of k products of entries bounded by amax and bmax stay below 2^62,
+// so they are exact in int64 and need no overflow checks.
+PFA_HD static inline bool pfa_int64_safe(uint64_t amax, uint64_t bmax, int k)
+{
+    return (double)amax * (double)bmax * (double)k < 0x1p62;
+}
+
 // C = A B (int64, exact checks); entries split over lanes.  C must not alias.
 PFA_HD static inline int pfa_matmul(const PfaTeam &T, const PfaWS &W, const int64_t *A,
                                     const int64_t *B, int64_t *C, int m, int k, int n)
 {
     int bad = 0;
+    if (pfa_int64_safe(pfa_team_absmax(T, A, m * k), pfa_team_absmax(T, B, k * n), k)) {
+        for (int e = T.lane; e < m * n; e += T.n) {
+            const int i = e / n, j = e % n;
+            int64_t s = 0;
+            for (int l = 0; l < k; ++l) s += A[i * k + l] * B[l * n + j];
+            C[e] = s;
+        }
+        PFA_SYNC();
+        return 0;
+    }
     for (int e = T.lane; e < m * n; e += T.n) {
         int i = e / n, j = e % n;
         pfa_i128 s = 0;
@@ -645,12 +679,20 @@ PFA_HD static inline int pfa_build_lattice(const PfaTeam &T, const PfaWS &W, con
     const int h = W.h, d = W.d;
     int bad = 0, rc;
     pfa_team_clear(T, W.flag);
-    for (int e = T.lane; e < h * h; e += T.n) {
-        pfa_i128 s = 0;
-        for (int k = 0; k < h; ++k)
-            if (pfa_add_ovf(s, (pfa_i128)kappa[(size_t)e * h + k] * p[k], &s)) bad = 1;
-        if (!pfa_fits64(s)) bad = 1;
-        W.Z[e] = (int64_t)s;
+    if (pfa_int64_safe(pfa_team_absmax(T, kappa, h * h * h), pfa_team_absmax(T, p, h), h)) {
+        for (int e = T.lane; e < h * h; e += T.n) {
+            int64_t s = 0;
+            for (int k = 0; k < h; ++k) s += kappa[(size_t)e * h + k] * p[k];
+            W.Z[e] = s;
+        }
+    } else {
+        for (int e = T.lane; e < h * h; e += T.n) {
+            pfa_i128 s = 0;
+            for (int k = 0; k < h; ++k)
+                if (pfa_add_ovf(s, (pfa_i128)kappa[(size_t)e * h + k] * p[k], &s)) bad = 1;
+            if (!pfa_fits64(s)) bad = 1;
+            W.Z[e] = (int64_t)s;
+        }
     }
     if (pfa_team_any(T, W.flag, bad)) return -1;
     // T = Z p into W.W2[0:h]; v = T^T Mbasis into W.vec
@@ -681,13 +723,22 @@ PFA_HD static inline int pfa_build_lattice(const PfaTeam &T, const PfaWS &W, con
     if ((rc = pfa_lll_euclid(T, W, O, d, h))) return rc * 10 - 2;
     // BT[a][i] = sum_l Mbasis[i][l] O[a][l]  -> W2 (d, h)
     int64_t *BT = W.W2;
-    for (int e = T.lane; e < d * h; e += T.n) {
-        int a = e / h, i = e % h;
-        pfa_i128 s = 0;
-        for (int l = 0; l < h; ++l)
-            if (pfa_add_ovf(s, (pfa_i128)Mbasis[i * h + l] * O[a * h + l], &s)) bad = 1;
-        if (!pfa_fits64(s)) bad = 1;
-        BT[e] = (int64_t)s;
+    if (pfa_int64_safe(pfa_team_absmax(T, Mbasis, h * h), pfa_team_absmax(T, O, d * h), h)) {
+        for (int e = T.lane; e < d * h; e += T.n) {
+            const int a = e / h, i = e % h;
+            int64_t s = 0;
+            for (int l = 0; l < h; ++l) s += Mbasis[i * h + l] * O[a * h + l];
+            BT[e] = s;
+        }
+    } else {
+        for (int e = T.lane; e < d * h; e += T.n) {
+            int a = e / h, i = e % h;
+            pfa_i128 s = 0;
+            for (int l = 0; l < h; ++l)
+                if (pfa_add_ovf(s, (pfa_i128)Mbasis[i * h + l] * O[a * h + l], &s)) bad = 1;
+            if (!pfa_fits64(s)) bad = 1;
+            BT[e] = (int64_t)s;
+        }
     }
     if (pfa_team_any(T, W.flag, bad)) return -1;
     // columns with Binter[0] == 0 first (stable); order in W3
@@ -704,13 +755,22 @@ PFA_HD static inline int pfa_build_lattice(const PfaTeam &T, const PfaWS &W, con
     }
     PFA_SYNC();
     if (pfa_matmul(T, W, W.Z, W.Bi, W.ZB, h, h, d)) return -1;
-    for (int e = T.lane; e < d * d; e += T.n) {
-        int a = e / d, b = e % d;
-        pfa_i128 s = 0;
-        for (int i = 0; i < h; ++i)
-            if (pfa_add_ovf(s, (pfa_i128)W.Bi[i * d + a] * W.ZB[i * d + b], &s)) bad = 1;
-        if (!pfa_fits64(s)) bad = 1;
-        W.mat[e] = (int64_t)(-s);
+    if (pfa_int64_safe(pfa_team_absmax(T, W.Bi, h * d), pfa_team_absmax(T, W.ZB, h * d), h)) {
+        for (int e = T.lane; e < d * d; e += T.n) {
+            const int a = e / d, b = e % d;
+            int64_t s = 0;
+            for (int i = 0; i < h; ++i) s += W.Bi[i * d + a] * W.ZB[i * d + b];
+            W.mat[e] = -s;
+        }
+    } else {
+        for (int e = T.lane; e < d * d; e += T.n) {
+            int a = e / d, b = e % d;
+            pfa_i128 s = 0;
+            for (int i = 0; i < h; ++i)
+                if (pfa_add_ovf(s, (pfa_i128)W.Bi[i * d + a] * W.ZB[i * d + b], &s)) bad = 1;
+            if (!pfa_fits64(s)) bad = 1;
+            W.mat[e] = (int64_t)(-s);
+        }
     }
     if (pfa_team_any(T, W.flag, bad)) return -1;
 
@@ -739,13 +799,22 @@ PFA_HD static inline int pfa_build_lattice(const PfaTeam &T, const PfaWS &W, con
     if (pfa_matmul(T, W, W.ZB, Tm, W.W3, h, d, d)) return -14;
     pfa_copy(T, W.ZB, W.W3, h * d);
     if (pfa_matmul(T, W, W.mat, Tm, W.W3, d, d, d)) return -14;   // MT = mat T
-    for (int e = T.lane; e < d * d; e += T.n) {
-        int a = e / d, b = e % d;
-        pfa_i128 s = 0;
-        for (int i = 0; i < d; ++i)
-            if (pfa_add_ovf(s, (pfa_i128)Tm[i * d + a] * W.W3[i * d + b], &s)) bad = 1;
-        if (!pfa_fits64(s)) bad = 1;
-        W.W1[e] = (int64_t)s;
+    if (pfa_int64_safe(pfa_team_absmax(T, Tm, d * d), pfa_team_absmax(T, W.W3, d * d), d)) {
+        for (int e = T.lane; e < d * d; e += T.n) {
+            const int a = e / d, b = e % d;
+            int64_t s = 0;
+            for (int i = 0; i < d; ++i) s += Tm[i * d + a] * W.W3[i * d + b];
+            W.W1[e] = s;
+        }
+    } else {
+        for (int e = T.lane; e < d * d; e += T.n) {
+            int a = e / d, b = e % d;
+            pfa_i128 s = 0;
+            for (int i = 0; i < d; ++i)
+                if (pfa_add_ovf(s, (pfa_i128)Tm[i * d + a] * W.W3[i * d + b], &s)) bad = 1;
+            if (!pfa_fits64(s)) bad = 1;
+            W.W1[e] = (int64_t)s;
+        }
     }
     if (pfa_team_any(T, W.flag, bad)) return -14;
     pfa_copy(T, W.mat, W.W1, d * d);
