@@ -177,6 +177,22 @@ int pfa_det_filter(int n_geo, const int32_t *h, const int64_t *kappa, int64_t co
  * counters.  No counterpart in the reference.  Returns 0 or 1. */
 int pfa_copy_probe(int device, int64_t bytes, int reps, double *ms_per_copy);
 
+/* All integer p with |p_j| <= B (n coordinates) and H p >= rhs row-wise
+ * (H: m x n int32, row-major), primitive ones only when `primitive`, in
+ * lexicographic order (p_0 outermost, ascending).  Replaces the
+ * enumeration the reference takes from latticepts (box_enum(B, H, rhs, cap,
+ * primitive) at pfvs/distributed.py:121-139 and
+ * tests/test_paper_examples.py:119-121; enum_lattice_points at
+ * pfvs/pvectors.py:58-89).  Runs on host threads (device < 0); device >= 0
+ * returns 2 (the device form is not enabled in this build).  The first
+ * min(total, cap) points go to out (cap x n
+ * int64); *n_out gets that number, *n_total the count of all, *ms (if not
+ * null) the time.  Returns 0; 1 when total > cap (out holds the first cap
+ * points); 2 on an error (message in err, 256 bytes).  m <= 256, n <= 64. */
+int pfa_box_enum(const int32_t *H, int m, int n, const int64_t *rhs, int B, int primitive,
+                 int device, int64_t cap, int64_t *out, int64_t *n_out, int64_t *n_total,
+                 double *ms, char *err);
+
 /* Grows the per-device buffers (device memory and both pinned result
  * staging sets) to hold a batch of n_p p-vectors and n_points points at
  * h11 <= h_max, so that later calls of that size allocate nothing; also

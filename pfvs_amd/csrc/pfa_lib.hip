@@ -31,6 +31,7 @@
 #include <thread>
 #include <vector>
 
+#include "pfa_box.h"
 #include "pfa_int.h"
 #include "pfa_post.h"
 #include "pfa_prep.h"
@@ -1499,7 +1500,8 @@ enum {
     B_GEO_H, B_GEO_Q, B_GEO_QMAX, B_GEO_LIN, B_KAPPA, B_MBASIS, B_KOFS, B_MOFS, B_PS, B_PGEO,
     B_RECS, B_RING, B_SEQ, B_STACKS, B_CTL, B_PT_PIDX, B_PT_Q, B_PT_C, B_CNT, B_M, B_KN, B_NEXP, B_BAD,
     B_PRIMES, B_KT, B_KTOFS, B_KTMAX, B_MODCNT, B_MODLIST, B_MODNEXP, B_MODM, B_NONSING, B_PREPCOLD,
-    B_TOT, B_KEEPCNT, B_KEEPKP, B_KEEPNEXP, B_KEEPMETA, B_PREPORDER, B_NBUF
+    B_TOT, B_KEEPCNT, B_KEEPKP, B_KEEPNEXP, B_KEEPMETA, B_PREPORDER,
+    B_NBUF
 };
 
 // the k largest primes below 2^31 with their Barrett constants
@@ -2411,6 +2413,85 @@ extern "C" void pfg_release(int device)
             b.n = 0;
         }
     }
+}
+
+// --------------------------------------------------- p-vector enumeration
+// prefix length: the fewest leading coordinates giving at least `target`
+// units, at most n and at most 2^24 units
+static int box_prefix(int n, int B, int64_t target)
+{
+    const int64_t W = 2 * (int64_t)B + 1;
+    int s = 0;
+    int64_t units = 1;
+    while (s < n && units < target && units * W <= (1 << 24)) {
+        units *= W;
+        ++s;
+    }
+    return s;
+}
+
+extern "C" int pfa_box_enum(const int32_t *H, int m, int n, const int64_t *rhs, int B, int primitive,
+                            int device, int64_t cap, int64_t *out, int64_t *n_out, int64_t *n_total,
+                            double *ms, char *err)
+{
+    char *errbuf = err;
+    *n_out = 0;
+    *n_total = 0;
+    if (ms) *ms = 0;
+    if (m < 0 || m > PFA_BOX_MAXM || n < 1 || n > PFA_BOX_MAXN || B < 0 || B > (1 << 20) || cap < 0) {
+        snprintf(errbuf, 256, "pfa_box_enum: unsupported sizes (m %d <= %d, 1 <= n %d <= %d, B %d)",
+                 m, PFA_BOX_MAXM, n, PFA_BOX_MAXN, B);
+        return 2;
+    }
+    // H^T and the reachable maxima rest[j][r] = B sum_{k >= j} |H_rk|
+    std::vector<int32_t> Ht((size_t)n * std::max(m, 1));
+    std::vector<int64_t> rest((size_t)(n + 1) * std::max(m, 1), 0);
+    for (int r = 0; r < m; ++r)
+        for (int j = 0; j < n; ++j) Ht[(size_t)j * m + r] = H[(size_t)r * n + j];
+    for (int j = n - 1; j >= 0; --j)
+        for (int r = 0; r < m; ++r) {
+            const int64_t a = H[(size_t)r * n + j];
+            rest[(size_t)j * m + r] = rest[(size_t)(j + 1) * m + r] + (int64_t)B * (a < 0 ? -a : a);
+        }
+    std::vector<int64_t> rhs_v(std::max(m, 1), 0);
+    for (int r = 0; r < m; ++r) rhs_v[r] = rhs[r];
+    if (device >= 0) {
+        snprintf(errbuf, 256, "pfa_box_enum: the device form is not enabled in this build; "
+                              "use device < 0 (host threads)");
+        return 2;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    try {
+        {
+            const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+            const int s = box_prefix(n, B, 64 * nth);
+            PfaBox X{m, n, B, primitive ? 1 : 0, s, Ht.data(), rest.data(), rhs_v.data()};
+            int64_t n_units = 1;
+            for (int k = 0; k < s; ++k) n_units *= 2 * (int64_t)B + 1;
+            std::vector<int64_t> counts(n_units), offs(n_units);
+            auto run = [&](bool write) {
+                parallel_chunks(n_units, 16, nth, [&](int64_t lo, int64_t hi) {
+                    int32_t p[PFA_BOX_MAXN], cur[PFA_BOX_MAXN], hh[PFA_BOX_MAXN];
+                    uint64_t g[PFA_BOX_MAXN];
+                    const PfaBoxTeam T{0, 1};
+                    for (int64_t u = lo; u < hi; ++u) {
+                        if (write) pfa_box_unit<PFA_BOX_MAXM>(X, T, u, p, cur, hh, g, out, offs[u], cap);
+                        else counts[u] = pfa_box_unit<PFA_BOX_MAXM>(X, T, u, p, cur, hh, g, nullptr, 0, 0);
+                    }
+                });
+            };
+            run(false);
+            int64_t tot = 0;
+            for (int64_t u = 0; u < n_units; ++u) { offs[u] = tot; tot += counts[u]; }
+            *n_total = tot;
+            run(true);
+            *n_out = std::min(tot, cap);
+        }
+    } catch (int) {
+        return 2;
+    }
+    if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return *n_total > cap ? 1 : 0;
 }
 
 extern "C" int pfa_reserve(int device, int64_t n_p, int64_t n_points, int h_max, char *err)

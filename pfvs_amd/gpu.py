@@ -138,6 +138,11 @@ def _load():
     lib.pfa_reserve.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
                                 ctypes.c_char_p]
     lib.pfa_reserve.restype = ctypes.c_int
+    lib.pfa_box_enum.argtypes = [_i32p, ctypes.c_int, ctypes.c_int, _i64p, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_int64, _i64p,
+                                 ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                 ctypes.POINTER(ctypes.c_double), ctypes.c_char_p]
+    lib.pfa_box_enum.restype = ctypes.c_int
     _lib, _lib_path = lib, path
     return lib
 
@@ -178,6 +183,34 @@ def release(device: int = 0) -> None:
     lib = _load()
     if lib is not None:
         lib.pfg_release(int(device))
+
+
+def box_enum(H, rhs, B, primitive=True, cap=10**8, device=0):
+    """The library's p-vector enumeration (pfa_box_enum): the integer p with
+    |p|_inf <= B and H p >= rhs, primitive if asked, in lexicographic order.
+    Returns (ps (n, ncols) int64 -- the first min(total, cap) --, status
+    (0, or 1 when total > cap), total, ms).  device < 0 runs the host code."""
+    lib = require()
+    H = np.ascontiguousarray(H, dtype=np.int32)
+    m, n = H.shape
+    rhs = np.ascontiguousarray(np.broadcast_to(np.asarray(rhs, dtype=np.int64), (m,)))
+    if H.size and np.abs(H.astype(np.int64)).max() >= 2**31:
+        raise ValueError("H entries must fit int32")
+    cap = max(int(cap), 0)
+    if cap <= 1 << 16:
+        out = np.empty((cap, n), dtype=np.int64)
+    else:                                       # count first, then hold exactly the rows
+        _, _, total, _ = box_enum(H, rhs, B, primitive, 0, device)
+        out = np.empty((min(total, cap), n), dtype=np.int64)
+    n_out, n_tot, ms = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_double(0)
+    err = ctypes.create_string_buffer(256)
+    rc = lib.pfa_box_enum(H.ctypes.data_as(_i32p), m, n, _p64(rhs), int(B), int(bool(primitive)),
+                          int(device), int(min(int(cap), len(out))), _p64(out), ctypes.byref(n_out),
+                          ctypes.byref(n_tot), ctypes.byref(ms), err)
+    if rc == 2:
+        raise RuntimeError(f"box_enum failed: {err.value.decode(errors='replace')}")
+    st = 1 if n_tot.value > cap else 0
+    return out[:n_out.value], st, int(n_tot.value), float(ms.value)
 
 
 def reserve(n_p, n_points, h_max=None, device=0):
