@@ -863,6 +863,14 @@ struct FiltArgs {
     const int64_t *kTofs;         // per geometry (identical kappas share one copy)
     const double *kTmax;          // per geometry max |kappa| (inf when >= 2^31)
     int lds_floats;               // LDS floats per block (0: no staging)
+    // sparse form of kappa per distinct kappa (geometries where every
+    // |kappa| < 2^23): for column j and row s of N, the nonzero terms
+    // kappa[s+1][j+1][k] packed (k << 24 | value & 0xffffff) in
+    // spKV[spKVofs + spR[spRofs + j n + s] .. + spR[spRofs + j n + s + 1])
+    const int32_t *spR, *spKV;
+    const int64_t *spRofs;        // per geometry, -1: no sparse form
+    const int64_t *spKVofs;
+    const int32_t *spInts;        // per geometry: LDS ints of its sparse form (n^2 + 1 + nnz)
     const PfaModP *primes;
     int np;
     int64_t modcap;               // list capacity; beyond it a point stays undecided
@@ -1044,6 +1052,45 @@ __device__ __forceinline__ void det_build_i24_lds(double *w, const int32_t *ktj,
     for (int s = 0; s < NMAX; ++s) w[s] = (double)acc[s];
 }
 
+// LDS writes of any lane of the wavefront before reads of any other
+// (wavefront-scope release, wave barrier, acquire)
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// N column j from the sparse form of kappa staged in LDS (ranges R, n^2 + 1,
+// then the packed terms), the group's M in sM (sM[k] = M_k): only the
+// nonzero kappa terms, same int32 sums as the dense forms (|kappa|, |M|
+// < 2^23 and every partial sum below 2^31).
+template <int NMAX>
+__device__ __forceinline__ void det_build_sparse_lds(double *w, const int32_t *sp, const int32_t *sM,
+                                                     int n, int j, bool col)
+{
+    const int32_t *R = sp + (col ? j * n : 0), *KV = sp + n * n + 1;
+    int32_t acc[NMAX];
+#pragma unroll
+    for (int s = 0; s < NMAX; ++s) acc[s] = 0;
+    if (col) {
+        int b = R[0];
+#pragma unroll
+        for (int s = 0; s < NMAX; ++s) {
+            if (s < n) {
+                const int e = R[s + 1];
+                for (int t = b; t < e; ++t) {
+                    const int32_t x = KV[t];
+                    acc[s] += (x << 8 >> 8) * sM[(x >> 24) & 63];
+                }
+                b = e;
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NMAX; ++s) w[s] = (double)acc[s];
+}
+
 template <int NMAX>
 __device__ __forceinline__ void det_build_f64(double *w, const int32_t *kt, double md, int src0,
                                               int hl, int n, bool col)
@@ -1058,12 +1105,16 @@ __device__ __forceinline__ void det_build_f64(double *w, const int32_t *kt, doub
     }
 }
 
-template <int NMAX>
+// SP: the launch has geometries with a sparse form of kappa (a separate
+// instantiation, so the dense-only kernel keeps its registers)
+template <int NMAX, bool SP>
 __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
 {
     extern __shared__ int32_t lds_kT[];
     __shared__ int64_t lds_ofs;
     __shared__ int lds_cnt;
+    __shared__ int lds_sparse;
+    __shared__ int32_t s_M[SP ? 256 : 1];          // the groups' M (sparse build)
     __shared__ int64_t s_kofs[256];                // kappa offset and bound of geometries < 256
     __shared__ double s_kmax[256];
     const DetLanes L = det_lanes(a.hl);
@@ -1080,29 +1131,47 @@ __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
     if (rd0 >= rd1) return;
     // stage kappa of the geometry of the span's first point
     int64_t cofs = -1;
+    int sparse = 0;
     if (a.lds_floats > 0) {
+        // the kappa of the span's first point: its sparse form when it has
+        // one that fits, else the padded dense layout when that fits
         if (threadIdx.x == 0) {
             int64_t o = -1;
-            int c = 0;
+            int c = 0, spg = 0;
             const int64_t t0 = rd0 * 4 * R;
             if (t0 < cnt) {
                 const int64_t kp = slot(t0);
                 const int32_t *ri = (const int32_t *)(a.recs + (int64_t)a.pidx[kp] * a.L.words);
                 const int h = ri[24], g = ri[26];
-                if (h > 1 && h - 1 <= NMAX && (NMAX + 1) * NMAX * NMAX <= a.lds_floats &&
-                    a.kTmax[g] < PFA_TWO52) {
+                if (SP && h > 1 && h - 1 <= NMAX && a.spRofs[g] >= 0 && a.spInts[g] <= a.lds_floats) {
+                    o = a.kTofs[g];
+                    c = h - 1;
+                    spg = g + 1;
+                } else if (h > 1 && h - 1 <= NMAX && (NMAX + 1) * NMAX * NMAX <= a.lds_floats &&
+                           a.kTmax[g] < PFA_TWO52) {
                     o = a.kTofs[g];
                     c = h - 1;
                 }
             }
-            lds_ofs = o;
+            lds_ofs = o;                       // identifies the staged kappa
             lds_cnt = c;
+            lds_sparse = spg;                  // geometry + 1 of the sparse form, or 0
         }
         __syncthreads();
         cofs = lds_ofs;
+        sparse = SP && lds_sparse > 0;
+        if (SP && sparse) {
+            // ranges, then the packed terms
+            const int g = lds_sparse - 1, nr = lds_cnt * lds_cnt + 1;
+            const int32_t *Rg = a.spR + a.spRofs[g];
+            const int32_t *KVg = a.spKV + a.spKVofs[g];
+            for (int e = threadIdx.x; e < a.spInts[g]; e += blockDim.x)
+                lds_kT[e] = e < nr ? Rg[e] : KVg[e - nr];
+            __syncthreads();
+        }
         // zero-padded copy: lds[(k NMAX + s) NMAX + j] = kT[(k n + s) n + j]
         const int nn = lds_cnt, cnt = (nn + 1) * nn * nn;
-        if (cofs >= 0) {
+        if (cofs >= 0 && !sparse) {
             for (int e = threadIdx.x; e < (NMAX + 1) * NMAX * NMAX; e += blockDim.x) lds_kT[e] = 0;
             __syncthreads();
             for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
@@ -1188,9 +1257,19 @@ __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
             const bool all_lds = !__ballot(col && !lds);
             const int32_t *kt = a.kT + (col ? kofs_of(geo) : 0) + i;
             bool done = false;
+            if constexpr (SP) {
+                if (sparse && all_lds && !__ballot(!small)) {
+                    // the group's M through LDS (lane i of the group holds M_i)
+                    wave_lds_sync();
+                    s_M[threadIdx.x] = (int32_t)Mi;
+                    wave_lds_sync();
+                    det_build_sparse_lds<NMAX>(w, lds_kT, s_M + (wv << 6) + L.src0, n, i, col);
+                    done = true;
+                }
+            }
             if constexpr (NMAX <= 24) {
-                // (larger NMAX never stage kappa: its padded slab exceeds 64 KB)
-                if (all_lds && !__ballot(!small)) {
+                // (larger NMAX never stage the padded slab: it exceeds 64 KB)
+                if (!done && !sparse && all_lds && !__ballot(!small)) {
                     // lanes without a column read column 0 and keep w = 0
                     det_build_i24_lds<NMAX>(w, lds_kT + (col ? i : 0), (int32_t)Mi, L.src0, n);
                     if (!col) {
@@ -1501,7 +1580,7 @@ enum {
     B_RECS, B_RING, B_SEQ, B_STACKS, B_CTL, B_PT_PIDX, B_PT_Q, B_PT_C, B_CNT, B_M, B_KN, B_NEXP, B_BAD,
     B_PRIMES, B_KT, B_KTOFS, B_KTMAX, B_MODCNT, B_MODLIST, B_MODNEXP, B_MODM, B_NONSING, B_PREPCOLD,
     B_TOT, B_KEEPCNT, B_KEEPKP, B_KEEPNEXP, B_KEEPMETA, B_PREPORDER,
-    B_NBUF
+    B_SPR, B_SPKV, B_SPROFS, B_SPKVOFS, B_SPINTS, B_NBUF
 };
 
 // the k largest primes below 2^31 with their Barrett constants
@@ -1647,10 +1726,47 @@ static void fill_geo_offsets(const pfa_run_input *in, std::vector<int64_t> &kofs
 // lanes of a point group read consecutive j), one copy per distinct kappa
 // (kTofs per geometry), and the largest |kappa| of each geometry
 // (infinity when one reaches 2^31).
+// The sparse form of one kappa (see FiltArgs): ranges spR (n^2 + 1, by
+// column then row) and packed terms spKV; empty when an entry needs 24 bits
+// or more, or when more than a quarter of the terms are nonzero (the dense
+// build's fixed, predicate-free layout is faster there).
+static void build_sparse(const int64_t *kap, int64_t h, std::vector<int32_t> &R,
+                         std::vector<int32_t> &KV)
+{
+    const int64_t n = h - 1;
+    R.assign(n * n + 1, 0);
+    KV.clear();
+    for (int64_t j = 0; j < n; ++j)
+        for (int64_t sr = 0; sr < n; ++sr) {
+            for (int64_t k = 0; k < h; ++k) {
+                const int64_t v = kap[((sr + 1) * h + (j + 1)) * h + k];
+                if (!v) continue;
+                if (v >= (1 << 23) || v <= -(1 << 23)) {
+                    R.clear();
+                    KV.clear();
+                    return;
+                }
+                KV.push_back((int32_t)(((uint32_t)k << 24) | ((uint32_t)v & 0xffffffu)));
+            }
+            R[j * n + sr + 1] = (int32_t)KV.size();
+        }
+    if (4 * (int64_t)KV.size() > n * n * h) {
+        R.clear();
+        KV.clear();
+    }
+}
+
 static void build_kT(const pfa_run_input *in, const std::vector<int64_t> &kofs,
                      std::vector<int32_t> &kT, std::vector<int64_t> &kTofs,
-                     std::vector<double> &kTmax)
+                     std::vector<double> &kTmax, std::vector<int32_t> &spR,
+                     std::vector<int32_t> &spKV, std::vector<int64_t> &spRofs,
+                     std::vector<int64_t> &spKVofs, std::vector<int32_t> &spInts)
 {
+    spRofs.assign(in->n_geo, -1);
+    spKVofs.assign(in->n_geo, 0);
+    spInts.assign(in->n_geo, 0);
+    spR.clear();
+    spKV.clear();
     kTofs.assign(in->n_geo, 0);
     kTmax.assign(in->n_geo, 0.0);
     kT.clear();
@@ -1670,7 +1786,21 @@ static void build_kT(const pfa_run_input *in, const std::vector<int64_t> &kofs,
         if (same >= 0) {
             kTofs[g] = kTofs[same];
             kTmax[g] = kTmax[same];
+            spRofs[g] = spRofs[same];
+            spKVofs[g] = spKVofs[same];
+            spInts[g] = spInts[same];
             continue;
+        }
+        {
+            std::vector<int32_t> R, KV;
+            build_sparse(kap, h, R, KV);
+            if (!R.empty()) {
+                spRofs[g] = (int64_t)spR.size();
+                spKVofs[g] = (int64_t)spKV.size();
+                spInts[g] = (int32_t)(R.size() + KV.size());
+                spR.insert(spR.end(), R.begin(), R.end());
+                spKV.insert(spKV.end(), KV.begin(), KV.end());
+            }
         }
         seen.push_back({hs, g});
         kTofs[g] = (int64_t)kT.size();
@@ -1686,6 +1816,8 @@ static void build_kT(const pfa_run_input *in, const std::vector<int64_t> &kofs,
         kTmax[g] = mx < 2147483648.0 ? mx : INFINITY;
     }
     if (kT.empty()) kT.push_back(0);
+    if (spR.empty()) spR.push_back(0);
+    if (spKV.empty()) spKV.push_back(0);
 }
 
 static void *upload(DevCtx *C, int slot, const void *src, size_t bytes, hipStream_t s, char *errbuf)
@@ -1703,20 +1835,29 @@ static void filter_run(DevCtx *C, hipStream_t s, const pfa_run_input *gin,
                        const std::vector<int64_t> &kofs, FiltArgs a, char *errbuf)
 {
     static const std::vector<PfaModP> pr = primes31(96);
-    std::vector<int32_t> kT;
+    std::vector<int32_t> kT, spR, spKV, spInts;
     std::vector<double> kTmax;
-    std::vector<int64_t> kTofs;
-    build_kT(gin, kofs, kT, kTofs, kTmax);
-    // LDS for kappa in the padded layout of the launch's NMAX, if it fits
+    std::vector<int64_t> kTofs, spRofs, spKVofs;
+    build_kT(gin, kofs, kT, kTofs, kTmax, spR, spKV, spRofs, spKVofs, spInts);
+    // LDS for kappa: the padded layout of the launch's NMAX if it fits, and
+    // room for the largest sparse form up to 48 KB
     const int nl = a.hl - 1;
     const int nmax = nl <= 8 ? 8 : nl <= 12 ? 12 : nl <= 16 ? 16 : nl <= 20 ? 20 : nl <= 24 ? 24 : nl <= 32 ? 32 : 40;
     int lds_floats = (nmax + 1) * nmax * nmax;
     if (lds_floats > 64 * 1024 / 4) lds_floats = 0;
+    if (nmax > 24)
+        for (int g = 0; g < gin->n_geo; ++g)
+            if (spRofs[g] >= 0 && spInts[g] <= 12 * 1024) lds_floats = std::max(lds_floats, (int)spInts[g]);
     unsigned long long *d_mc = (unsigned long long *)dev_get(C, B_MODCNT, 8, errbuf);
     HIPCHK(hipMemsetAsync(d_mc, 0, 8, s));
     a.kT = (const int32_t *)upload(C, B_KT, kT.data(), 4 * kT.size(), s, errbuf);
     a.kTofs = (const int64_t *)upload(C, B_KTOFS, kTofs.data(), 8 * kTofs.size(), s, errbuf);
     a.kTmax = (const double *)upload(C, B_KTMAX, kTmax.data(), 8 * kTmax.size(), s, errbuf);
+    a.spR = (const int32_t *)upload(C, B_SPR, spR.data(), 4 * spR.size(), s, errbuf);
+    a.spKV = (const int32_t *)upload(C, B_SPKV, spKV.data(), 4 * spKV.size(), s, errbuf);
+    a.spRofs = (const int64_t *)upload(C, B_SPROFS, spRofs.data(), 8 * spRofs.size(), s, errbuf);
+    a.spKVofs = (const int64_t *)upload(C, B_SPKVOFS, spKVofs.data(), 8 * spKVofs.size(), s, errbuf);
+    a.spInts = (const int32_t *)upload(C, B_SPINTS, spInts.data(), 4 * spInts.size(), s, errbuf);
     a.n_geo = gin->n_geo;
     a.lds_floats = a.filter ? lds_floats : 0;
     a.primes = (const PfaModP *)upload(C, B_PRIMES, pr.data(), sizeof(PfaModP) * pr.size(), s, errbuf);
@@ -1745,11 +1886,18 @@ static void filter_run(DevCtx *C, hipStream_t s, const pfa_run_input *gin,
     const int64_t rounds = (a.n + 4 * R - 1) / (4 * R);
     const int grid = (int)std::min<int64_t>(rounds, 16384);
     const size_t lds = (size_t)a.lds_floats * 4;
+    // the sparse form where the padded dense layout cannot be staged (NMAX
+    // > 24); below that the unrolled dense build is faster even at 20 %
+    // density
+    bool any_sparse = false;
+    for (int g = 0; g < gin->n_geo; ++g)
+        any_sparse |= nmax > 24 && spRofs[g] >= 0 && spInts[g] <= a.lds_floats;
 #define PFA_FILTER_LAUNCH(NM)                                                                  \
     do {                                                                                        \
-        HIPCHK(hipFuncSetAttribute((const void *)k_filter<NM>,                                 \
+        auto kf = any_sparse ? k_filter<NM, true> : k_filter<NM, false>;                       \
+        HIPCHK(hipFuncSetAttribute((const void *)kf,                                           \
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));      \
-        hipLaunchKernelGGL(k_filter<NM>, dim3(grid), dim3(256), lds, s, a);                     \
+        hipLaunchKernelGGL(kf, dim3(grid), dim3(256), lds, s, a);                               \
         HIPCHK(hipGetLastError());                                                              \
         if (a.filter) {                                                                         \
             hipLaunchKernelGGL(k_det_mod<NM>, dim3(512), dim3(256), 0, s, a);                   \

@@ -66,6 +66,20 @@ def kappa_dependent(rng, h, scale=1):
     return kap
 
 
+def kappa_sparse(rng, h, density=0.08, amp=4):
+    """Symmetric kappa with about `density` of its entries nonzero (the
+    device's sparse form of kappa applies from h11 > 25)."""
+    kap = np.zeros((h, h, h), dtype=object)
+    for a in range(h):
+        for b in range(a, h):
+            for c in range(b, h):
+                if rng.random() < density:
+                    v = int(rng.integers(1, amp + 1)) * int(rng.choice([-1, 1]))
+                    for x, y, z in {(a, b, c), (a, c, b), (b, a, c), (b, c, a), (c, a, b), (c, b, a)}:
+                        kap[x, y, z] = v
+    return kap
+
+
 def as_i64(kap):
     return np.array(kap.tolist(), dtype=np.int64)
 
@@ -118,6 +132,17 @@ def cases(seed):
     kap = kappa_dense(rng, h, amp=1, scale=1 << 61)
     M = np.array([[4, 4, 4, 4, 4, 4], [0, 1, 0, 0, 0, 0], [8, -8, 8, -8, 8, -8]])
     out.append(("overflow h=6", [kap], M, None))
+    # sparse kappa beyond h11 = 25 (the sparse form of kappa), with a
+    # rank-deficient member (a column of M-independent zeros)
+    for h in (26, 30, 41):
+        kap = kappa_sparse(rng, h)
+        M = rng.integers(-3, 4, size=(6, h))
+        out.append((f"sparse h={h}", [kap], M, None))
+    kap = kappa_sparse(rng, 30, density=0.05)
+    kap[:, 5, :] = 0
+    kap[5, :, :] = 0
+    kap[:, :, 5] = 0
+    out.append(("sparse singular h=30", [kap], rng.integers(-3, 4, size=(4, 30)), None))
     # geometries of different h11 in one call
     kaps = [kappa_dense(rng, 5), kappa_low_rank(rng, 20), kappa_dependent(rng, 12)]
     geo = rng.integers(0, 3, size=24).astype(np.int32)
@@ -175,6 +200,23 @@ def test_host_filter_matches_oracle():
 @pytest.mark.gpu
 def test_gpu_filter_matches_oracle(need_gpu):
     assert _check(0) > 100
+
+
+@pytest.mark.gpu
+def test_gpu_filter_matches_host_on_h30_points(need_gpu):
+    """The sparse form of kappa on real lattice points: G(30, 300, 6) (14 %
+    of kappa nonzero), kernel against the host code."""
+    from pfvs_amd.instances import config_large_tree, geoms_for_qmax
+    inst, ps, qmax = config_large_tree(n=24)
+    geoms, pgeo = geoms_for_qmax(inst, qmax)
+    pf = np.hstack([np.zeros((len(ps), 1), np.int64), ps])
+    r = gpu.run(geoms, pf, pgeo, mode="batched", filter=False)
+    M = r["M"][:, :30][:30000]
+    assert len(M) > 1000
+    kap = inst.data.kappa_cob
+    dev = gpu.det_filter([kap], M, device=0)
+    host = gpu.det_filter([kap], M, device=-1)
+    assert (dev == host).all()
 
 
 @pytest.mark.gpu
