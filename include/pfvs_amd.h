@@ -97,6 +97,8 @@ typedef struct {
     int sort;             /* 1: group by p and sort by (M, Kn) */
     int keep_points;      /* 0: return counters only (no point arrays) */
     int verbose;
+    int assemble;         /* with filter and sort: also the finished PFVs (K, M) of every
+                             p-vector the library can finish (see pfa_run_output.pdone) */
 } pfa_run_input;
 
 /* stats[]: see PFA_ST_* */
@@ -126,6 +128,15 @@ typedef struct {
     int8_t *nonsing;      /* n: 1 det != 0, 0 det == 0, 2 undecided */
     double stats[PFA_ST_N];
     char err[256];
+    /* assemble: the PFVs of coniZpM's post-processing (pfvs/coni.py:313-412)
+     * for every p-vector with pdone[i] = 1 -- rows (n_pfv x pfg_max_h11(),
+     * zero-padded) in the reference's order: by p-vector, then g, then the
+     * points in (M, Kn) order, then K0.  pdone[i] = 0: the caller finishes
+     * p-vector i (a point left undecided, pstat != 0, or a value beyond
+     * int64). */
+    int64_t n_pfv;
+    int64_t *Ks, *Ms, *pfv_pidx;
+    int8_t *pdone;        /* n_p */
 } pfa_run_output;
 
 int pfa_coni_run(const pfa_run_input *in, pfa_run_output *out);

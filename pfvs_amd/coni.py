@@ -266,13 +266,13 @@ def coniZpM(data, ps, Q=None, M0min=13, ellipsoid_dilation=1, use_gcd_lattice=Fa
     threads = 0 if n_jobs in (-1, None) else max(1, int(n_jobs))
     try:
         r = gpu.run([geo], p_full, np.zeros(len(ps), np.int32), mode=mode, filter=True,
-                    sort=True, host_threads=threads)
+                    sort=True, host_threads=threads, assemble=True)
     except RuntimeError as e:
         if device == "gpu" or not use_gpu:
             raise
         warnings.warn(f"GPU path failed ({e}); using the CPU", RuntimeWarning, stacklevel=2)
         r = gpu.run([geo], p_full, np.zeros(len(ps), np.int32), mode="host", filter=True,
-                    sort=True, host_threads=threads)
+                    sort=True, host_threads=threads, assemble=True)
     Ks, Ms = assemble(r, ps, p_full, kappa, Mbasis, Q, np.full(len(ps), qmax), M0min,
                       max_N_pfvs, verbosity)
     if return_formal_pfvs:
@@ -297,8 +297,19 @@ def assemble(r, ps, p_full, kappa, Mbasis, Q, qmax, M0min, max_N_pfvs=1_000_000_
     if verbosity >= 1:
         print(f"found {len(r['q'])} lattice points for {len(ps)} p-vectors "
               f"({int(np.sum(pstat == 1))} handled by the exact host path)...")
-    pieces = [_pfvs_from_points(r["M"][:, :h].T, r["Kn"][:, :h].T, r["q"], r["pidx"], kappa, h,
-                                Q, M0min, nonsing=r["nonsing"], verbosity=verbosity)]
+    if "pdone" in r:
+        # the library finished every p-vector with pdone; the others here
+        done = np.asarray(r["pdone"]).astype(bool)
+        rest = ~done[r["pidx"]] if len(r["pidx"]) else np.zeros(0, bool)
+        pf = r["pfv"]
+        pieces = [(pf["K"][:, :h], pf["M"][:, :h], pf["pidx"])]
+        if rest.any():
+            pieces.append(_pfvs_from_points(r["M"][rest, :h].T, r["Kn"][rest, :h].T, r["q"][rest],
+                                            r["pidx"][rest], kappa, h, Q, M0min,
+                                            nonsing=r["nonsing"][rest], verbosity=verbosity))
+    else:
+        pieces = [_pfvs_from_points(r["M"][:, :h].T, r["Kn"][:, :h].T, r["q"], r["pidx"], kappa,
+                                    h, Q, M0min, nonsing=r["nonsing"], verbosity=verbosity)]
     for ip in np.flatnonzero(pstat == 1):
         Mv, Kv, qv = _exact_host_path(kappa, Mbasis, p_full[ip], Q, int(qmax[ip]), M0min,
                                       max_N_pfvs)

@@ -29,6 +29,7 @@
 #include <mutex>
 #include <numeric>
 #include <thread>
+#include <string>
 #include <vector>
 
 #include "pfa_box.h"
@@ -1914,6 +1915,133 @@ static void filter_run(DevCtx *C, hipStream_t s, const pfa_run_input *gin,
 #undef PFA_FILTER_LAUNCH
 }
 
+// coniZpM's post-processing (pfvs/coni.py:313-412) for the points out
+// holds (sorted canonically): per p-vector, for its kept (n_exp > 0)
+// nonsingular points with Kgcd = gcd(Kn[1:]) (0 -> 1), raw = (q + M0 Kn0)
+// / Kgcd and g = 1 .. (Q Kgcd - 1) / q, every K0 from
+// ceil((g raw - Q) / M0) to min(floor((g raw - Q) / M0), ceil(Kn0 g / Kgcd) - 1)
+// gives K = (K0, g Kn[1:] / Kgcd) with the point's M; rows by g, then the
+// points in their order, then K0.  The tadpole -K.M <= Q is checked (an
+// error, as coni.py:374-386 raises).  A p-vector with an undecided point, a
+// status, or a value beyond int64 is left to the caller (pdone 0).
+// Returns 0, or 1 with a message.
+static int assemble_pfvs(const pfa_run_input *in, pfa_run_output *out, int nth, char *errbuf)
+{
+    const int MH = PFA_MAX_H11;
+    const int64_t n_p = in->n_p, n = out->n;
+    out->pdone = (int8_t *)calloc((size_t)std::max<int64_t>(n_p, 1), 1);
+    std::vector<int64_t> pb(n_p + 1, 0), rows(n_p, -1), base(n_p, 0);
+    for (int64_t t = 0; t < n; ++t) pb[out->pidx[t] + 1]++;
+    for (int64_t i = 0; i < n_p; ++i) pb[i + 1] += pb[i];
+    std::mutex emu;
+    std::string emsg;
+    // rows of p-vector ip (-1: left to the caller); writes them from row
+    // `at` when write is set
+    auto one = [&](int64_t ip, bool write, int64_t at) -> int64_t {
+        if (out->pstat[ip] != 0) return -1;
+        const int geo = in->pgeo[ip], h = in->h[geo];
+        const pfa_i128 Q = in->Q[geo];
+        struct Pt { int64_t t; pfa_i128 kg, raw; int64_t gmax; };
+        std::vector<Pt> pts;
+        int64_t G = 0;
+        for (int64_t t = pb[ip]; t < pb[ip + 1]; ++t) {
+            if (out->nexp[t] <= 0) continue;
+            if (out->nonsing[t] == 2) return -1;
+            if (out->nonsing[t] != 1) continue;
+            const int64_t *Kn = out->Kn + t * MH, *M = out->M + t * MH;
+            uint64_t kg = 0;
+            for (int i = 1; i < h; ++i)
+                kg = pfa_box_gcd(kg, (uint64_t)(Kn[i] < 0 ? -(pfa_i128)Kn[i] : (pfa_i128)Kn[i]));
+            if (kg == 0) kg = 1;
+            const pfa_i128 q = out->q[t], QK = Q * (pfa_i128)kg, M0 = M[0];
+            if (!(q < QK) || q <= 0 || M0 <= 0) return -1;
+            const pfa_i128 num = q + M0 * (pfa_i128)Kn[0];
+            if (num % (pfa_i128)kg != 0) return -1;
+            const pfa_i128 gm = (QK - 1) / q;
+            if (gm > 1000000) return -1;
+            pts.push_back(Pt{t, (pfa_i128)kg, num / (pfa_i128)kg, (int64_t)gm});
+            G = std::max<int64_t>(G, (int64_t)gm);
+        }
+        int64_t cnt = 0;
+        std::vector<int64_t> per(pts.size(), 0);
+        for (int64_t g = 1; g <= G; ++g)
+            for (size_t k = 0; k < pts.size(); ++k) {
+                const Pt &P = pts[k];
+                if (g > P.gmax) continue;
+                const int64_t *Kn = out->Kn + P.t * MH, *M = out->M + P.t * MH;
+                const pfa_i128 M0 = M[0], a = (pfa_i128)g * P.raw - Q;
+                const pfa_i128 lo = pfa_cdiv_pos(a, M0);
+                pfa_i128 up = pfa_fdiv128(a, M0);
+                const pfa_i128 cap = pfa_cdiv_pos((pfa_i128)Kn[0] * g, P.kg) - 1;
+                if (cap < up) up = cap;
+                if (up < lo) continue;
+                if (!pfa_fits64(lo) || !pfa_fits64(up)) return -1;
+                for (pfa_i128 K0 = lo; K0 <= up; ++K0) {
+                    pfa_i128 tad = -(K0 * (pfa_i128)M[0]);
+                    int64_t *Ko = write ? out->Ks + (at + cnt) * MH : nullptr;
+                    if (Ko) Ko[0] = (int64_t)K0;
+                    for (int i = 1; i < h; ++i) {
+                        const pfa_i128 Ki = (pfa_i128)g * ((pfa_i128)Kn[i] / P.kg);
+                        if (!pfa_fits64(Ki)) return -1;
+                        tad -= Ki * (pfa_i128)M[i];
+                        if (Ko) Ko[i] = (int64_t)Ki;
+                    }
+                    if (tad > Q) {
+                        std::lock_guard<std::mutex> lk(emu);
+                        if (emsg.empty()) {
+                            char b[256];
+                            snprintf(b, sizeof b, "Tadpole violation: -dot(K,M)=%lld > Q=%lld",
+                                     (long long)tad, (long long)Q);
+                            emsg = b;
+                        }
+                        return -1;
+                    }
+                    if (write) {
+                        int64_t *Mo = out->Ms + (at + cnt) * MH;
+                        for (int i = 0; i < MH; ++i) Mo[i] = i < h ? M[i] : 0;
+                        for (int i = h; i < MH; ++i) out->Ks[(at + cnt) * MH + i] = 0;
+                        out->pfv_pidx[at + cnt] = ip;
+                    }
+                    ++cnt;
+                    ++per[k];
+                }
+            }
+        // the device's expansion count of every point must agree
+        for (size_t k = 0; k < pts.size(); ++k)
+            if (per[k] != out->nexp[pts[k].t]) {
+                std::lock_guard<std::mutex> lk(emu);
+                if (emsg.empty()) emsg = "expansion count differs from the filter's";
+                return -1;
+            }
+        return cnt;
+    };
+    parallel_chunks(n_p, 64, nth, [&](int64_t lo, int64_t hi) {
+        for (int64_t ip = lo; ip < hi; ++ip) rows[ip] = one(ip, false, 0);
+    });
+    if (!emsg.empty()) {
+        snprintf(errbuf, 256, "%s", emsg.c_str());
+        return 1;
+    }
+    int64_t tot = 0;
+    for (int64_t ip = 0; ip < n_p; ++ip) {
+        base[ip] = tot;
+        if (rows[ip] >= 0) tot += rows[ip];
+    }
+    const size_t nb = (size_t)std::max<int64_t>(tot, 1);
+    out->n_pfv = tot;
+    out->Ks = (int64_t *)malloc(8 * nb * MH);
+    out->Ms = (int64_t *)malloc(8 * nb * MH);
+    out->pfv_pidx = (int64_t *)malloc(8 * nb);
+    parallel_chunks(n_p, 64, nth, [&](int64_t lo, int64_t hi) {
+        for (int64_t ip = lo; ip < hi; ++ip)
+            if (rows[ip] >= 0) {
+                one(ip, true, base[ip]);
+                out->pdone[ip] = 1;
+            }
+    });
+    return 0;
+}
+
 // Final assembly shared by the device and host runs: statuses, sort, copy.
 static int finish_run(const pfa_run_input *in, pfa_run_output *out, int64_t npts, int HS,
                       const int32_t *pidx32, const int64_t *q, const int64_t *M, const int64_t *Kn,
@@ -1980,10 +2108,12 @@ static int finish_run(const pfa_run_input *in, pfa_run_output *out, int64_t npts
     out->stats[PFA_ST_NONSING] = (double)nns;
     out->stats[PFA_ST_UNDECIDED] = (double)und;
     out->stats[PFA_ST_HOST_P] = (double)nhost;
+    int rc = 0;
+    if (in->assemble && in->sort && in->filter && nexp && nonsing)
+        rc = assemble_pfvs(in, out, nth, errbuf);
     out->stats[PFA_ST_MS_FINISH] =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    (void)errbuf;
-    return 0;
+    return rc;
 }
 
 // --------------------------------------------------------- device run
@@ -2487,9 +2617,12 @@ extern "C" void pfa_run_output_free(pfa_run_output *out)
 {
     free(out->M); free(out->Kn); free(out->q); free(out->pidx);
     free(out->pstat); free(out->pcode); free(out->nexp); free(out->nonsing);
+    free(out->Ks); free(out->Ms); free(out->pfv_pidx); free(out->pdone);
     out->M = out->Kn = out->q = out->pidx = out->nexp = nullptr;
-    out->pstat = nullptr; out->nonsing = nullptr; out->pcode = nullptr;
+    out->Ks = out->Ms = out->pfv_pidx = nullptr;
+    out->pstat = nullptr; out->nonsing = nullptr; out->pcode = nullptr; out->pdone = nullptr;
     out->n = 0;
+    out->n_pfv = 0;
 }
 
 // ----------------------------------------------- reference-compatible ABI

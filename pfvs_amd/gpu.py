@@ -59,14 +59,17 @@ class _RunInput(ctypes.Structure):
                 ("waves_per_cu", ctypes.c_int), ("budget", ctypes.c_int),
                 ("stack_cap", ctypes.c_int), ("host_threads", ctypes.c_int),
                 ("filter", ctypes.c_int), ("sort", ctypes.c_int),
-                ("keep_points", ctypes.c_int), ("verbose", ctypes.c_int)]
+                ("keep_points", ctypes.c_int), ("verbose", ctypes.c_int),
+                ("assemble", ctypes.c_int)]
 
 
 class _RunOutput(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("M", _i64p), ("Kn", _i64p),
                 ("q", _i64p), ("pidx", _i64p), ("pstat", _i8p), ("pcode", _i32p),
                 ("nexp", _i64p), ("nonsing", _i8p),
-                ("stats", ctypes.c_double * len(STAT_NAMES)), ("err", ctypes.c_char * 256)]
+                ("stats", ctypes.c_double * len(STAT_NAMES)), ("err", ctypes.c_char * 256),
+                ("n_pfv", ctypes.c_int64), ("Ks", _i64p), ("Ms", _i64p),
+                ("pfv_pidx", _i64p), ("pdone", _i8p)]
 
 
 class _KernelInput(ctypes.Structure):
@@ -346,10 +349,12 @@ class _Malloced:
 
 
 def run(geoms, ps, pgeo, mode="batched", device=0, waves_per_cu=0, budget=0, stack_cap=0,
-        host_threads=0, filter=True, sort=True, keep_points=True, verbose=False):
+        host_threads=0, filter=True, sort=True, keep_points=True, verbose=False, assemble=False):
     """This library's run call.  Returns a dict with M, Kn (n x max_h11()),
     q, pidx, pstat, pcode, nexp, nonsing (per point) and `stats` (counters
-    and stage timings in ms)."""
+    and stage timings in ms).  With assemble (and filter, sort, keep_points)
+    also `pfv` = {K, M (n_pfv x max_h11()), pidx}: coniZpM's PFVs of every
+    p-vector with pdone[i] == 1, in the reference's order."""
     lib = require()
     k = _pack(geoms, ps, pgeo)
     MH, n_p = k["MH"], k["n_p"]
@@ -357,7 +362,8 @@ def run(geoms, ps, pgeo, mode="batched", device=0, waves_per_cu=0, budget=0, sta
                     _p64(k["linmin"]), _p64(k["kap"]), _p64(k["Mb"]), n_p, _p64(k["P"]),
                     k["pgeo"].ctypes.data_as(_i32p), int(device), _MODES[mode],
                     int(waves_per_cu), int(budget), int(stack_cap), int(host_threads),
-                    int(bool(filter)), int(bool(sort)), int(bool(keep_points)), int(bool(verbose)))
+                    int(bool(filter)), int(bool(sort)), int(bool(keep_points)), int(bool(verbose)),
+                    int(bool(assemble)))
     out = _RunOutput()
     rc = lib.pfa_coni_run(ctypes.byref(inp), ctypes.byref(out))
     if rc != 0:
@@ -372,15 +378,16 @@ def run(geoms, ps, pgeo, mode="batched", device=0, waves_per_cu=0, budget=0, sta
                 return np.zeros(shape, dtype=dt)
             return np.ctypeslib.as_array(ptr, shape=shape).copy()
 
-        def own(name):
-            """The library's malloc'd (n, MH) buffer as an array, without a
+        def own(name, rows=None):
+            """The library's malloc'd (rows, MH) buffer as an array, without a
             copy; freed with the last array over it."""
+            rows = n if rows is None else rows
             ptr = getattr(out, name)
-            if n == 0 or not ptr:
-                return np.zeros((n, MH), dtype=np.int64)
-            a = np.asarray(_Malloced(ctypes.addressof(ptr.contents), 8 * n * MH))
+            if rows == 0 or not ptr:
+                return np.zeros((rows, MH), dtype=np.int64)
+            a = np.asarray(_Malloced(ctypes.addressof(ptr.contents), 8 * rows * MH))
             setattr(out, name, None)
-            return a.view(np.int64).reshape(n, MH)
+            return a.view(np.int64).reshape(rows, MH)
         res = dict(M=own("M"), Kn=own("Kn"), q=arr(out.q, (n,)),
                    pidx=arr(out.pidx, (n,)), nexp=arr(out.nexp, (n,)),
                    nonsing=arr(out.nonsing, (n,), np.int8))
@@ -389,6 +396,13 @@ def run(geoms, ps, pgeo, mode="batched", device=0, waves_per_cu=0, budget=0, sta
         res["pcode"] = np.ctypeslib.as_array(out.pcode, shape=(n_p,)).copy() if n_p \
             else np.zeros(0, np.int32)
         res["stats"] = {name: float(out.stats[i]) for i, name in enumerate(STAT_NAMES)}
+        if assemble and out.pdone:
+            npf = int(out.n_pfv)
+            res["pfv"] = dict(K=own("Ks", npf), M=own("Ms", npf),
+                              pidx=np.ctypeslib.as_array(out.pfv_pidx, shape=(npf,)).copy()
+                              if npf else np.zeros(0, np.int64))
+            res["pdone"] = np.ctypeslib.as_array(out.pdone, shape=(n_p,)).copy() if n_p \
+                else np.zeros(0, np.int8)
     finally:
         lib.pfa_run_output_free(ctypes.byref(out))
     return res

@@ -58,6 +58,7 @@ class Job:
         self.result = None
         self.points = None
         self.points_sum = None
+        self.pdone = None                 # per p-vector: assembled by the library
         self.error = None
         self.done = threading.Event()
 
@@ -166,7 +167,7 @@ class Server:
     # ------------------------------------------------------------- scheduler
     def _run(self, geoms, ps, pgeo):
         return gpu.run(geoms, ps, pgeo, mode="batched", device=self.device, filter=True,
-                       sort=True, **self.params)
+                       sort=True, assemble=True, **self.params)
 
     def _admit(self):
         """The (job, lo, hi) pieces of the next iteration, in arrival order."""
@@ -245,13 +246,22 @@ class Server:
                                 sorted({job.h for job, _, _ in pieces})))
         pidx = np.asarray(r["pidx"])
         self.points_per_p = 0.5 * self.points_per_p + 0.5 * max(len(pidx) / n, 1.0)
+        lib = "pdone" in r                  # the library assembled what it could
+        fp = np.asarray(r["pfv"]["pidx"]) if lib else None
         done = []
         with self._cv:
             for job, lo, b0, k in owner:
                 a, b = np.searchsorted(pidx, [b0, b0 + k])
-                job.rows.append((lo, dict(M=r["M"][a:b], Kn=r["Kn"][a:b], q=r["q"][a:b],
-                                          pidx=pidx[a:b] - b0 + lo,
-                                          nonsing=r["nonsing"][a:b])))
+                piece = dict(M=r["M"][a:b], Kn=r["Kn"][a:b], q=r["q"][a:b],
+                             pidx=pidx[a:b] - b0 + lo, nonsing=r["nonsing"][a:b])
+                if lib:
+                    fa, fb = np.searchsorted(fp, [b0, b0 + k])
+                    piece.update(pfK=r["pfv"]["K"][fa:fb], pfM=r["pfv"]["M"][fa:fb],
+                                 pfp=fp[fa:fb] - b0 + lo)
+                    if job.pdone is None:
+                        job.pdone = np.zeros(len(job.ps), np.int8)
+                    job.pdone[lo:lo + k] = r["pdone"][b0:b0 + k]
+                job.rows.append((lo, piece))
                 job.pstat[lo:lo + k] = r["pstat"][b0:b0 + k]
                 job.returned += k
                 if job.returned == len(job.ps) and job.error is None:
@@ -268,6 +278,11 @@ class Server:
             for k in ("M", "Kn"):
                 rr[k] = np.vstack([p[k] for p in rows])
             rr["pstat"] = job.pstat
+            if job.pdone is not None:
+                rr["pdone"] = job.pdone
+                rr["pfv"] = dict(K=np.vstack([p["pfK"] for p in rows]),
+                                 M=np.vstack([p["pfM"] for p in rows]),
+                                 pidx=np.concatenate([p["pfp"] for p in rows]))
             job.result = coni.assemble(rr, job.ps, job.p_full, job.kappa, job.Mbasis, job.Q,
                                        job.qmax, job.M0min, self.max_N)
         except Exception as e:

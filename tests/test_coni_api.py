@@ -102,10 +102,67 @@ def test_gpu_matches_cpu(need_gpu, g, dil):
 
 
 @pytest.mark.gpu
-def test_gpu_G_matches_oracle_postprocessing(need_gpu):
+@pytest.mark.parametrize("n,coord,seed,dil,n_pfv", [(64, 12, 1003, 4, 0), (8, 6, 77, 150, 1)])
+def test_gpu_G_matches_oracle_postprocessing(need_gpu, n, coord, seed, dil, n_pfv):
+    """G(10, 150, 3): a batch with no PFVs, and one with a few hundred."""
     inst = make_G(10, 150, 3)
-    ps = random_pvecs(inst, 64, 12, seed=1003)
-    Kg, Mg = coniZpM(inst.data, ps, Q=inst.Q, ellipsoid_dilation=4, device="gpu")
-    Ko, Mo = oracle_pfvs(inst.data, ps, inst.Q, 4)
+    ps = random_pvecs(inst, 64, coord, seed=seed)[:n]
+    Kg, Mg = coniZpM(inst.data, ps, Q=inst.Q, ellipsoid_dilation=dil, device="gpu")
+    print(f"{len(Kg)} PFVs", flush=True)
+    assert (len(Kg) > 0) == bool(n_pfv)
+    Ko, Mo = oracle_pfvs(inst.data, ps, inst.Q, dil)
     np.testing.assert_array_equal(Kg, Ko)
     np.testing.assert_array_equal(Mg, Mo)
+
+
+def _both_assemblies(data, ps, Q, dil, mode):
+    """coni.assemble over one pipeline result with the library's native
+    post-processing and with the numpy form only; also the share of
+    p-vectors the library finished."""
+    from pfvs_amd import coni, gpu
+    import math
+    p_full = np.hstack([np.zeros((len(ps), 1), np.int64), np.asarray(ps, np.int64)])
+    geo = dict(kappa=data.kappa_cob, Mbasis=data.M_lattice(), Q=Q, dilation=dil, M0min=13)
+    qmax = np.full(len(ps), int(math.floor(dil * Q + 1e-7)))
+    out = []
+    for lib in (True, False):
+        r = gpu.run([geo], p_full, np.zeros(len(ps), np.int32), mode=mode, filter=True,
+                    sort=True, assemble=lib)
+        assert ("pdone" in r) == lib
+        if lib:
+            done = r["pdone"].astype(bool)
+            assert not done[r["pstat"] != 0].any()
+            frac = done.mean()
+            assert np.all(np.diff(r["pfv"]["pidx"]) >= 0)
+        out.append(coni.assemble(r, ps, p_full, data.kappa_cob, data.M_lattice(), Q, qmax, 13))
+    (Kl, Ml), (Kn, Mn) = out
+    np.testing.assert_array_equal(Kl, Kn)
+    np.testing.assert_array_equal(Ml, Mn)
+    return frac, len(Kl)
+
+
+@pytest.mark.parametrize("g", SMALL[:6], ids=lambda g: f"h11={g['h11']}-poly{g['polyID']}")
+def test_library_assembly_matches_numpy_cpu(g):
+    data = cydata_of(g)
+    ps = box_enum(g["B"], data.H_cob, 1, True)
+    if len(ps) == 0:
+        pytest.skip("no p-vectors")
+    frac, n = _both_assemblies(data, ps, data.h11 + data.h21 + 4, g["D"], "host")
+    assert frac == 1.0 or n == 0
+
+
+def test_library_assembly_matches_numpy_G_cpu():
+    inst = make_G(10, 150, 3)
+    ps = random_pvecs(inst, 24, 10, seed=77)
+    frac, n = _both_assemblies(inst.data, ps, inst.Q, 150, "host")
+    assert n > 0 and frac > 0.5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dil", [150, 1000])
+def test_library_assembly_matches_numpy_gpu(need_gpu, dil):
+    inst = make_G(10, 150, 3)
+    ps = random_pvecs(inst, 64, 6, seed=77)
+    print(f"G(10,150,3) 64 p-vectors dil {dil}", flush=True)
+    frac, n = _both_assemblies(inst.data, ps, inst.Q, dil, "batched")
+    assert n > 0 and frac > 0.5
