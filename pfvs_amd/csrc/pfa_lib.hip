@@ -835,7 +835,11 @@ PFA_HD static inline int kappa_M(const int64_t *kappa, int h, const int64_t *M, 
 //   2. fraction-free elimination (Bareiss) in double, every product and
 //      difference checked below 2^53 so each value is the exact integer
 //      minor: a column without a pivot among the unused rows proves
-//      det = 0, n pivots prove det != 0;
+//      det = 0, n pivots prove det != 0.  The exact division by the
+//      previous pivot is a product with its rounded reciprocal, rounded to
+//      the nearest integer: y = x fl(1/prev) is within |q| 2^-52 (1 + 2^-52)
+//      of the integer quotient q, so rint(y) = q whenever |y| < 2^50
+//      (checked with the products);
 //   3. points whose values reach 2^53 go on a second list (with their M)
 //      for k_det_mod: N in int128 from the int64 kappa (an entry outside
 //      int64 leaves the point undecided, 2), then det modulo 31-bit primes
@@ -1003,56 +1007,6 @@ __global__ __launch_bounds__(64) void k_post(FiltArgs a)
     }
 }
 
-// w[s] = N[s][j] = sum_k kappa[s+1][j+1][k] M[k] for the column j of a
-// lane (kt: that column's kT entries, LDS or global); M[k] comes from lane
-// src0 + k.  Every lane runs every shuffle.  det_build_i24: exact when
-// |kappa|, |M| < 2^23 and every partial sum fits int32; det_build_f64:
-// exact when every partial sum stays below 2^53.
-template <int NMAX>
-__device__ __forceinline__ void det_build_i24(double *w, const int32_t *kt, int32_t mi, int src0,
-                                              int hl, int n, bool col)
-{
-    int32_t acc[NMAX];
-#pragma unroll
-    for (int s = 0; s < NMAX; ++s) acc[s] = 0;
-    for (int k = 0; k < hl; ++k) {
-        const int32_t mk = __shfl(mi, src0 + k);
-        if (!col || k > n) continue;
-        const int32_t *kk = kt + (int64_t)k * n * n;
-#pragma unroll
-        for (int s = 0; s < NMAX; ++s)
-            if (s < n) acc[s] += __mul24(kk[s * n], mk);
-    }
-#pragma unroll
-    for (int s = 0; s < NMAX; ++s) w[s] = (double)acc[s];
-}
-
-// The same from LDS in the zero-padded layout lds[(k NMAX + s) NMAX + j]
-// (entries with k > n, s >= n or j >= n are 0; every offset is a
-// compile-time constant from the lane's base j): all shuffles first, then
-// the fully unrolled multiply-adds with no per-entry predicate (padding
-// adds zeros).
-template <int NMAX>
-__device__ __forceinline__ void det_build_i24_lds(double *w, const int32_t *ktj, int32_t mi,
-                                                  int src0, int n)
-{
-    int32_t mk[NMAX + 1];
-#pragma unroll
-    for (int k = 0; k <= NMAX; ++k) {
-        const int32_t v = __shfl(mi, src0 + k);
-        mk[k] = k <= n ? v : 0;
-    }
-    int32_t acc[NMAX];
-#pragma unroll
-    for (int s = 0; s < NMAX; ++s) acc[s] = 0;
-#pragma unroll
-    for (int k = 0; k <= NMAX; ++k)
-#pragma unroll
-        for (int s = 0; s < NMAX; ++s) acc[s] += __mul24(ktj[(k * NMAX + s) * NMAX], mk[k]);
-#pragma unroll
-    for (int s = 0; s < NMAX; ++s) w[s] = (double)acc[s];
-}
-
 // LDS writes of any lane of the wavefront before reads of any other
 // (wavefront-scope release, wave barrier, acquire)
 __device__ __forceinline__ void wave_lds_sync()
@@ -1062,48 +1016,51 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// N column j from the sparse form of kappa staged in LDS (ranges R, n^2 + 1,
-// then the packed terms), the group's M in sM (sM[k] = M_k): only the
-// nonzero kappa terms, same int32 sums as the dense forms (|kappa|, |M|
-// < 2^23 and every partial sum below 2^31).
+// Entry N[s][c] = sum_{k <= n} kappa[s+1][c+1][k] M[k] of column c at the
+// lane's row s (kT[(k n + s) n + c]), in four forms, each exact under the
+// condition k_filter checks for the group:
+//  - col_i24_lds: the zero-padded LDS layout lds[(c NMAX + s)(NMAX + 1) + k]
+//    (rowp = the row's slice of column c), M as int32 in registers, 24-bit
+//    multiply-adds (|kappa|, |M| < 2^23 and every partial sum below 2^31);
+//  - col_sparse_lds: the sparse form staged in LDS (column c, row s: the
+//    packed terms of R[c n + s] .. R[c n + s + 1]), the same int32 sums;
+//  - col_i24: from kT in global memory, the same int32 sums;
+//  - col_f64: from kT in global memory in double (every product and
+//    partial sum an integer below 2^53).
+// gM / gMd: the group's M (M[k] at gM[k]) in LDS.
 template <int NMAX>
-__device__ __forceinline__ void det_build_sparse_lds(double *w, const int32_t *sp, const int32_t *sM,
-                                                     int n, int j, bool col)
+__device__ __forceinline__ int32_t col_i24_lds(const int32_t *rowp, const int32_t *mk)
 {
-    const int32_t *R = sp + (col ? j * n : 0), *KV = sp + n * n + 1;
-    int32_t acc[NMAX];
+    int32_t acc = 0;
 #pragma unroll
-    for (int s = 0; s < NMAX; ++s) acc[s] = 0;
-    if (col) {
-        int b = R[0];
-#pragma unroll
-        for (int s = 0; s < NMAX; ++s) {
-            if (s < n) {
-                const int e = R[s + 1];
-                for (int t = b; t < e; ++t) {
-                    const int32_t x = KV[t];
-                    acc[s] += (x << 8 >> 8) * sM[(x >> 24) & 63];
-                }
-                b = e;
-            }
-        }
-    }
-#pragma unroll
-    for (int s = 0; s < NMAX; ++s) w[s] = (double)acc[s];
+    for (int k = 0; k <= NMAX; ++k) acc += __mul24(rowp[k], mk[k]);
+    return acc;
 }
 
-template <int NMAX>
-__device__ __forceinline__ void det_build_f64(double *w, const int32_t *kt, double md, int src0,
-                                              int hl, int n, bool col)
+__device__ __forceinline__ int32_t col_sparse_lds(const int32_t *sp, const int32_t *gM, int n, int s,
+                                                  int c)
 {
-    for (int k = 0; k < hl; ++k) {
-        const double mk = __shfl(md, src0 + k);
-        if (!col || k > n) continue;
-        const int32_t *kk = kt + (int64_t)k * n * n;
-#pragma unroll
-        for (int s = 0; s < NMAX; ++s)
-            if (s < n) w[s] = fma((double)kk[s * n], mk, w[s]);
+    const int32_t *R = sp + c * n + s, *KV = sp + n * n + 1;
+    int32_t acc = 0;
+    for (int t = R[0]; t < R[1]; ++t) {
+        const int32_t x = KV[t];
+        acc += (x << 8 >> 8) * gM[(x >> 24) & 63];
     }
+    return acc;
+}
+
+__device__ __forceinline__ int32_t col_i24(const int32_t *kt, const int32_t *gM, int n, int s, int c)
+{
+    int32_t acc = 0;
+    for (int k = 0; k <= n; ++k) acc += __mul24(kt[((int64_t)k * n + s) * n + c], gM[k]);
+    return acc;
+}
+
+__device__ __forceinline__ double col_f64(const int32_t *kt, const double *gMd, int n, int s, int c)
+{
+    double w = 0.0;
+    for (int k = 0; k <= n; ++k) w = fma((double)kt[((int64_t)k * n + s) * n + c], gMd[k], w);
+    return w;
 }
 
 // SP: the launch has geometries with a sparse form of kappa (a separate
@@ -1111,16 +1068,28 @@ __device__ __forceinline__ void det_build_f64(double *w, const int32_t *kt, doub
 template <int NMAX, bool SP>
 __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
 {
+    // dynamic LDS: kappa (lds_floats ints, rounded up to 4), then per
+    // wavefront the groups' step tables: (pivot, 1 / previous pivot) as
+    // double2 and the pivot row, R NMAX entries each
     extern __shared__ int32_t lds_kT[];
     __shared__ int64_t lds_ofs;
     __shared__ int lds_cnt;
     __shared__ int lds_sparse;
-    __shared__ int32_t s_M[SP ? 256 : 1];          // the groups' M (sparse build)
+    __shared__ int32_t s_Mi[256];                  // the groups' M: int32 (small) ...
+    __shared__ double s_Md[256];                   // ... and double (exact)
     __shared__ int64_t s_kofs[256];                // kappa offset and bound of geometries < 256
     __shared__ double s_kmax[256];
     const DetLanes L = det_lanes(a.hl);
     const int R = L.R, i = L.j, nlmax = a.hl - 1;
     const int wv = threadIdx.x >> 6;
+    const int SW = R * NMAX;
+    double2 *s_e = (double2 *)(lds_kT + ((a.lds_floats + 3) & ~3)) + wv * SW;
+    int32_t *s_p = (int32_t *)((double2 *)(lds_kT + ((a.lds_floats + 3) & ~3)) + 4 * SW) + wv * SW;
+    {
+        const int rr = L.inlane ? L.r : 0;
+        s_e += rr * NMAX;
+        s_p += rr * NMAX;
+    }
     // the points: the list k_post made, or every slot when M is given
     const int64_t cnt = a.keepkp ? (int64_t)*a.keepcount : a.n;
     auto slot = [&](int64_t t) -> int64_t { return a.keepkp ? a.keepkp[t] : t; };
@@ -1170,14 +1139,15 @@ __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
                 lds_kT[e] = e < nr ? Rg[e] : KVg[e - nr];
             __syncthreads();
         }
-        // zero-padded copy: lds[(k NMAX + s) NMAX + j] = kT[(k n + s) n + j]
+        // zero-padded copy by column and row: lds[(c NMAX + s)(NMAX + 1) + k]
+        // = kT[(k n + s) n + c]
         const int nn = lds_cnt, cnt = (nn + 1) * nn * nn;
         if (cofs >= 0 && !sparse) {
             for (int e = threadIdx.x; e < (NMAX + 1) * NMAX * NMAX; e += blockDim.x) lds_kT[e] = 0;
             __syncthreads();
             for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
-                const int k = e / (nn * nn), r = e - k * nn * nn, s2 = r / nn, j2 = r - s2 * nn;
-                lds_kT[(k * NMAX + s2) * NMAX + j2] = a.kT[cofs + e];
+                const int k = e / (nn * nn), r = e - k * nn * nn, s2 = r / nn, c2 = r - s2 * nn;
+                lds_kT[(c2 * NMAX + s2) * (NMAX + 1) + k] = a.kT[cofs + e];
             }
         }
         __syncthreads();
@@ -1216,7 +1186,7 @@ __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
         }
         const int64_t *rec = a.recs + pi * a.L.words;
         const int n = h > 0 ? h - 1 : 0;
-        const bool col = need && i < n;
+        const bool rowl = need && i < n;          // the lane's row s = i of N
         int res = (need && n == 0) ? 1 : 0;
 
         // M_i (checked into int64 by k_post already)
@@ -1236,10 +1206,10 @@ __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
             Mi = row ? (int64_t)sm : 0;
         }
 
-        // 1. N: group bound sum|M| (a shuffle tree), then 24-bit integer or
-        // double multiply-adds (the whole wavefront takes one form)
-        // (terms capped at 2^57 so the sum cannot wrap; a capped sum fails
-        // every test below)
+        // 1. the group bound sum|M| (a shuffle tree) decides the arithmetic
+        // of the entries of N: 24-bit integer or double (the whole wavefront
+        // takes one form); terms capped at 2^57 so the sum cannot wrap (a
+        // capped sum fails every test below)
         uint64_t am = (need && i < h) ? (uint64_t)(Mi < 0 ? -Mi : Mi) : 0;
         if (am > ((uint64_t)1 << 57)) am = (uint64_t)1 << 57;
         for (int off = 1; off < a.hl; off <<= 1) {
@@ -1248,82 +1218,103 @@ __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
         }
         const double msum = (double)shfl_u64(am, L.src0);
         const double kmax = need ? kmax_of(geo) : 0.0;
-        const bool exact = !col || kmax * msum < PFA_TWO52;
-        const bool small = !col || (msum < 8388608.0 && kmax < 8388608.0 && kmax * msum < 2147483648.0);
-        double w[NMAX];
+        const bool exact = !rowl || kmax * msum < PFA_TWO52;
+        const bool small = !rowl || (msum < 8388608.0 && kmax < 8388608.0 && kmax * msum < 2147483648.0);
+        const bool exact_g = !(__ballot(rowl && !exact) & L.gm);
+        const bool all_small = !__ballot(!small);
+        // the group's M through LDS (lane i of the group holds M_i)
+        wave_lds_sync();
+        s_Mi[threadIdx.x] = (int32_t)Mi;
+        s_Md[threadIdx.x] = (double)Mi;
+        wave_lds_sync();
+        const int32_t *gM = s_Mi + (wv << 6) + L.src0;
+        const double *gMd = s_Md + (wv << 6) + L.src0;
+        // the form of the column builds: 0 dense LDS, 1 sparse LDS, 2 global
+        // int32, 3 global double
+        const bool lds = rowl && kofs_of(geo) == cofs;
+        const bool all_lds = !__ballot(rowl && !lds);
+        int form = all_small ? 2 : 3;
+        if (SP && sparse && all_lds && all_small) form = 1;
+        if (NMAX <= 24 && !sparse && cofs >= 0 && all_lds && all_small) form = 0;
+        constexpr int MKN = NMAX <= 24 ? NMAX + 1 : 1;
+        int32_t mk[MKN];
 #pragma unroll
-        for (int s = 0; s < NMAX; ++s) w[s] = 0.0;
-        {
-            const bool lds = col && kofs_of(geo) == cofs;
-            const bool all_lds = !__ballot(col && !lds);
-            const int32_t *kt = a.kT + (col ? kofs_of(geo) : 0) + i;
-            bool done = false;
-            if constexpr (SP) {
-                if (sparse && all_lds && !__ballot(!small)) {
-                    // the group's M through LDS (lane i of the group holds M_i)
-                    wave_lds_sync();
-                    s_M[threadIdx.x] = (int32_t)Mi;
-                    wave_lds_sync();
-                    det_build_sparse_lds<NMAX>(w, lds_kT, s_M + (wv << 6) + L.src0, n, i, col);
-                    done = true;
-                }
-            }
-            if constexpr (NMAX <= 24) {
-                // (larger NMAX never stage the padded slab: it exceeds 64 KB)
-                if (!done && !sparse && all_lds && !__ballot(!small)) {
-                    // lanes without a column read column 0 and keep w = 0
-                    det_build_i24_lds<NMAX>(w, lds_kT + (col ? i : 0), (int32_t)Mi, L.src0, n);
-                    if (!col) {
+        for (int k = 0; k < MKN; ++k) mk[k] = 0;
+        if constexpr (NMAX <= 24) {
+            if (form == 0) {
 #pragma unroll
-                        for (int s2 = 0; s2 < NMAX; ++s2) w[s2] = 0.0;
-                    }
-                    done = true;
-                }
+                for (int k = 0; k <= NMAX; ++k) mk[k] = k <= n ? gM[k] : 0;
             }
-            if (done) {
-            } else if (!__ballot(!small))
-                det_build_i24<NMAX>(w, kt, (int32_t)Mi, L.src0, a.hl, n, col);
-            else
-                det_build_f64<NMAX>(w, kt, (double)Mi, L.src0, a.hl, n, col);
         }
-        const bool exact_g = !(__ballot(col && !exact) & L.gm);
+        const int32_t *rowp = lds_kT + (i < NMAX ? i : NMAX - 1) * (NMAX + 1);
+        const int32_t *kt = a.kT + (rowl ? kofs_of(geo) : 0);
+        auto build = [&](int c) -> double {
+            if constexpr (NMAX <= 24) {
+                if (form == 0) return (double)col_i24_lds<NMAX>(rowp + c * NMAX * (NMAX + 1), mk);
+            }
+            if (!rowl || c >= n) return 0.0;
+            if (form == 1) return (double)col_sparse_lds(lds_kT, gM, n, i, c);
+            if (form == 2) return (double)col_i24(kt, gM, n, i, c);
+            return col_f64(kt, gMd, n, i, c);
+        };
 
-        // 2. Bareiss in double
+        // 2. Bareiss elimination in double, left-looking: column k is built
+        // at step k and takes the updates of steps t < k,
+        //   v <- (v pk_t - pc_t v[p_t]) / pk_{t-1}     (pk_{-1} = 1)
+        // on the rows that are no pivot yet (pc_t: the row's entry of the
+        // pivot column of step t), which gives every value the
+        // right-looking order gives, in particular the same integer minors.
+        // Every product and difference is checked below 2^53; the exact
+        // division is a product with the rounded reciprocal, rounded to the
+        // nearest integer: y = x fl(1/pk_{t-1}) lies within |q| 2^-52
+        // (1 + 2^-52) of the integer quotient q, so rint(y) = q whenever
+        // |y| < 2^50 (checked with the products).  The pivot of step k is
+        // the lowest row that is no pivot yet with a nonzero entry; a column
+        // without one proves det = 0 (only the columns up to the rank are
+        // built), n pivots prove det != 0.
         bool run = need && n > 0 && exact_g, trip = false;
-        uint64_t used = 0;
+        int ustep = 255;                          // the step this row was the pivot of
         double prev = 1.0;
+        double pc[NMAX];
+        // (the step loop stays rolled; the loops over t run to NMAX with
+        // wave-uniform predicates so every index of pc is a constant)
         for (int k = 0; k < nlmax; ++k) {
             if (run && k >= n) { run = false; res = 1; }
             if (!__ballot(run)) break;
-            int cand = -1;
-#pragma unroll
-            for (int s = NMAX - 1; s >= 0; --s)
-                if (s < n && !((used >> s) & 1) && w[s] != 0.0) cand = s;
-            const int srck = L.inlane ? L.src0 + k : L.lane;
-            const int p = __shfl(cand, srck);
-            bool act = run;
-            if (act && p < 0) { run = false; act = false; res = 0; }
-            double pj = 0.0;
-#pragma unroll
-            for (int s = 0; s < NMAX; ++s)
-                if (s == p) pj = w[s];
-            const double pk = __shfl(pj, srck);
+            double v = build(k);
             int okb = 1;
 #pragma unroll
-            for (int s = 0; s < NMAX; ++s) {
-                const double ik = __shfl(w[s], srck);
-                if (act && col && s < n && s != p && !((used >> s) & 1)) {
-                    const double x1 = w[s] * pk, x2 = ik * pj, x = x1 - x2;
-                    okb &= (int)(fabs(x1) < PFA_TWO53) & (int)(fabs(x2) < PFA_TWO53) &
-                           (int)(fabs(x) < PFA_TWO53);
-                    w[s] = k == 0 ? x : x / prev;
+            for (int t = 0; t < NMAX; ++t) {
+                if (t < k) {                      // wave-uniform
+                    const int pt = s_p[t];
+                    const double2 e = s_e[t];
+                    const double cp = __shfl(v, L.inlane ? L.src0 + pt : L.lane);
+                    if (run && i < n && ustep > t) {
+                        const double x1 = v * e.x, x2 = pc[t] * cp, x = x1 - x2, y = x * e.y;
+                        // |x1|, |x2|, |x| < 2^53 and |y| < 2^50 (8|y| is exact)
+                        const double m = fmax(fmax(fabs(x1), fabs(x2)), fmax(fabs(x), 8.0 * fabs(y)));
+                        okb &= (int)(m < PFA_TWO53);
+                        v = rint(y);
+                    }
                 }
             }
-            if (act && (__ballot(!okb) & L.gm)) { run = false; act = false; trip = true; }
-            if (act) {
-                used |= 1ull << p;
+            if (run && (__ballot(!okb) & L.gm)) { run = false; trip = true; }
+            const uint64_t cm = __ballot(run && i < n && ustep > k && v != 0.0) & L.gm;
+            const int pl = cm ? (int)__builtin_ctzll(cm) : L.lane;
+            const double pk = __shfl(v, pl);
+            if (run && !cm) { run = false; res = 0; }
+#pragma unroll
+            for (int t = 0; t < NMAX; ++t)
+                if (t == k) pc[t] = v;
+            if (run) {
+                if (L.lane == pl) ustep = k;
+                if (i == 0) {
+                    s_p[k] = pl - L.src0;
+                    s_e[k] = make_double2(pk, 1.0 / prev);
+                }
                 prev = pk;
             }
+            wave_lds_sync();
         }
         if (run) res = 1;
 
@@ -1886,7 +1877,8 @@ static void filter_run(DevCtx *C, hipStream_t s, const pfa_run_input *gin,
     // list length unknown on the host: size the grid by the slots
     const int64_t rounds = (a.n + 4 * R - 1) / (4 * R);
     const int grid = (int)std::min<int64_t>(rounds, 16384);
-    const size_t lds = (size_t)a.lds_floats * 4;
+    // kappa, then 4 wavefronts x R groups x NMAX steps of (double2, int32)
+    const size_t lds = (size_t)((a.lds_floats + 3) & ~3) * 4 + (size_t)4 * R * nmax * 20;
     // the sparse form where the padded dense layout cannot be staged (NMAX
     // > 24); below that the unrolled dense build is faster even at 20 %
     // density
