@@ -151,18 +151,13 @@ PFA_HD static inline void prep_one(const PfaTeam &T, char *ws_hot, char *ws_cold
         if (T.lane == 0) ri[25] = st;
         return;
     }
-    // H = HNF(ZB[1:]) in the GG region; v, w, pivots in W1..W3
+    // H = HNF(ZB[1:]) over the GG and W1 regions (free after the last
+    // LLL; the scratch hv, hw, hpiv shares the mu region)
     pfa_i128 *H = W.GG;
     int hst = pfa_hnf(T, W, W.ZB + d, d, d, H, W.hv, W.hw, W.hpiv);
-    double *U = (double *)W.W1, *X = (double *)W.W2, phi = 0.0;
     PFA_SYNC();
-    int cst = pfa_cholesky(T, W.mat, d, U, X, &phi, W.flag);
-    if (cst) {
-        PFA_SYNC();
-        if (T.lane == 0) ri[25] = cst == PFA_NOT_PD ? -60 : -70;
-        return;
-    }
-    // row levels of H (pivot columns) and whether each row fits int64
+    // row levels of H (pivot columns) and whether each row fits int64;
+    // H's rows go to the record before the factor reuses W1
     int *lvl = (int *)W.W3;             // d levels + d fit flags
     if (T.lane == 0) {
         for (int r = 0; r < d; ++r) {
@@ -177,12 +172,24 @@ PFA_HD static inline void prep_one(const PfaTeam &T, char *ws_hot, char *ws_cold
             lvl[d + r] = fit;
         }
     }
-    PFA_SYNC();
     int64_t *rU = rec + L.U, *rM = rec + L.mat, *rH = rec + L.H, *rL = rec + L.L;
+    for (int e = T.lane; e < d * d; e += T.n) rH[e] = 0;
+    PFA_SYNC();
+    for (int e = T.lane; e < d * d; e += T.n) {
+        int r = e / d, j = e % d;
+        if (lvl[r] >= 0 && lvl[d + r]) rH[lvl[r] * d + j] = (int64_t)H[r * d + j];
+    }
+    PFA_SYNC();
+    double *U = (double *)W.W1, *X = (double *)W.W2, phi = 0.0;
+    int cst = pfa_cholesky(T, W.mat, d, U, X, &phi, W.flag);
+    if (cst) {
+        PFA_SYNC();
+        if (T.lane == 0) ri[25] = cst == PFA_NOT_PD ? -60 : -70;
+        return;
+    }
     for (int e = T.lane; e < d * d; e += T.n) {
         ((double *)rU)[e] = U[e];
         rM[e] = W.mat[e];
-        rH[e] = 0;
     }
     for (int e = T.lane; e < d; e += T.n) {
         ((double *)(rec + L.Uinv))[e] = 1.0 / U[e * d + e];
@@ -204,10 +211,6 @@ PFA_HD static inline void prep_one(const PfaTeam &T, char *ws_hot, char *ws_cold
         rec[15] = mz;
     }
     PFA_SYNC();
-    for (int e = T.lane; e < d * d; e += T.n) {
-        int r = e / d, j = e % d;
-        if (lvl[r] >= 0 && lvl[d + r]) rH[lvl[r] * d + j] = (int64_t)H[r * d + j];
-    }
     uint64_t hmask = 0;
     int leafg = hst != 0;
     for (int r = 0; r < d; ++r) {
@@ -1194,16 +1197,35 @@ __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
         if (a.Min) {
             if (need && i < h) Mi = a.Min[kp * a.MH + i];
         } else {
-            // lane t < n holds c[t]; every lane forms its entry with shuffles
+            // lane t < n holds c[t]; every lane gets all of c by shuffles.
+            // M_i = Binter[i] . c in double, valid when max|Binter| sum|c| <
+            // 2^53 (every product and partial sum is then an exact integer)
+            // and |Binter| < 2^31 (the low word of each entry holds it); else
+            // again in int128 (the whole wavefront runs those shuffles)
             const int32_t cv = (need && i < n) ? a.c[kp * a.D + i] : 0;
             const int64_t *bi = rec + a.L.Bi + (int64_t)i * n;
+            const int32_t *b32 = (const int32_t *)bi;
             const bool row = need && i < h;
-            pfa_i128 sm = 0;
-            for (int t = 0; t < nlmax; ++t) {
+            const int64_t bmax = rec[14];
+            const bool r32 = row && bmax < ((int64_t)1 << 31);
+            double acc = 0.0, csum = 0.0;
+#pragma unroll 4
+            for (int t = 0; t < NMAX; ++t) {
                 const int32_t ct = __shfl(cv, L.src0 + t);
-                if (row && t < n) sm += (pfa_i128)bi[t] * ct;
+                const double cd = t < n ? (double)ct : 0.0;
+                csum += fabs(cd);
+                if (r32 && t < n) acc = fma((double)b32[2 * t], cd, acc);
             }
-            Mi = row ? (int64_t)sm : 0;
+            const bool fast = r32 && (double)bmax * csum < PFA_TWO53;
+            Mi = fast ? (int64_t)acc : 0;
+            if (__ballot(row && !fast)) {
+                pfa_i128 sm = 0;
+                for (int t = 0; t < nlmax; ++t) {
+                    const int32_t ct = __shfl(cv, L.src0 + t);
+                    if (row && !fast && t < n) sm += (pfa_i128)bi[t] * ct;
+                }
+                if (row && !fast) Mi = (int64_t)sm;
+            }
         }
 
         // 1. the group bound sum|M| (a shuffle tree) decides the arithmetic

@@ -33,13 +33,20 @@ struct PfaTeam {
 };
 
 // Working storage of one preparation.  The part the LLL and HNF loops
-// touch lives in LDS on the device ("hot"): W1 (h*h int64: the basis rows
-// being reduced), GG ((h+1)^2 int128: their exact Gram matrix, later the
-// HNF), mu (h*h double), bb, mw, vec (h), the HNF's row scratch hv, hw
-// (h int128) and pivots hpiv (h int), flag.  The part only O(h^3)
-// products touch ("cold") is per-workgroup global memory on the device:
-// Z, W2, W3 (h*h int64), Binter, ZB (h*(h-1)), mat ((h-1)^2).  On the
-// host both parts are one buffer.
+// touch lives in LDS on the device ("hot"), sized so that 16 wavefronts
+// of h11 <= 20 share a CU's LDS (10 KB each):
+//   GG  the exact Gram matrix (int128) of the rows being reduced, lower
+//       triangle ((h+1)(h+2)/2 entries, GG[pfa_gi(i, j)]);
+//   W1  h*h int64: the basis rows being reduced;
+//       H, the HNF (d*d int128), spans GG and W1 once the last LLL is done;
+//   mu  the Gram-Schmidt coefficients, strict lower triangle (mu[pfa_mi(k,
+//       j)], j < k); outside the LLL the same region holds the scratch hv,
+//       hw (h int128), hpiv (h int) of the HNF and the unimodular
+//       completion, and lll_form's Gy (h int128) before its LLL starts;
+//   bb, mw, vec (h), flag.
+// The part only O(h^3) products touch ("cold") is per-workgroup global
+// memory on the device: Z, W2, W3 (h*h int64), Binter, ZB (h*(h-1)), mat
+// ((h-1)^2).  On the host both parts are one buffer.
 struct PfaWS {
     int h, d;
     int64_t *Z, *Bi, *ZB, *mat, *W1, *W2, *W3, *vec;
@@ -50,15 +57,36 @@ struct PfaWS {
 
 #define PFA_WS_ROUND(x) (((size_t)(x) + 15) & ~(size_t)15)
 
+// index of (i, j) in a symmetric matrix stored as its lower triangle
+PFA_HD static inline int pfa_gi(int i, int j)
+{
+    return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
+}
+
+// index of (k, j), j < k, in a strict lower triangle
+PFA_HD static inline int pfa_mi(int k, int j) { return k * (k - 1) / 2 + j; }
+
+PFA_HD static inline size_t pfa_ws_gg_bytes(int h) { return PFA_WS_ROUND((size_t)(h + 1) * (h + 2) / 2 * 16); }
+
+// the mu region: the strict triangle, or the HNF scratch, or Gy
+PFA_HD static inline size_t pfa_ws_mu_bytes(int h)
+{
+    size_t w = (size_t)h * (h - 1) / 2 * 8;
+    const size_t scr = PFA_WS_ROUND((size_t)h * 16) * 2 + PFA_WS_ROUND((size_t)h * 4);
+    if (scr > w) w = scr;
+    if ((size_t)h * 16 > w) w = (size_t)h * 16;
+    return PFA_WS_ROUND(w);
+}
+
 // Bytes of the hot (LDS) and cold (global) parts for h, 16-byte aligned.
+// (GG and W1 together hold H: (h+1)(h+2) 8 + 8 h^2 >= 16 (h-1)^2.)
 PFA_HD static inline size_t pfa_ws_lds_bytes(int h)
 {
     size_t w = 0;
-    w += PFA_WS_ROUND((size_t)(h + 1) * (h + 1) * 16);   // GG / H
-    w += PFA_WS_ROUND((size_t)h * h * 8) * 2;            // W1, mu
+    w += pfa_ws_gg_bytes(h);                             // GG (H with W1)
+    w += PFA_WS_ROUND((size_t)h * h * 8);                // W1
+    w += pfa_ws_mu_bytes(h);                             // mu / hv, hw, hpiv / Gy
     w += PFA_WS_ROUND((size_t)h * 8) * 3;                // bb, mw, vec
-    w += PFA_WS_ROUND((size_t)h * 16) * 2;               // hv, hw
-    w += PFA_WS_ROUND((size_t)h * 4);                    // hpiv
     w += 16;                                             // flag
     return w;
 }
@@ -80,15 +108,16 @@ PFA_HD static inline void pfa_ws_carve(PfaWS &W, char *hot, char *cold, int h)
     auto take = [](char *&p, size_t bytes) { char *r = p; p += PFA_WS_ROUND(bytes); return r; };
     W.h = h;
     W.d = (int)d;
-    W.GG = (pfa_i128 *)take(hot, (size_t)(h + 1) * (h + 1) * 16);
+    W.GG = (pfa_i128 *)take(hot, pfa_ws_gg_bytes(h));
     W.W1 = (int64_t *)take(hot, (size_t)h * h * 8);
-    W.mu = (double *)take(hot, (size_t)h * h * 8);
+    char *mr = take(hot, pfa_ws_mu_bytes(h));
+    W.mu = (double *)mr;
+    W.hv = (pfa_i128 *)mr;
+    W.hw = (pfa_i128 *)(mr + PFA_WS_ROUND((size_t)h * 16));
+    W.hpiv = (int *)(mr + 2 * PFA_WS_ROUND((size_t)h * 16));
     W.bb = (double *)take(hot, (size_t)h * 8);
     W.mw = (double *)take(hot, (size_t)h * 8);
     W.vec = (int64_t *)take(hot, (size_t)h * 8);
-    W.hv = (pfa_i128 *)take(hot, (size_t)h * 16);
-    W.hw = (pfa_i128 *)take(hot, (size_t)h * 16);
-    W.hpiv = (int *)take(hot, (size_t)h * 4);
     W.flag = (int *)take(hot, 16);
     W.Z = (int64_t *)take(cold, (size_t)h * h * 8);
     W.W2 = (int64_t *)take(cold, (size_t)h * h * 8);
@@ -264,19 +293,19 @@ PFA_HD static inline int pfa_unimodular_reduced(const PfaTeam &T, const int64_t 
 // value is final and is broadcast.  Each lane subtracts its terms in the
 // host's order (l ascending) with the host's association, so every value is
 // bit-identical to the sequential host loop.
-PFA_HD static inline void pfa_gs_row(const PfaTeam &T, const PfaWS &W, int kk, int L,
-                                     const pfa_i128 *GG)
+PFA_HD static inline void pfa_gs_row(const PfaTeam &T, const PfaWS &W, int kk, const pfa_i128 *GG)
 {
     double *mu = W.mu, *bb = W.bb;
-    const int S = W.h;
+    const int rk = pfa_mi(kk, 0);
 #if defined(__HIP_DEVICE_COMPILE__)
     if (T.n > kk) {
         const int j = T.lane;
-        double r = j <= kk ? (double)GG[kk * L + j] : 0.0;
+        double r = j <= kk ? (double)GG[pfa_gi(kk, j)] : 0.0;
+        const int rj = pfa_mi(j, 0);
         for (int l = 0; l < kk; ++l) {
             const double m = __shfl(r, l) / bb[l];
-            if (j == l) mu[kk * S + l] = m;
-            else if (j > l && j < kk) r -= mu[j * S + l] * m * bb[l];
+            if (j == l) mu[rk + l] = m;
+            else if (j > l && j < kk) r -= mu[rj + l] * m * bb[l];
             else if (j == kk) r -= m * m * bb[l];
         }
         if (j == kk) bb[kk] = r;
@@ -285,12 +314,12 @@ PFA_HD static inline void pfa_gs_row(const PfaTeam &T, const PfaWS &W, int kk, i
     }
 #endif
     for (int j = 0; j < kk; ++j) {
-        double dd = (double)GG[kk * L + j];
-        for (int l = 0; l < j; ++l) dd -= mu[j * S + l] * mu[kk * S + l] * bb[l];
-        mu[kk * S + j] = dd / bb[j];
+        double dd = (double)GG[pfa_gi(kk, j)];
+        for (int l = 0; l < j; ++l) dd -= mu[pfa_mi(j, l)] * mu[rk + l] * bb[l];
+        mu[rk + j] = dd / bb[j];
     }
-    double dd = (double)GG[kk * L + kk];
-    for (int l = 0; l < kk; ++l) dd -= mu[kk * S + l] * mu[kk * S + l] * bb[l];
+    double dd = (double)GG[pfa_gi(kk, kk)];
+    for (int l = 0; l < kk; ++l) dd -= mu[rk + l] * mu[rk + l] * bb[l];
     if (T.lane == 0) bb[kk] = dd;
     PFA_SYNC();
 }
@@ -298,27 +327,26 @@ PFA_HD static inline void pfa_gs_row(const PfaTeam &T, const PfaWS &W, int kk, i
 // Coefficients mw[0:m] of the row `last` (row m of GG) on the reduced
 // basis: mw[j] = (G[m][j] - sum_{l<j} mu[j][l] mw[l] bb[l]) / bb[j], right-
 // looking over the lanes on the device as pfa_gs_row (bit-identical).
-PFA_HD static inline void pfa_gs_last(const PfaTeam &T, const PfaWS &W, int m, int L,
-                                      const pfa_i128 *GG)
+PFA_HD static inline void pfa_gs_last(const PfaTeam &T, const PfaWS &W, int m, const pfa_i128 *GG)
 {
     double *mu = W.mu, *bb = W.bb, *mw = W.mw;
-    const int S = W.h;
 #if defined(__HIP_DEVICE_COMPILE__)
     if (T.n >= m) {
         const int j = T.lane;
-        double r = j < m ? (double)GG[m * L + j] : 0.0;
+        double r = j < m ? (double)GG[pfa_gi(m, j)] : 0.0;
+        const int rj = pfa_mi(j, 0);
         for (int l = 0; l < m; ++l) {
             const double w = __shfl(r, l) / bb[l];
             if (j == l) mw[l] = w;
-            else if (j > l && j < m) r -= mu[j * S + l] * w * bb[l];
+            else if (j > l && j < m) r -= mu[rj + l] * w * bb[l];
         }
         PFA_SYNC();
         return;
     }
 #endif
     for (int j = 0; j < m; ++j) {
-        double dd = (double)GG[m * L + j];
-        for (int l = 0; l < j; ++l) dd -= mu[j * S + l] * mw[l] * bb[l];
+        double dd = (double)GG[pfa_gi(m, j)];
+        for (int l = 0; l < j; ++l) dd -= mu[pfa_mi(j, l)] * mw[l] * bb[l];
         if (T.lane == 0) mw[j] = dd / bb[j];
         PFA_SYNC();
     }
@@ -327,8 +355,7 @@ PFA_HD static inline void pfa_gs_last(const PfaTeam &T, const PfaWS &W, int m, i
 // b_k -= q b_j on the rows (columns split over lanes) and on GG (exact).
 // Row index m means `last`.
 PFA_HD static inline int pfa_lll_sub(const PfaTeam &T, const PfaWS &W, int64_t *B, int64_t *last,
-                                     int m, int n, pfa_i128 *GG, int L, int M, int kk, int jj,
-                                     int64_t q)
+                                     int m, int n, pfa_i128 *GG, int M, int kk, int jj, int64_t q)
 {
     int64_t *bk = kk == m ? last : &B[kk * n];
     const int64_t *bj = jj == m ? last : &B[jj * n];
@@ -339,37 +366,37 @@ PFA_HD static inline int pfa_lll_sub(const PfaTeam &T, const PfaWS &W, int64_t *
         bk[c] = (int64_t)x;
     }
     // new G_kk = G_kk - 2 q G_kj + q^2 G_jj, computed before row kk changes
-    pfa_i128 kj = GG[kk * L + jj], jv = GG[jj * L + jj], t1, t2;
+    pfa_i128 kj = GG[pfa_gi(kk, jj)], jv = GG[pfa_gi(jj, jj)], t1, t2;
     int badk = pfa_mul_ovf(jv, (pfa_i128)q, &t1) || pfa_sub_ovf(t1, 2 * kj, &t1) ||
-               pfa_mul_ovf(t1, (pfa_i128)q, &t2) || pfa_add_ovf(GG[kk * L + kk], t2, &t2);
+               pfa_mul_ovf(t1, (pfa_i128)q, &t2) || pfa_add_ovf(GG[pfa_gi(kk, kk)], t2, &t2);
     PFA_SYNC();
     for (int l = T.lane; l < M; l += T.n) {
         if (l == kk) continue;
         pfa_i128 t3;
-        if (pfa_mul_ovf(GG[jj * L + l], (pfa_i128)q, &t3) || pfa_sub_ovf(GG[kk * L + l], t3, &t3))
+        if (pfa_mul_ovf(GG[pfa_gi(jj, l)], (pfa_i128)q, &t3) ||
+            pfa_sub_ovf(GG[pfa_gi(kk, l)], t3, &t3))
             bad = 1;
-        GG[kk * L + l] = t3;
-        GG[l * L + kk] = t3;
+        GG[pfa_gi(kk, l)] = t3;
     }
-    if (T.lane == 0) GG[kk * L + kk] = t2;
+    if (T.lane == 0) GG[pfa_gi(kk, kk)] = t2;
     return pfa_team_any(T, W.flag, bad || badk);
 }
 
 // LLL (delta 0.99) of the m rows of B (length n, stride n) given their exact
-// Gram matrix GG (stride L = m + 1); `last` (row m of GG) is size-reduced
+// Gram matrix GG (lower triangle); `last` (row m of GG) is size-reduced
 // afterwards.  0, -1 overflow, -2 rank loss, -3 no convergence.
 PFA_HD static inline int pfa_lll_core(const PfaTeam &T, const PfaWS &W, int64_t *B, int m, int n,
                                       pfa_i128 *GG, int64_t *last)
 {
-    const int L = m + 1, M = m + (last != nullptr), S = W.h;
+    const int M = m + (last != nullptr);
     double *mu = W.mu, *bb = W.bb, *mw = W.mw;
     int iters = 0;
     pfa_team_clear(T, W.flag);
     if (m >= 1) {
-        pfa_gs_row(T, W, 0, L, GG);
+        pfa_gs_row(T, W, 0, GG);
         if (!(bb[0] > 0)) return -2;
     }
-    if (m >= 2) pfa_gs_row(T, W, 1, L, GG);
+    if (m >= 2) pfa_gs_row(T, W, 1, GG);
     PFA_SYNC();
     int k = 1;
     while (k < m) {
@@ -377,59 +404,65 @@ PFA_HD static inline int pfa_lll_core(const PfaTeam &T, const PfaWS &W, int64_t 
         for (int pass = 0; pass < 4; ++pass) {
             int changed = 0;
             for (int j = k - 1; j >= 0; --j) {
-                double q = nearbyint(mu[k * S + j]);
+                double q = nearbyint(mu[pfa_mi(k, j)]);
                 if (q == 0.0) continue;
                 if (!(fabs(q) <= 9.0e18)) return -1;
-                if (pfa_lll_sub(T, W, B, last, m, n, GG, L, M, k, j, (int64_t)q)) return -1;
-                for (int l = T.lane; l < j; l += T.n) mu[k * S + l] -= q * mu[j * S + l];
-                if (T.lane == 0) mu[k * S + j] -= q;
+                if (pfa_lll_sub(T, W, B, last, m, n, GG, M, k, j, (int64_t)q)) return -1;
+                for (int l = T.lane; l < j; l += T.n) mu[pfa_mi(k, l)] -= q * mu[pfa_mi(j, l)];
+                if (T.lane == 0) mu[pfa_mi(k, j)] -= q;
                 changed = 1;
                 PFA_SYNC();
             }
             if (!changed) break;
-            pfa_gs_row(T, W, k, L, GG);
+            pfa_gs_row(T, W, k, GG);
             PFA_SYNC();
         }
         if (!(bb[k] > 0)) return -2;
-        if (bb[k] < (0.99 - mu[k * S + k - 1] * mu[k * S + k - 1]) * bb[k - 1]) {
+        const double mk1 = mu[pfa_mi(k, k - 1)];
+        if (bb[k] < (0.99 - mk1 * mk1) * bb[k - 1]) {
             for (int c = T.lane; c < n; c += T.n) {
                 int64_t t = B[k * n + c];
                 B[k * n + c] = B[(k - 1) * n + c];
                 B[(k - 1) * n + c] = t;
             }
+            // exchange indices k-1 and k of the symmetric GG: entries (k, l)
+            // and (k-1, l) for l outside {k-1, k}, and the two diagonals
+            // ((k, k-1) stays)
             for (int l = T.lane; l < M; l += T.n) {
-                pfa_i128 t = GG[k * L + l];
-                GG[k * L + l] = GG[(k - 1) * L + l];
-                GG[(k - 1) * L + l] = t;
-            }
-            PFA_SYNC();
-            for (int l = T.lane; l < M; l += T.n) {
-                pfa_i128 t = GG[l * L + k];
-                GG[l * L + k] = GG[l * L + k - 1];
-                GG[l * L + k - 1] = t;
+                if (l == k || l == k - 1) {
+                    if (l == k) {
+                        pfa_i128 t = GG[pfa_gi(k, k)];
+                        GG[pfa_gi(k, k)] = GG[pfa_gi(k - 1, k - 1)];
+                        GG[pfa_gi(k - 1, k - 1)] = t;
+                    }
+                    continue;
+                }
+                pfa_i128 t = GG[pfa_gi(k, l)];
+                GG[pfa_gi(k, l)] = GG[pfa_gi(k - 1, l)];
+                GG[pfa_gi(k - 1, l)] = t;
             }
             PFA_SYNC();
             k = k > 1 ? k - 1 : 1;
-            if (k == 1) pfa_gs_row(T, W, 0, L, GG);
-            pfa_gs_row(T, W, k - 1, L, GG);
-            pfa_gs_row(T, W, k, L, GG);
+            if (k == 1) pfa_gs_row(T, W, 0, GG);
+            pfa_gs_row(T, W, k - 1, GG);
+            pfa_gs_row(T, W, k, GG);
             PFA_SYNC();
         } else {
             k++;
-            if (k < m) pfa_gs_row(T, W, k, L, GG);
+            if (k < m) pfa_gs_row(T, W, k, GG);
             PFA_SYNC();
         }
     }
     if (last && m > 0) {
         for (int pass = 0; pass < 4; ++pass) {
-            pfa_gs_last(T, W, m, L, GG);
+            pfa_gs_last(T, W, m, GG);
             int changed = 0;
             for (int j = m - 1; j >= 0; --j) {
                 double q = nearbyint(mw[j]);
                 if (q == 0.0) continue;
                 if (!(fabs(q) <= 9.0e18)) return -1;
-                if (pfa_lll_sub(T, W, B, last, m, n, GG, L, M, m, j, (int64_t)q)) return -1;
-                for (int l = T.lane; l < j; l += T.n) mw[l] -= q * mu[j * S + l];
+                if (pfa_lll_sub(T, W, B, last, m, n, GG, M, m, j, (int64_t)q)) return -1;
+                for (int l = T.lane; l < j; l += T.n) mw[l] -= q * mu[pfa_mi(j, l)];
                 if (T.lane == 0) mw[j] -= q;
                 changed = 1;
                 PFA_SYNC();
@@ -445,7 +478,6 @@ PFA_HD static inline int pfa_lll_core(const PfaTeam &T, const PfaWS &W, int64_t 
 PFA_HD static inline int pfa_lll_euclid(const PfaTeam &T, const PfaWS &W, int64_t *B, int m, int n)
 {
     if (m < 2) return 0;
-    const int L = m + 1;
     int bad = 0;
     for (int e = T.lane; e < m * m; e += T.n) {
         int i = e / m, j = e % m;
@@ -453,8 +485,7 @@ PFA_HD static inline int pfa_lll_euclid(const PfaTeam &T, const PfaWS &W, int64_
         pfa_i128 s = 0;
         for (int c = 0; c < n; ++c)
             if (pfa_add_ovf(s, (pfa_i128)B[i * n + c] * B[j * n + c], &s)) bad = 1;
-        W.GG[i * L + j] = s;
-        W.GG[j * L + i] = s;
+        W.GG[pfa_gi(i, j)] = s;
     }
     pfa_team_clear(T, W.flag);
     if (pfa_team_any(T, W.flag, bad)) return -1;
@@ -466,7 +497,7 @@ PFA_HD static inline int pfa_lll_euclid(const PfaTeam &T, const PfaWS &W, int64_
 PFA_HD static inline int pfa_lll_form(const PfaTeam &T, const PfaWS &W, int64_t *B, int m, int n,
                                       const int64_t *G, int64_t *last)
 {
-    const int L = m + 1, M = m + (last != nullptr);
+    const int M = m + (last != nullptr);
     pfa_i128 *Gy = (pfa_i128 *)W.mu;        // mu is free until pfa_lll_core
     int bad = 0;
     pfa_team_clear(T, W.flag);
@@ -486,8 +517,7 @@ PFA_HD static inline int pfa_lll_form(const PfaTeam &T, const PfaWS &W, int64_t 
                 if (!x[c]) continue;
                 if (pfa_mul_ovf(Gy[c], (pfa_i128)x[c], &t) || pfa_add_ovf(s, t, &s)) bad = 1;
             }
-            W.GG[i * L + j] = s;
-            W.GG[j * L + i] = s;
+            W.GG[pfa_gi(i, j)] = s;
         }
         PFA_SYNC();
     }
