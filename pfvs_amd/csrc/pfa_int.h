@@ -77,9 +77,25 @@ PFA_HD static inline int pfa_ctz128(pfa_u128 x)
     return lo ? pfa_ctz64(lo) : 64 + pfa_ctz64((uint64_t)(x >> 64));
 }
 
-// Stein's binary gcd
+// Stein's binary gcd on 32 bits
+PFA_HD static inline uint32_t pfa_gcd32(uint32_t u, uint32_t v)
+{
+    if (u == 0) return v;
+    if (v == 0) return u;
+    const int sh = __builtin_ctz(u | v);
+    u >>= __builtin_ctz(u);
+    do {
+        v >>= __builtin_ctz(v);
+        if (u > v) { uint32_t t = v; v = u; u = t; }
+        v -= u;
+    } while (v);
+    return u << sh;
+}
+
+// Stein's binary gcd (the 32-bit loop when both operands fit)
 PFA_HD static inline uint64_t pfa_gcd64(uint64_t u, uint64_t v)
 {
+    if (((u | v) >> 32) == 0) return pfa_gcd32((uint32_t)u, (uint32_t)v);
     if (u == 0) return v;
     if (v == 0) return u;
     int sh = pfa_ctz64(u | v);

@@ -925,6 +925,43 @@ __device__ static inline uint64_t wave_sum_u64(uint64_t v)
     return v;
 }
 
+// NMAX >= D (the launch's largest dimension): the coordinates sit in
+// registers as doubles, and a row product runs as unrolled FMAs when
+// max|row| sum|c| < 2^53 and max|row| < 2^31 (every product and partial sum
+// is then an exact integer; the low word of each int64 entry holds it),
+// else as pfa_dot_c (int64 under its bound, int128 beyond).
+template <int NMAX>
+__device__ __forceinline__ int64_t post_dot(const int64_t *row, const double *cd, const int32_t *cc,
+                                            int d, bool fast, int64_t rmax, int64_t cmax, int *ok)
+{
+    if (fast) {
+        const int32_t *r32 = (const int32_t *)row;
+        double s = 0.0;
+#pragma unroll
+        for (int t = 0; t < NMAX; ++t)
+            if (t < d) s = fma((double)r32[2 * t], cd[t], s);
+        return (int64_t)s;
+    }
+    return pfa_dot_c(row, cc, d, rmax, cmax, ok);
+}
+
+// The same product when the whole wavefront shares one p-vector: the row is
+// read through the scalar cache (constant address space, wave-uniform
+// address), the coordinates are the lane's own.
+typedef __attribute__((address_space(4))) const int64_t pfa_cint64;
+// (all NMAX entries are read, so the loads merge: cd[t] = 0 for t >= d
+// adds nothing, and the records buffer extends past the last row)
+template <int NMAX>
+__device__ __forceinline__ int64_t post_dot_uni(const int64_t *row, const double *cd)
+{
+    const pfa_cint64 *r64 = (const pfa_cint64 *)row;
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < NMAX; ++t) s = fma((double)(int32_t)r64[t], cd[t], s);
+    return (int64_t)s;
+}
+
+template <int NMAX>
 __global__ __launch_bounds__(64) void k_post(FiltArgs a)
 {
     extern __shared__ int64_t stage[];             // rows 64 x MH (when kept), then c 64 x D
@@ -934,10 +971,18 @@ __global__ __launch_bounds__(64) void k_post(FiltArgs a)
     unsigned long long t_pts = 0, t_kept = 0;
     for (int64_t base = (int64_t)blockIdx.x * 64; base < a.n; base += (int64_t)gridDim.x * 64) {
         const int64_t kp = base + lane;
-        const bool valid = kp < a.n && a.pidx[kp] >= 0;
-        const int64_t *rec = a.recs + (valid ? (int64_t)a.pidx[kp] * a.L.words : 0);
+        const int32_t pid = kp < a.n ? a.pidx[kp] : -1;
+        const bool valid = pid >= 0;
+        const int64_t *rec = a.recs + (valid ? (int64_t)pid * a.L.words : 0);
         const int32_t *ri = (const int32_t *)rec;
         const int d = valid ? ri[1] : 0, h = d + 1;
+        // one p-vector for the whole wavefront (the usual case: slots are
+        // handed out in chunks per p-vector): its rows come through the
+        // scalar cache
+        const int32_t p0 = __builtin_amdgcn_readfirstlane(pid);
+        const bool uni = p0 >= 0 && !__ballot(valid && pid != p0);
+        const int64_t *urec = a.recs + (uni ? (int64_t)p0 * a.L.words : 0);
+        const int du = __builtin_amdgcn_readfirstlane(d);
         // the wavefront's 64 c vectors are one contiguous block: stage it
         const int64_t nc = (a.n - base < 64 ? a.n - base : 64) * a.D;
         __syncthreads();
@@ -947,15 +992,24 @@ __global__ __launch_bounds__(64) void k_post(FiltArgs a)
         int ok = 1;
         int64_t Kn0 = 0, M0 = 0, cmax = 0;
         uint64_t kg = 0;
-        for (int t = 0; t < d; ++t) {
-            const int64_t x = cc[t] < 0 ? -(int64_t)cc[t] : cc[t];
-            cmax = x > cmax ? x : cmax;
+        double cd[NMAX], csum = 0.0;
+#pragma unroll
+        for (int t = 0; t < NMAX; ++t) {
+            const int32_t x = t < d ? cc[t] : 0;
+            cd[t] = (double)x;
+            csum += fabs(cd[t]);
+            const int64_t ax = x < 0 ? -(int64_t)x : x;
+            cmax = ax > cmax ? ax : cmax;
         }
+        const int64_t zmax = valid ? rec[15] : 0, bmax = valid ? rec[14] : 0;
+        const bool fz = zmax < ((int64_t)1 << 31) && (double)zmax * csum < PFA_TWO53;
+        const bool fb = bmax < ((int64_t)1 << 31) && (double)bmax * csum < PFA_TWO53;
         // Kn (staged when kept)
         for (int i = 0; i < a.MH; ++i) {
             int64_t v = 0;
             if (valid && i < h) {
-                v = pfa_dot_c(rec + a.L.ZB + (int64_t)i * d, cc, d, rec[15], cmax, &ok);
+                v = uni && fz ? post_dot_uni<NMAX>(urec + a.L.ZB + (int64_t)i * du, cd)
+                              : post_dot<NMAX>(rec + a.L.ZB + (int64_t)i * d, cd, cc, d, fz, zmax, cmax, &ok);
                 if (i == 0) Kn0 = v;
                 else if (kg != 1) kg = pfa_gcd64(kg, (uint64_t)(v < 0 ? -v : v));
             }
@@ -971,7 +1025,8 @@ __global__ __launch_bounds__(64) void k_post(FiltArgs a)
         for (int i = 0; i < (store ? a.MH : 1); ++i) {
             int64_t v = 0;
             if (valid && i < h) {
-                v = pfa_dot_c(rec + a.L.Bi + (int64_t)i * d, cc, d, rec[14], cmax, &ok);
+                v = uni && fb ? post_dot_uni<NMAX>(urec + a.L.Bi + (int64_t)i * du, cd)
+                              : post_dot<NMAX>(rec + a.L.Bi + (int64_t)i * d, cd, cc, d, fb, bmax, cmax, &ok);
                 if (i == 0) M0 = v;
             }
             if (store) stage[lane * a.MH + i] = v;
@@ -1549,8 +1604,19 @@ static void ctx_init(DevCtx *C, int device, char *errbuf)
                                                       : prop.sharedMemPerBlock;
 }
 
+enum {
+    B_GEO_H, B_GEO_Q, B_GEO_QMAX, B_GEO_LIN, B_KAPPA, B_MBASIS, B_KOFS, B_MOFS, B_PS, B_PGEO,
+    B_RECS, B_RING, B_SEQ, B_STACKS, B_CTL, B_PT_PIDX, B_PT_Q, B_PT_C, B_CNT, B_M, B_KN, B_NEXP, B_BAD,
+    B_PRIMES, B_KT, B_KTOFS, B_KTMAX, B_MODCNT, B_MODLIST, B_MODNEXP, B_MODM, B_NONSING, B_PREPCOLD,
+    B_TOT, B_KEEPCNT, B_KEEPKP, B_KEEPNEXP, B_KEEPMETA, B_PREPORDER,
+    B_SPR, B_SPKV, B_SPROFS, B_SPKVOFS, B_SPINTS, B_NBUF
+};
+
 static void *dev_get(DevCtx *C, int slot, size_t bytes, char *errbuf)
 {
+    // k_post reads whole NMAX-wide rows of the records (up to PFA_DMAX words
+    // past a p-vector's last row): the records buffer keeps 1 KB beyond
+    if (slot == B_RECS) bytes += 1024;
     if ((int)C->bufs.size() <= slot) C->bufs.resize(slot + 1);
     DevBuf &b = C->bufs[slot];
     if (b.n < bytes) {
@@ -1589,13 +1655,6 @@ static void *host_get(DevCtx *C, Stage &S, int slot, size_t bytes, char *errbuf)
 
 #define CTL_BYTES (16 * 8 * 16)
 
-enum {
-    B_GEO_H, B_GEO_Q, B_GEO_QMAX, B_GEO_LIN, B_KAPPA, B_MBASIS, B_KOFS, B_MOFS, B_PS, B_PGEO,
-    B_RECS, B_RING, B_SEQ, B_STACKS, B_CTL, B_PT_PIDX, B_PT_Q, B_PT_C, B_CNT, B_M, B_KN, B_NEXP, B_BAD,
-    B_PRIMES, B_KT, B_KTOFS, B_KTMAX, B_MODCNT, B_MODLIST, B_MODNEXP, B_MODM, B_NONSING, B_PREPCOLD,
-    B_TOT, B_KEEPCNT, B_KEEPKP, B_KEEPNEXP, B_KEEPMETA, B_PREPORDER,
-    B_SPR, B_SPKV, B_SPROFS, B_SPKVOFS, B_SPINTS, B_NBUF
-};
 
 // the k largest primes below 2^31 with their Barrett constants
 static std::vector<PfaModP> primes31(int k)
@@ -1891,7 +1950,10 @@ static void filter_run(DevCtx *C, hipStream_t s, const pfa_run_input *gin,
         a.keepcount = d_kc;
         const int pgrid = (int)std::min<int64_t>((a.n + 63) / 64, 65536);
         const size_t plds = (a.M ? (size_t)64 * a.MH * 8 : 0) + (size_t)64 * a.D * 4;
-        hipLaunchKernelGGL(k_post, dim3(pgrid), dim3(64), plds, s, a);
+        auto kpost = a.D <= 8 ? k_post<8> : a.D <= 12 ? k_post<12> : a.D <= 16 ? k_post<16> :
+                     a.D <= 20 ? k_post<20> : a.D <= 24 ? k_post<24> : a.D <= 32 ? k_post<32> :
+                     k_post<40>;
+        hipLaunchKernelGGL(kpost, dim3(pgrid), dim3(64), plds, s, a);
         HIPCHK(hipGetLastError());
         if (!a.filter) return;
     }

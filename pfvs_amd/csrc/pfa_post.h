@@ -77,8 +77,8 @@ PFA_HD static inline int64_t pfa_count_expansions_fast(int64_t M0, int64_t Kn0, 
     if (!(q < QK)) return 0;
     if (QK >= B52) return pfa_count_expansions(M0, Kn0, kg, q, Q);
     const int64_t num = q + M0 * Kn0;               // |.| < 2^63
-    if (num % k != 0) return -1;                    // exact by construction
-    const int64_t raw = num / k;
+    if (k != 1 && num % k != 0) return -1;          // exact by construction
+    const int64_t raw = k == 1 ? num : num / k;
     const int64_t gmax = pfa_fdiv_small(QK - 1, q);
     if (gmax > 1000000) return -1;
     const double gd = (double)gmax;
