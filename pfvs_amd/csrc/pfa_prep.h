@@ -57,6 +57,18 @@ struct PfaWS {
 
 #define PFA_WS_ROUND(x) (((size_t)(x) + 15) & ~(size_t)15)
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// lane l's double on every lane, l wave-uniform (two readlanes: no LDS
+// round trip, the same bits as a shuffle from lane l)
+__device__ static inline double pfa_readlane_f64(double x, int l)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+#endif
+
 // index of (i, j) in a symmetric matrix stored as its lower triangle
 PFA_HD static inline int pfa_gi(int i, int j)
 {
@@ -303,7 +315,7 @@ PFA_HD static inline void pfa_gs_row(const PfaTeam &T, const PfaWS &W, int kk, c
         double r = j <= kk ? (double)GG[pfa_gi(kk, j)] : 0.0;
         const int rj = pfa_mi(j, 0);
         for (int l = 0; l < kk; ++l) {
-            const double m = __shfl(r, l) / bb[l];
+            const double m = pfa_readlane_f64(r, l) / bb[l];
             if (j == l) mu[rk + l] = m;
             else if (j > l && j < kk) r -= mu[rj + l] * m * bb[l];
             else if (j == kk) r -= m * m * bb[l];
@@ -336,7 +348,7 @@ PFA_HD static inline void pfa_gs_last(const PfaTeam &T, const PfaWS &W, int m, c
         double r = j < m ? (double)GG[pfa_gi(m, j)] : 0.0;
         const int rj = pfa_mi(j, 0);
         for (int l = 0; l < m; ++l) {
-            const double w = __shfl(r, l) / bb[l];
+            const double w = pfa_readlane_f64(r, l) / bb[l];
             if (j == l) mw[l] = w;
             else if (j > l && j < m) r -= mu[rj + l] * w * bb[l];
         }
