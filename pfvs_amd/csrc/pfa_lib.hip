@@ -1123,6 +1123,19 @@ __device__ __forceinline__ double col_f64(const int32_t *kt, const double *gMd, 
 
 // SP: the launch has geometries with a sparse form of kappa (a separate
 // instantiation, so the dense-only kernel keeps its registers)
+// f(0), f(1), ..., f(k - 1) with constant arguments, as nested tests (a
+// step k evaluates k + 1 of them)
+template <int T, int NMAX, class F>
+__device__ __forceinline__ void det_chain(int k, F &f)
+{
+    if constexpr (T < NMAX) {
+        if (T < k) {
+            f(T);
+            det_chain<T + 1, NMAX>(k, f);
+        }
+    }
+}
+
 template <int NMAX, bool SP>
 __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
 {
@@ -1360,9 +1373,11 @@ __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
             if (!__ballot(run)) break;
             double v = build(k);
             int okb = 1;
-#pragma unroll
-            for (int t = 0; t < NMAX; ++t) {
-                if (t < k) {                      // wave-uniform
+            // the steps t < k: a chain of k + 1 wave-uniform tests up to NMAX
+            // 20; beyond, one test per t (the chain's registers would cost
+            // occupancy there)
+            if constexpr (NMAX <= 20) {
+                auto upd = [&](int t) {
                     const int pt = s_p[t];
                     const double2 e = s_e[t];
                     const double cp = __shfl(v, L.inlane ? L.src0 + pt : L.lane);
@@ -1372,6 +1387,22 @@ __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
                         const double m = fmax(fmax(fabs(x1), fabs(x2)), fmax(fabs(x), 8.0 * fabs(y)));
                         okb &= (int)(m < PFA_TWO53);
                         v = rint(y);
+                    }
+                };
+                det_chain<0, NMAX>(k, upd);
+            } else {
+#pragma unroll
+                for (int t = 0; t < NMAX; ++t) {
+                    if (t < k) {                  // wave-uniform
+                        const int pt = s_p[t];
+                        const double2 e = s_e[t];
+                        const double cp = __shfl(v, L.inlane ? L.src0 + pt : L.lane);
+                        if (run && i < n && ustep > t) {
+                            const double x1 = v * e.x, x2 = pc[t] * cp, x = x1 - x2, y = x * e.y;
+                            const double m = fmax(fmax(fabs(x1), fabs(x2)), fmax(fabs(x), 8.0 * fabs(y)));
+                            okb &= (int)(m < PFA_TWO53);
+                            v = rint(y);
+                        }
                     }
                 }
             }
