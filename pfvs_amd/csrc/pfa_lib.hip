@@ -2918,6 +2918,11 @@ extern "C" int pfa_reserve(int device, int64_t n_p, int64_t n_points, int h_max,
         dev_get(C, B_KEEPKP, 8 * n1, errbuf);
         dev_get(C, B_KEEPNEXP, 8 * n1, errbuf);
         dev_get(C, B_KEEPMETA, 8 * n1, errbuf);
+        // the filter's list of points for the modular determinant (up to 2^20)
+        const size_t mc = std::min<size_t>(n1, (size_t)1 << 20);
+        dev_get(C, B_MODLIST, 8 * mc, errbuf);
+        dev_get(C, B_MODNEXP, 8 * mc, errbuf);
+        dev_get(C, B_MODM, 8 * mc * HS, errbuf);
         for (Stage &S : C->stage) {
             std::lock_guard<std::mutex> sl(S.mu);
             host_get(C, S, H_PIDX, 4 * n1, errbuf);
