@@ -101,6 +101,41 @@ PFA_HD static inline PfaProb rec_view(const int64_t *r, const RecL &L)
     return P;
 }
 
+// The same view for a wave-uniform record: the header through the scalar
+// cache (constant address space; the header does not change during a search,
+// only the status word, which is not read here), the row pointers uniform.
+typedef __attribute__((address_space(4))) const int64_t pfa_chdr64;
+typedef __attribute__((address_space(4))) const int32_t pfa_chdr32;
+__device__ static inline PfaProb rec_view_uni(const int64_t *r, const RecL &L)
+{
+    const pfa_chdr64 *r64 = (const pfa_chdr64 *)r;
+    const pfa_chdr32 *ri = (const pfa_chdr32 *)r;
+    PfaProb P;
+    P.d = ri[1];
+    P.m0_level = ri[20];
+    P.leafg = ri[21];
+    P.nL = ri[22];
+    P.strict = ri[23];
+    P.qmax = r64[1];
+    P.Q = r64[2];
+    P.linmin = r64[3];
+    P.Qe = __builtin_bit_cast(double, r64[4]);
+    P.QGmul = __builtin_bit_cast(double, r64[5]);
+    P.phi = __builtin_bit_cast(double, r64[6]);
+    P.slack = __builtin_bit_cast(double, r64[7]);
+    P.Kerr = __builtin_bit_cast(double, r64[8]);
+    P.hmask = (uint64_t)r64[9];
+    P.matmax = r64[16];
+    P.hmax = r64[17];
+    P.U = (const double *)(r + L.U);
+    P.Uinv = (const double *)(r + L.Uinv);
+    P.mat = r + L.mat;
+    P.linvec = r + L.lin;
+    P.H = r + L.H;
+    P.L = r + L.L;
+    return P;
+}
+
 PFA_HD static inline int32_t &rec_status(int64_t *r) { return ((int32_t *)r)[0]; }
 
 // scalar constants of the search for a problem of dimension d with factor
@@ -603,8 +638,8 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
             nd = *(const PfaNode *)e;
             pidx = ((const int32_t *)(e + 128))[0];
             const int32_t *ec = (const int32_t *)(e + 144);
-            for (int k = nd.i + 1; k < D; ++k) c[k] = ec[k];
-            for (int k = 0; k <= nd.i; ++k) c[k] = 0;
+            // (one loop of the wave-uniform length D: no divergent trip counts)
+            for (int k = 0; k < D; ++k) c[k] = k > nd.i ? ec[k] : 0;
             v = nd.cur + (lane - (j > 0 ? s_incl[j - 1] : 0));
         }
         int part_new_cur = 0;
@@ -619,9 +654,13 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
         int has_child = 0, emit = 0;
         int64_t q = 0;
         PfaNode ch;
+        // one p-vector for every candidate of the step (the usual case): its
+        // header through the scalar cache
+        const int32_t p0 = __builtin_amdgcn_readfirstlane(j >= 0 ? pidx : -1);
+        const bool uni = p0 >= 0 && !__ballot(j >= 0 && pidx != p0);
         if (j >= 0) {
             const int64_t *rec = a.recs + (int64_t)pidx * a.L.words;
-            PfaProb P = rec_view(rec, a.L);
+            PfaProb P = uni ? rec_view_uni(a.recs + (int64_t)p0 * a.L.words, a.L) : rec_view(rec, a.L);
             n_cand++;
             int r = pfa_cand(P, nd, v, ch);
             if (r == PFA_CAND_OVF) {
