@@ -1176,7 +1176,7 @@ __device__ __forceinline__ void det_chain(int k, F &f)
 }
 
 template <int NMAX, bool SP>
-__global__ __launch_bounds__(256) void k_filter(FiltArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_filter(FiltArgs a)
 {
     // dynamic LDS: kappa (lds_floats ints, rounded up to 4), then per
     // wavefront the groups' step tables: (pivot, 1 / previous pivot) as
@@ -1187,8 +1187,12 @@ __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
     __shared__ int lds_sparse;
     __shared__ int32_t s_Mi[256];                  // the groups' M: int32 (small) ...
     __shared__ double s_Md[256];                   // ... and double (exact)
-    __shared__ int64_t s_kofs[256];                // kappa offset and bound of geometries < 256
-    __shared__ double s_kmax[256];
+    __shared__ int64_t s_kofs[128];                // kappa offset and bound of geometries < 128
+    __shared__ double s_kmax[128];
+    // per wavefront: Binter of one p-vector as int32 (NMAX <= 24), the
+    // p-vector staged (bp, wave-uniform) and its max|Binter|
+    constexpr int BSZ = NMAX <= 24 ? (NMAX + 1) * NMAX : 1;
+    __shared__ int32_t s_B[4 * BSZ];
     const DetLanes L = det_lanes(a.hl);
     const int R = L.R, i = L.j, nlmax = a.hl - 1;
     const int wv = threadIdx.x >> 6;
@@ -1262,14 +1266,17 @@ __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
         }
         __syncthreads();
     }
-    const int ng = a.n_geo < 256 ? a.n_geo : 256;
+    const int ng = a.n_geo < 128 ? a.n_geo : 128;
     for (int g = threadIdx.x; g < ng; g += blockDim.x) {
         s_kofs[g] = a.kTofs[g];
         s_kmax[g] = a.kTmax[g];
     }
     __syncthreads();
-    auto kofs_of = [&](int g) -> int64_t { return g < 256 ? s_kofs[g] : a.kTofs[g]; };
-    auto kmax_of = [&](int g) -> double { return g < 256 ? s_kmax[g] : a.kTmax[g]; };
+    auto kofs_of = [&](int g) -> int64_t { return g < 128 ? s_kofs[g] : a.kTofs[g]; };
+    auto kmax_of = [&](int g) -> double { return g < 128 ? s_kmax[g] : a.kTmax[g]; };
+    int32_t *sB = s_B + wv * BSZ;
+    int32_t bp = -1;                               // p-vector whose Binter sB holds
+    int64_t bB = 0;                                // its max|Binter|
     unsigned long long t_exp = 0, t_ns = 0, t_und = 0;
     for (int64_t rd = rd0; rd < rd1; ++rd) {
         const int64_t tl = (rd * 4 + wv) * R + L.r;
@@ -1313,15 +1320,49 @@ __global__ __launch_bounds__(256) void k_filter(FiltArgs a)
             const int64_t *bi = rec + a.L.Bi + (int64_t)i * n;
             const int32_t *b32 = (const int32_t *)bi;
             const bool row = need && i < h;
-            const int64_t bmax = rec[14];
+            // the wavefront's points usually share one p-vector: its Binter
+            // (|entries| < 2^31) is copied once into the wavefront's LDS
+            // slice and the rows are read from there
+            bool lb = false;
+            if constexpr (NMAX <= 24) {
+                const int32_t p0 = __builtin_amdgcn_readfirstlane(need ? (int32_t)pi : -1);
+                if (p0 >= 0 && p0 != bp) {                 // wave-uniform
+                    const int64_t *r0 = a.recs + (int64_t)p0 * a.L.words;
+                    const int h0 = __builtin_amdgcn_readfirstlane(h);   // lane 0 holds p0's point
+                    const int64_t b0 = r0[14];
+                    wave_lds_sync();                       // the old copy's reads
+                    if (b0 < ((int64_t)1 << 31) && h0 >= 2 && h0 - 1 <= NMAX) {
+                        for (int e = L.lane; e < h0 * (h0 - 1); e += 64) sB[e] = (int32_t)r0[a.L.Bi + e];
+                        bp = p0;
+                        bB = b0;
+                    } else {
+                        bp = -1;
+                    }
+                    wave_lds_sync();
+                }
+                lb = need && bp >= 0 && pi == (int64_t)bp;
+            }
+            const int64_t bmax = lb ? bB : rec[14];
             const bool r32 = row && bmax < ((int64_t)1 << 31);
             double acc = 0.0, csum = 0.0;
+            if (!__ballot(row && !lb)) {
+                // every row from LDS
+                const int32_t *br = sB + (i < h ? i : 0) * n;
 #pragma unroll 4
-            for (int t = 0; t < NMAX; ++t) {
-                const int32_t ct = __shfl(cv, L.src0 + t);
-                const double cd = t < n ? (double)ct : 0.0;
-                csum += fabs(cd);
-                if (r32 && t < n) acc = fma((double)b32[2 * t], cd, acc);
+                for (int t = 0; t < NMAX; ++t) {
+                    const int32_t ct = __shfl(cv, L.src0 + t);
+                    const double cd = t < n ? (double)ct : 0.0;
+                    csum += fabs(cd);
+                    if (row && t < n) acc = fma((double)br[t], cd, acc);
+                }
+            } else {
+#pragma unroll 4
+                for (int t = 0; t < NMAX; ++t) {
+                    const int32_t ct = __shfl(cv, L.src0 + t);
+                    const double cd = t < n ? (double)ct : 0.0;
+                    csum += fabs(cd);
+                    if (r32 && t < n) acc = fma((double)b32[2 * t], cd, acc);
+                }
             }
             const bool fast = r32 && (double)bmax * csum < PFA_TWO53;
             Mi = fast ? (int64_t)acc : 0;
