@@ -1686,6 +1686,7 @@ struct DevCtx {
     size_t lds_max = 0;                    // per workgroup
     size_t lds_cu = 0;                     // per CU
     unsigned long long pcap_hint = 0;      // largest point count seen
+    unsigned long long pts_reserved = 0;   // point slots pfa_reserve set aside (0: none)
     double ppp_hint = 0;                   // largest points per p-vector seen (n_p >= 64)
     double ms_alloc = 0;                   // this run's buffer growth (host ms)
 };
@@ -2383,8 +2384,12 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     // points buffer: grow and rerun the search on overflow
     unsigned long long pcap = std::max<unsigned long long>(
         std::max<int64_t>(1 << 16, 16 * n_p), C->pcap_hint + C->pcap_hint / 8);
-    pcap = std::max<unsigned long long>(
-        pcap, (unsigned long long)std::min(1.25 * C->ppp_hint * (double)n_p, (double)(1 << 26)));
+    unsigned long long est =
+        (unsigned long long)std::min(1.25 * C->ppp_hint * (double)n_p, (double)(1 << 26));
+    // after pfa_reserve the per-p-vector estimate stays within the
+    // reservation (a search that overflows grows the buffer and reruns)
+    if (C->pts_reserved > 0) est = std::min(est, std::max(C->pts_reserved, pcap));
+    pcap = std::max(pcap, est);
     // defaults measured on MI355X (h11 = 20 batch): fewer, deeper wavefronts
     // keep the stacks' hot tops in L2
     int waves_per_cu = in->waves_per_cu > 0 ? in->waves_per_cu : 4;
@@ -2982,6 +2987,7 @@ extern "C" int pfa_reserve(int device, int64_t n_p, int64_t n_points, int h_max,
         dev_get(C, B_PREPCOLD, pfa_ws_cold_bytes(h_max) * std::min<size_t>(np1, 16384), errbuf);
         // a search sizes its point buffer at 9/8 of the largest count seen
         const size_t pc = n1 + n1 / 8 + 1;
+        C->pts_reserved = std::max<unsigned long long>(C->pts_reserved, pc);
         dev_get(C, B_PT_PIDX, 4 * pc, errbuf);
         dev_get(C, B_PT_Q, 8 * pc, errbuf);
         dev_get(C, B_PT_C, 4 * pc * D, errbuf);
