@@ -196,3 +196,27 @@ def test_gpu_serving_results_same_alone_and_in_mix(need_gpu):
             assert exp <= got, j.id
             n_pfvs += len(Ks)
     assert n_pfvs > 0
+
+
+@pytest.mark.gpu
+def test_gpu_reserved_server_allocates_nothing(need_gpu):
+    """After pfa_reserve at start, no iteration of a mixed-h11 stream grows a
+    point, record, list or staging buffer (those took 100+ ms; only the
+    small per-geometry tables are allocated at first use, well under 10 ms)
+    and no search reruns for lack of point slots."""
+    from pfvs_amd.instances import serving_request
+    srv = Server(max_batch_p=512, keep_points="checksum", reserve=(4096, 2_000_000, 41)).start()
+    jobs = []
+    for r in range(12):
+        inst, ps, qmax = serving_request(r, n_max=256)
+        qmax = np.broadcast_to(np.asarray(qmax, np.int64), (len(ps),)).copy()
+        jobs.append(srv.submit(inst.data.kappa_cob, inst.data.M_lattice(), inst.Q, ps, qmax))
+    for j in jobs:
+        j.wait(timeout=120)
+    srv.stop(timeout=30)
+    st = srv.stats()
+    print(f"serving: {st}", flush=True)
+    assert st["errors"] == 0
+    its = [it[5] for it in srv.iterations]         # per-iteration stage stats
+    assert its and all(ms["ms_alloc"] < 10.0 for ms in its), [ms["ms_alloc"] for ms in its]
+    assert all(ms["attempts"] == 1.0 for ms in its), [ms["attempts"] for ms in its]
