@@ -160,9 +160,31 @@ PFA_HD static inline int64_t pfa_fdiv64(int64_t a, int64_t b)
     return q;
 }
 
+// a / b and a % b on int128 (C semantics: the quotient truncated), through
+// the int64 operations when both operands fit (the GPU has no integer
+// divider: an int128 division is a long shift-subtract loop, an int64 one a
+// short inline sequence); INT64_MIN / -1 stays on the int128 path
+PFA_HD static inline bool pfa_div_in64(pfa_i128 a, pfa_i128 b)
+{
+    const pfa_i128 lo = -((pfa_i128)1 << 63), hi = ((pfa_i128)1 << 63) - 1;
+    return a >= lo && a <= hi && b >= lo && b <= hi && !(a == lo && b == -1);
+}
+
+PFA_HD static inline pfa_i128 pfa_tdiv128(pfa_i128 a, pfa_i128 b)
+{
+    if (pfa_div_in64(a, b)) return (pfa_i128)((int64_t)a / (int64_t)b);
+    return a / b;
+}
+
+PFA_HD static inline pfa_i128 pfa_tmod128(pfa_i128 a, pfa_i128 b)
+{
+    if (pfa_div_in64(a, b)) return (pfa_i128)((int64_t)a % (int64_t)b);
+    return a % b;
+}
+
 PFA_HD static inline pfa_i128 pfa_fdiv128(pfa_i128 a, pfa_i128 b)
 {
-    pfa_i128 q = a / b;
-    if ((a % b != 0) && ((a < 0) != (b < 0))) q--;
+    pfa_i128 q = pfa_tdiv128(a, b);
+    if ((pfa_tmod128(a, b) != 0) && ((a < 0) != (b < 0))) q--;
     return q;
 }

@@ -670,21 +670,21 @@ PFA_HD static inline int pfa_hnf(const PfaTeam &T, const PfaWS &W, const int64_t
             pfa_i128 *hr = &H[t * n];
             pfa_i128 a = hr[j], b = v[j];
             int bad = 0;
-            if (b % a == 0) {
-                pfa_i128 q = b / a;
+            if (pfa_tmod128(b, a) == 0) {
+                pfa_i128 q = pfa_tdiv128(b, a);
                 PFA_SYNC();
                 bad = pfa_hnf_rowsub(T, v, hr, q, j, n);
                 if (pfa_team_any(T, W.flag, bad)) return -1;
             } else {
                 pfa_i128 r0 = a, r1 = b, s0 = 1, s1 = 0, x0 = 0, x1 = 1;
                 while (r1) {
-                    pfa_i128 q = r0 / r1, tt;
+                    pfa_i128 q = pfa_tdiv128(r0, r1), tt;
                     tt = r0 - q * r1; r0 = r1; r1 = tt;
                     tt = s0 - q * s1; s0 = s1; s1 = tt;
                     tt = x0 - q * x1; x0 = x1; x1 = tt;
                 }
                 if (r0 < 0) { r0 = -r0; s0 = -s0; x0 = -x0; }
-                pfa_i128 ag = a / r0, bg = b / r0;
+                pfa_i128 ag = pfa_tdiv128(a, r0), bg = pfa_tdiv128(b, r0);
                 PFA_SYNC();
                 for (int c = j + T.lane; c < n; c += T.n) {
                     pfa_i128 hn = 0, vn = 0, t1 = 0;

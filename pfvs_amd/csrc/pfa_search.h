@@ -175,11 +175,11 @@ PFA_HD static inline int pfa_enter(const PfaProb &P, const int32_t *c, PfaNode &
         pfa_i128 need = (pfa_i128)P.linmin - nd.m0p;
         pfa_i128 l = P.linvec[i];
         if (l > 0) {
-            pfa_i128 b = need >= 0 ? (need + l - 1) / l : -((-need) / l);
+            pfa_i128 b = need >= 0 ? pfa_tdiv128(need + l - 1, l) : -pfa_tdiv128(-need, l);
             if (b > lo) lo = b > hi ? hi + 1 : (int32_t)b;
         } else {
             pfa_i128 nl = -l;
-            pfa_i128 b = need <= 0 ? (-need) / nl : -((need + nl - 1) / nl);
+            pfa_i128 b = need <= 0 ? pfa_tdiv128(-need, nl) : -pfa_tdiv128(need + nl - 1, nl);
             if (b < hi) hi = b < lo ? lo - 1 : (int32_t)b;
         }
     }
