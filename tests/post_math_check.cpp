@@ -96,6 +96,36 @@ int main()
             cases += 2;
         }
     }
+    // int128 division through int64 when both operands fit (pfa_tdiv128,
+    // pfa_tmod128, pfa_fdiv128) against the int128 operators: random
+    // magnitudes up to 2^100, both signs, and the boundary values
+    {
+        auto big = [&]() {
+            const int bits = (int)uni(0, 100);
+            pfa_i128 v = (pfa_i128)(rng() >> 1);
+            if (bits > 63) v = (v << (bits - 63)) ^ (pfa_i128)rng();
+            else v >>= (63 - bits);
+            return (rng() & 1) ? -v : v;
+        };
+        const pfa_i128 m63 = -((pfa_i128)1 << 63), p63 = ((pfa_i128)1 << 63) - 1;
+        const pfa_i128 edge[] = {0, 1, -1, 2, -2, 7, -7, m63, m63 + 1, p63, p63 - 1, p63 + 1, m63 - 1,
+                                 (pfa_i128)1 << 64, -((pfa_i128)1 << 64)};
+        auto check = [&](pfa_i128 a, pfa_i128 b) -> bool {
+            if (b == 0) return true;
+            const pfa_i128 q = a / b, r = a % b;
+            pfa_i128 f = q;
+            if (r != 0 && ((a < 0) != (b < 0))) f--;
+            ++cases;
+            return pfa_tdiv128(a, b) == q && pfa_tmod128(a, b) == r && pfa_fdiv128(a, b) == f;
+        };
+        for (pfa_i128 a : edge)
+            for (pfa_i128 b : edge)
+                if (!check(a, b)) return fail("div128 edge", (long long)a, (long long)(a >> 64), (long long)b, (long long)(b >> 64));
+        for (int it = 0; it < 200000; ++it) {
+            const pfa_i128 a = big(), b = big();
+            if (!check(a, b)) return fail("div128", (long long)a, (long long)(a >> 64), (long long)b, (long long)(b >> 64));
+        }
+    }
     std::printf("ok %ld\n", cases);
     return 0;
 }

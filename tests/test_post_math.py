@@ -1,7 +1,8 @@
 """The post-processing arithmetic of pfvs_amd/csrc/pfa_post.h (the fast
 int64 expansion count, floor division, bounded dot products, Montgomery
-products and signed reduction) against its int128 / plain reference forms,
-compiled for the host with g++ (tests/post_math_check.cpp)."""
+products and signed reduction) and the int128 division of pfa_int.h (int64
+operations when the operands fit) against their int128 / plain reference
+forms, compiled for the host with g++ (tests/post_math_check.cpp)."""
 
 import os
 import shutil
