@@ -866,22 +866,24 @@ PFA_HD static inline int kappa_M(const int64_t *kappa, int h, const int64_t *M, 
 // ------------------------------------------------------------ k_filter
 // det((kappa M)[1:, 1:]) != 0, exactly, for the listed points.  A
 // wavefront holds R = 64 / hl points (hl: the largest h of the launch);
-// lane i of a group computes M_i and owns column i of N (n = h - 1
-// columns, rows s < NMAX in registers).  Per point:
-//   1. N from kappa as int32 (|kappa| < 2^31, else step 3), staged in LDS
-//      for the geometry of the block's contiguous span of points: with
-//      24-bit integer multiply-adds when sum|M| < 2^23 and
+// lane i of a group computes M_i and owns row i of N (n = h - 1 rows; the
+// row's entries of the pivot columns so far in registers).  Per point:
+//   1. the columns of N from kappa as int32 (|kappa| < 2^31, else step 3),
+//      staged in LDS for the geometry of the block's contiguous span of
+//      points: with 24-bit integer multiply-adds when sum|M| < 2^23 and
 //      max|kappa| sum|M| < 2^31 (every partial sum then fits int32), else
 //      in double when max|kappa| sum|M| < 2^52 (every product and partial
 //      sum is then an exact integer), else step 3;
-//   2. fraction-free elimination (Bareiss) in double, every product and
-//      difference checked below 2^53 so each value is the exact integer
-//      minor: a column without a pivot among the unused rows proves
-//      det = 0, n pivots prove det != 0.  The exact division by the
-//      previous pivot is a product with its rounded reciprocal, rounded to
-//      the nearest integer: y = x fl(1/prev) is within |q| 2^-52 (1 + 2^-52)
-//      of the integer quotient q, so rint(y) = q whenever |y| < 2^50
-//      (checked with the products);
+//   2. fraction-free elimination (Bareiss) in double, left-looking: column
+//      k is built at step k and updated by the steps before it, every
+//      product and difference checked below 2^53 so each value is the
+//      exact integer minor: a column without a pivot among the unused rows
+//      proves det = 0 (a rank-r matrix builds r + 1 columns), n pivots
+//      prove det != 0.  The exact division by the previous pivot is a
+//      product with its rounded reciprocal, rounded to the nearest integer:
+//      y = x fl(1/prev) is within |q| 2^-52 (1 + 2^-52) of the integer
+//      quotient q, so rint(y) = q whenever |y| < 2^50 (checked with the
+//      products);
 //   3. points whose values reach 2^53 go on a second list (with their M)
 //      for k_det_mod: N in int128 from the int64 kappa (an entry outside
 //      int64 leaves the point undecided, 2), then det modulo 31-bit primes
