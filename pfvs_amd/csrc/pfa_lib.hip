@@ -2,21 +2,29 @@
 //
 // Pipeline per call (one HIP stream, inputs uploaded once):
 //   k_prep      one wavefront (64 lanes; the LLL/HNF working set in LDS,
-//               the matrix products' in global memory) per p-vector:
-//               lattice, HNF, FP64 Cholesky with its error margin, record.
-//   k_roots     one lane per p-vector: top-level node -> frontier queue.
+//               8.9 KB at h11 = 20, the matrix products' in global memory)
+//               per p-vector, one launch per h11 size class: lattice, HNF,
+//               FP64 Cholesky with its error margin, record.
+//   k_roots     one lane per p-vector: top-level node -> frontier ring.
 //   search      PLAIN: k_plain, one lane per p-vector runs the whole
 //               depth-first search (the one-launch-per-batch form).
 //               BATCHED: k_batched, persistent wavefronts each holding a
-//               LIFO stack of nodes; every step expands up to 64 sibling
-//               candidates of the top nodes at once (prefix sum over their
-//               ranges), pushes surviving children with a ballot, writes
-//               leaves with one atomic per wavefront; after `budget` steps
-//               a wavefront donates its stack to the global frontier ring,
-//               which the next pass redistributes over all wavefronts.
-//   k_post      one lane per point: M = Binter c, Kn = ZB c (checked),
-//               the K-integrality count; k_det: det((kappa M)[1:,1:]) != 0.
-// The host path runs the same preparation and search code on host threads.
+//               LIFO deque of nodes (its top 128 entries cached in LDS);
+//               every step expands up to 64 sibling candidates of the top
+//               nodes at once (prefix sum over their ranges), pushes
+//               surviving children with a ballot and writes leaves into
+//               chunks of point slots; deep deques donate their bottom to
+//               a ticketed frontier ring that idle wavefronts take from;
+//               an outstanding-work counter ends the kernel.
+//   k_post      one lane per point slot: Kn = ZB c and M0 (exact double
+//               FMAs, rows through the scalar cache), the K-integrality
+//               count, the list of kept points.
+//   k_filter    det((kappa M)[1:,1:]) != 0 for the kept points, one group
+//               of h11 - 1 lanes per point (left-looking Bareiss);
+//               k_det_mod decides the points whose values left 2^53.
+// The host path runs the same preparation and search code on host threads;
+// the finish (canonical order, copy-out, PFV assembly) runs on host threads
+// after the device is released.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
