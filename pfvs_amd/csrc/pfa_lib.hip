@@ -79,45 +79,15 @@ static RecL rec_layout(int D)
 // [3] linmin [4..8] Qe QGmul phi slack Kerr (double) [9] hmask
 // [14] max |Binter| [15] max |ZB| [16] max |mat| [17] max |H| of the
 // pruning rows (int64 bounds for the search's exact prefix sums)
-PFA_HD static inline PfaProb rec_view(const int64_t *r, const RecL &L)
+// A record's header fields and row pointers.  H64 / H32: the types the
+// header is read through (plain, or the constant address space for a
+// wave-uniform record on the device: the header does not change during a
+// search, only the status word, which is not read here).
+template <class H64, class H32>
+PFA_HD static inline PfaProb rec_view_as(const int64_t *r, const RecL &L)
 {
-    const int32_t *ri = (const int32_t *)r;
-    const double *rd = (const double *)r;
-    PfaProb P;
-    P.d = ri[1];
-    P.m0_level = ri[20];
-    P.leafg = ri[21];
-    P.nL = ri[22];
-    P.strict = ri[23];
-    P.qmax = r[1];
-    P.Q = r[2];
-    P.linmin = r[3];
-    P.Qe = rd[4];
-    P.QGmul = rd[5];
-    P.phi = rd[6];
-    P.slack = rd[7];
-    P.Kerr = rd[8];
-    P.hmask = (uint64_t)r[9];
-    P.matmax = r[16];
-    P.hmax = r[17];
-    P.U = (const double *)(r + L.U);
-    P.Uinv = (const double *)(r + L.Uinv);
-    P.mat = r + L.mat;
-    P.linvec = r + L.lin;
-    P.H = r + L.H;
-    P.L = r + L.L;
-    return P;
-}
-
-// The same view for a wave-uniform record: the header through the scalar
-// cache (constant address space; the header does not change during a search,
-// only the status word, which is not read here), the row pointers uniform.
-typedef __attribute__((address_space(4))) const int64_t pfa_chdr64;
-typedef __attribute__((address_space(4))) const int32_t pfa_chdr32;
-__device__ static inline PfaProb rec_view_uni(const int64_t *r, const RecL &L)
-{
-    const pfa_chdr64 *r64 = (const pfa_chdr64 *)r;
-    const pfa_chdr32 *ri = (const pfa_chdr32 *)r;
+    const H64 *r64 = (const H64 *)r;
+    const H32 *ri = (const H32 *)r;
     PfaProb P;
     P.d = ri[1];
     P.m0_level = ri[20];
@@ -127,11 +97,11 @@ __device__ static inline PfaProb rec_view_uni(const int64_t *r, const RecL &L)
     P.qmax = r64[1];
     P.Q = r64[2];
     P.linmin = r64[3];
-    P.Qe = __builtin_bit_cast(double, r64[4]);
-    P.QGmul = __builtin_bit_cast(double, r64[5]);
-    P.phi = __builtin_bit_cast(double, r64[6]);
-    P.slack = __builtin_bit_cast(double, r64[7]);
-    P.Kerr = __builtin_bit_cast(double, r64[8]);
+    P.Qe = __builtin_bit_cast(double, (int64_t)r64[4]);
+    P.QGmul = __builtin_bit_cast(double, (int64_t)r64[5]);
+    P.phi = __builtin_bit_cast(double, (int64_t)r64[6]);
+    P.slack = __builtin_bit_cast(double, (int64_t)r64[7]);
+    P.Kerr = __builtin_bit_cast(double, (int64_t)r64[8]);
     P.hmask = (uint64_t)r64[9];
     P.matmax = r64[16];
     P.hmax = r64[17];
@@ -142,6 +112,19 @@ __device__ static inline PfaProb rec_view_uni(const int64_t *r, const RecL &L)
     P.H = r + L.H;
     P.L = r + L.L;
     return P;
+}
+
+PFA_HD static inline PfaProb rec_view(const int64_t *r, const RecL &L)
+{
+    return rec_view_as<int64_t, int32_t>(r, L);
+}
+
+// through the scalar cache, for a record every lane of the wavefront reads
+typedef __attribute__((address_space(4))) const int64_t pfa_chdr64;
+typedef __attribute__((address_space(4))) const int32_t pfa_chdr32;
+__device__ static inline PfaProb rec_view_uni(const int64_t *r, const RecL &L)
+{
+    return rec_view_as<pfa_chdr64, pfa_chdr32>(r, L);
 }
 
 PFA_HD static inline int32_t &rec_status(int64_t *r) { return ((int32_t *)r)[0]; }
