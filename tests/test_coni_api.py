@@ -102,6 +102,71 @@ def test_gpu_matches_cpu(need_gpu, g, dil):
 
 
 @pytest.mark.gpu
+def test_gpu_matches_cpu_large_dilation(need_gpu):
+    """h11 7..11 at dilation 800 (reference tests/test_gpu.py:67-82): deep
+    trees, the first 300 p-vectors of each geometry."""
+    npts = 0
+    for g in [g for g in GEOMS if 7 <= g["h11"] <= 11]:
+        data = cydata_of(g)
+        ps = box_enum(g["B"], data.H_cob, 1, True)[:300]
+        if len(ps) == 0:
+            continue
+        print(f"h11={g['h11']} poly{g['polyID']} {len(ps)} p-vectors dil 800", flush=True)
+        Kc, Mc = coniZpM(data, ps, ellipsoid_dilation=800, device="cpu")
+        Kg, Mg = coniZpM(data, ps, ellipsoid_dilation=800, device="gpu")
+        np.testing.assert_array_equal(Kg, Kc)
+        np.testing.assert_array_equal(Mg, Mc)
+        npts += len(Kc)
+    assert npts > 0
+
+
+@pytest.mark.gpu
+def test_multi_geometry_batch_matches_single(need_gpu):
+    """One coni_batch_multi call over several geometries returns, per
+    geometry, exactly what a call for that geometry alone returns (reference
+    tests/test_gpu.py:85-112)."""
+    from pfvs_amd import gpu
+    geoms, ps_all, pgeo, singles = [], [], [], []
+    for g in GPU_GEOMS[:6]:
+        data = cydata_of(g)
+        ps = box_enum(g["B"], data.H_cob, 1, True)[:200]
+        if len(ps) == 0:
+            continue
+        p_full = np.hstack([np.zeros((len(ps), 1), dtype=np.int64), ps])
+        spec = dict(kappa=data.kappa_cob, Mbasis=data.M_lattice(),
+                    Q=g["h11"] + g["h21"] + 4, dilation=150, M0min=13)
+        k = len(geoms)
+        geoms.append(spec)
+        ps_all += list(p_full)
+        pgeo += [k] * len(p_full)
+        singles.append(gpu.coni_batch_multi([spec], p_full, np.zeros(len(p_full), dtype=np.int32)))
+    assert len(geoms) >= 2
+    print(f"{len(geoms)} geometries, {len(ps_all)} p-vectors", flush=True)
+    M, Kn, q, pidx, pstat, _ = gpu.coni_batch_multi(geoms, ps_all, np.array(pgeo))
+    off = 0
+    for M1, K1, q1, p1, s1, _ in singles:
+        n = len(s1)
+        sel = (pidx >= off) & (pidx < off + n)
+        np.testing.assert_array_equal(M[sel], M1)
+        np.testing.assert_array_equal(Kn[sel], K1)
+        np.testing.assert_array_equal(q[sel], q1)
+        np.testing.assert_array_equal(pidx[sel] - off, p1)
+        np.testing.assert_array_equal(pstat[off:off + n], s1)
+        off += n
+    assert off == len(ps_all) and len(pidx) == sum(len(s[3]) for s in singles)
+
+
+def test_gpu_required_raises_outside_the_path():
+    """device='gpu' with a lattice option the device path does not take
+    raises, as pfvs/coni.py:419-443 does (reference tests/test_gpu.py:115-121)."""
+    g = SMALL[0]
+    data = cydata_of(g)
+    ps = box_enum(g["B"], data.H_cob, 1, True)
+    with pytest.raises(RuntimeError, match="device='gpu' unavailable.*use_c_lattice"):
+        coniZpM(data, ps, device="gpu", use_c_lattice=False)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("n,coord,seed,dil,n_pfv", [(64, 12, 1003, 4, 0), (8, 6, 77, 150, 1)])
 def test_gpu_G_matches_oracle_postprocessing(need_gpu, n, coord, seed, dil, n_pfv):
     """G(10, 150, 3): a batch with no PFVs, and one with a few hundred."""
