@@ -148,8 +148,9 @@ void pfa_run_output_free(pfa_run_output *out);
  * int64), leaf rows L (nl[k] rows, int64; used when some H rows were
  * dropped for not fitting int64), linvec (d, or has_lin[k] = 0), Q, qmax,
  * linmin, strict.  Arrays of all problems are concatenated.  Outputs the
- * points in depth-first order per problem (pts int32 (n, dmax), qs, pidx)
- * and per-problem status and node counts. */
+ * points in depth-first order per problem (pts int32 (n, dmax), qs, pidx),
+ * per-problem status, and per-problem node counts (host mode; the device
+ * kernels count nodes only in total, and report 0 here). */
 typedef struct {
     int n_prob;
     const int32_t *dims, *nh, *nl, *has_lin, *strict;

@@ -55,7 +55,9 @@ def _use_gpu(device, h11, n_ps, use_c_lattice=True, use_gcd_lattice=False, extra
 
 
 def _raise_kernel_status(p, status: int):
-    reason = {-2: "(more than max_N_pfvs outputs; increase max_N_pfvs)",
+    reason = {1: "(beyond the exact search's int64/int128 range: an ellipsoid matrix "
+                 "entry above 2^53 or a leaf gcd beyond int128)",
+              -2: "(more than max_N_pfvs outputs; increase max_N_pfvs)",
               -8: "(lattice-point coordinates exceed int32)",
               -9: "(the ellipsoid matrix is not positive definite, so its "
                   "lattice points cannot be enumerated)"}.get(status, "")
@@ -225,7 +227,7 @@ def _exact_host_path(kappa, Mbasis, p_full, Q, qmax, M0min, max_N):
                 linmin=M0min, strict=True, H=np.array(small, dtype=np.int64).reshape(-1, d), L=big)
     res, st, _ = gpu.kernel_batch([prob], mode="host")
     if st[0] != 0:
-        _raise_kernel_status(p_full[1:], int(st[0]) if st[0] < 0 else -9)
+        _raise_kernel_status(p_full[1:], int(st[0]))
     pts, qs = res[0]
     if len(pts) > max_N:
         _raise_kernel_status(p_full[1:], -2)

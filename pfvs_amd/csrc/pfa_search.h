@@ -263,7 +263,18 @@ PFA_HD static inline pfa_u128 pfa_leaf_gcd(const PfaProb &P, const int32_t *c, p
     }
     pfa_u128 g = g0;
     for (int r = 0; r < P.nL; ++r) {
-        if (g == 0 || (g >> 62) != 0) { *ok = 0; return 0; }
+        if (g == 0) {
+            // nothing to reduce by: the row's value itself, checked in int128
+            const int64_t *Lr = P.L + (size_t)r * d * 2;
+            pfa_i128 s = 0, t;
+            for (int j = 0; j < d; ++j) {
+                pfa_i128 e = (pfa_i128)(((pfa_u128)(uint64_t)Lr[2 * j + 1] << 64) | (uint64_t)Lr[2 * j]);
+                if (pfa_mul_ovf(e, (pfa_i128)c[j], &t) || pfa_add_ovf(s, t, &s)) { *ok = 0; return 0; }
+            }
+            g = pfa_abs128(s);
+            continue;
+        }
+        if ((g >> 62) != 0) { *ok = 0; return 0; }
         const pfa_i128 m = (pfa_i128)g;
         const int64_t *Lr = P.L + (size_t)r * d * 2;
         pfa_i128 s = 0;

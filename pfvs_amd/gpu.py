@@ -20,6 +20,8 @@ import os
 
 import numpy as np
 
+from . import lattice
+
 _i64p = ctypes.POINTER(ctypes.c_int64)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i8p = ctypes.POINTER(ctypes.c_int8)
@@ -408,6 +410,78 @@ def run(geoms, ps, pgeo, mode="batched", device=0, waves_per_cu=0, budget=0, sta
     return res
 
 
+def _echelon_rows(p):
+    """The problem with H in echelon form.  g = gcd(H c) does not change
+    under unimodular row operations on H, so a general H (repeated leading
+    columns, which the reference's contract accepts: fp_kernel.h:40-44) is
+    replaced by the row HNF of all its rows, H and L together: the same cut,
+    pruned at least as early.  Rows of the HNF that fit int64 prune, wider
+    ones join the gcd at the leaves."""
+    d = len(p["mat"])
+    H = np.asarray(p.get("H", np.zeros((0, d))), dtype=np.int64).reshape(-1, d)
+    lead = [int(np.flatnonzero(r)[0]) for r in H if r.any()]
+    if len(lead) == len(set(lead)):
+        return p
+    rows = [[int(x) for x in r] for r in H] + [[int(x) for x in r] for r in p.get("L", [])]
+    Hn = lattice.hnf_rows(rows)
+    small = [r for r in Hn if any(r) and all(abs(x) < 2**63 for x in r)]
+    big = [r for r in Hn if not all(abs(x) < 2**63 for x in r)]
+    return dict(p, H=np.array(small, dtype=np.int64).reshape(-1, d), L=big)
+
+
+def _reduced_problem(p):
+    """(problem in an LLL-reduced basis of its form, T).  With c = T c'
+    (T unimodular, columns the reduced vectors): q(c) = c'^T (T^T mat T) c',
+    linvec.c = (T^T linvec).c', gcd(H c) = gcd((H T) c'), so the point sets
+    correspond one to one with the same q.  An ill-conditioned mat (the FP64
+    factor's margin too wide, status 1) is well-conditioned in this basis."""
+    d = len(p["mat"])
+    mat = [[int(x) for x in r] for r in np.asarray(p["mat"]).tolist()]
+    T = lattice.lll_gram_exact(mat)                              # columns = reduced vectors
+    matT = lattice.matmul(lattice.matmul(lattice.transpose(T), mat), T)
+    rows = [[int(x) for x in r] for r in np.asarray(p.get("H", np.zeros((0, d)))).reshape(-1, d).tolist()]
+    rows += [[int(x) for x in r] for r in p.get("L", [])]
+    rows = [r for r in lattice.hnf_rows(lattice.matmul(rows, T)) if any(r)] if rows else []
+    lin = p.get("linvec")
+    q = dict(p, mat=matT,
+             linvec=None if lin is None else [sum(T[i][k] * int(lin[i]) for i in range(d)) for k in range(d)],
+             H=np.array([r for r in rows if all(abs(x) < 2**63 for x in r)], dtype=object).reshape(-1, d),
+             L=[r for r in rows if not all(abs(x) < 2**63 for x in r)])
+    if any(abs(x) >= 2**63 for r in matT for x in r) or \
+            (q["linvec"] is not None and any(abs(x) >= 2**63 for x in q["linvec"])):
+        return None, None
+    q["H"] = q["H"].astype(np.int64)
+    return q, T
+
+
+def _retry_reduced(problems, res, status, nodes, mode, device):
+    """Problems with status 1 searched again in an LLL-reduced basis; their
+    points are mapped back (c = T c') and put in the search's order (last
+    coordinate outermost, each ascending), which is lexicographic order on
+    the reversed coordinates.  Points beyond int32 give status -8."""
+    idx = [k for k in range(len(problems)) if status[k] == 1 and len(problems[k]["mat"]) > 1]
+    red = [(k,) + _reduced_problem(problems[k]) for k in idx]
+    red = [(k, q, T) for k, q, T in red if q is not None]
+    if not red:
+        return
+    r2, s2, n2 = _kernel_call([q for _, q, _ in red], mode, device)
+    for (k, _, T), (pts, qs), s, n in zip(red, r2, s2, n2):
+        if s != 0:
+            status[k] = s
+            continue
+        C = np.asarray(pts, dtype=object) @ np.array(T, dtype=object).T if len(pts) else \
+            np.zeros((0, len(T)), dtype=object)
+        if len(C) and max(abs(int(x)) for x in C.ravel()) >= 2**31:
+            status[k] = -8
+            res[k] = (np.zeros((0, len(T)), np.int32), np.zeros(0, np.int64))
+            continue
+        order = sorted(range(len(C)), key=lambda i: tuple(int(x) for x in C[i][::-1]))
+        res[k] = (np.array([[int(x) for x in C[i]] for i in order], dtype=np.int32).reshape(-1, len(T)),
+                  np.asarray(qs)[order].astype(np.int64))
+        status[k] = 0
+        nodes[k] = n
+
+
 def kernel_batch(problems, mode="plain", device=0):
     """Single-problem searches (the conipfv_kernel contract) in one call.
 
@@ -415,7 +489,20 @@ def kernel_batch(problems, mode="plain", device=0):
     linmin, strict, H (echelon rows fitting int64) and optionally L (rows
     as Python ints that need more than int64, combined with H's gcd at the
     leaves).  Returns (list of (pts int32 (n, d), qs int64) in depth-first
-    order, statuses, node counts)."""
+    order, statuses, node counts (per problem in host mode; 0 on the device)).  H may be any integer matrix (rows
+    with repeated leading columns are brought to echelon form first).
+    A problem whose FP64 factor margin is too wide (an ill-conditioned
+    mat) is searched again in an LLL-reduced basis of its form.  Status 1
+    that remains: the problem needs an exact search beyond this one (mat
+    above 2^53, or a leaf gcd beyond int128)."""
+    problems = [_echelon_rows(p) for p in problems]
+    res, status, nodes = _kernel_call(problems, mode, device)
+    _retry_reduced(problems, res, status, nodes, mode, device)
+    return res, status, nodes
+
+
+def _kernel_call(problems, mode, device):
+    """One pfa_kernel_batch call over problems with H in echelon form."""
     lib = require()
     n = len(problems)
     dims = np.array([len(p["mat"]) for p in problems], dtype=np.int32)

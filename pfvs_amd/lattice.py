@@ -12,6 +12,7 @@ PFV set, so only the lattices (and the canonical HNF) must agree.
 from __future__ import annotations
 
 import math
+from fractions import Fraction
 from typing import List, Sequence
 
 Matrix = List[List[int]]
@@ -176,6 +177,58 @@ def lll_rows(B: Matrix, gram: Matrix | None = None, last: List[int] | None = Non
                 break
     out = rows[:m]
     return out, (rows[m] if last is not None else None)
+
+
+def lll_gram_exact(G: Matrix, delta: Fraction = Fraction(99, 100)) -> Matrix:
+    """Unimodular T (columns = the reduced vectors, in the original
+    coordinates) with T^T G T LLL-reduced for the positive definite form G,
+    in exact rationals throughout (Cohen, A Course in Computational
+    Algebraic Number Theory, Alg. 2.6.3, on the Gram matrix): no
+    conditioning limit, for the rare forms the floating-point
+    Gram-Schmidt of lll_rows cannot take.  The Gram-Schmidt data are
+    recomputed after each swap (O(n^3) per swap)."""
+    n = len(G)
+    Bv = [[int(i == j) for j in range(n)] for i in range(n)]      # rows b_k
+    GG = [[int(x) for x in r] for r in G]
+
+    def gs():
+        mu = [[Fraction(0)] * n for _ in range(n)]
+        bb = [Fraction(0)] * n
+        for i in range(n):
+            for j in range(i):
+                t = Fraction(GG[i][j]) - sum(mu[j][l] * mu[i][l] * bb[l] for l in range(j))
+                mu[i][j] = t / bb[j]
+            bb[i] = Fraction(GG[i][i]) - sum(mu[i][l] * mu[i][l] * bb[l] for l in range(i))
+            if bb[i] <= 0:
+                raise ValueError("the form is not positive definite")
+        return mu, bb
+
+    mu, bb = gs()
+    k = 1
+    while k < n:
+        for j in range(k - 1, -1, -1):
+            q = round(mu[k][j])
+            if q == 0:
+                continue
+            Bv[k] = [a - q * b for a, b in zip(Bv[k], Bv[j])]
+            gkk = GG[k][k] - 2 * q * GG[k][j] + q * q * GG[j][j]
+            for l in range(n):
+                if l != k:
+                    GG[k][l] = GG[l][k] = GG[k][l] - q * GG[j][l]
+            GG[k][k] = gkk
+            for l in range(j):
+                mu[k][l] -= q * mu[j][l]
+            mu[k][j] -= q
+        if bb[k] >= (delta - mu[k][k - 1] ** 2) * bb[k - 1]:
+            k += 1
+            continue
+        Bv[k], Bv[k - 1] = Bv[k - 1], Bv[k]
+        GG[k], GG[k - 1] = GG[k - 1], GG[k]
+        for r in GG:
+            r[k], r[k - 1] = r[k - 1], r[k]
+        mu, bb = gs()
+        k = max(k - 1, 1)
+    return [[Bv[c][i] for c in range(n)] for i in range(n)]
 
 
 def hnf_rows(A: Matrix) -> Matrix:
