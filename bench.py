@@ -1,19 +1,24 @@
-"""Benchmark: FP nodes/s of the batched persistent coniZpM search against the
-plain one-thread-per-p form, on the h11 = 20 member of the mixed batch
-(G(20, 250, 5), 4096 seeded p-vectors, cone coordinates <= 12, per-p
-dilation uniform in 1..4), with the oracle's CPU path timed on a bounded
-sample of the same p-vectors.
+"""Benchmark: FP nodes/s of the batched persistent coniZpM search on the
+h11 = 20 member of the mixed batch (G(20, 250, 5), 4096 seeded p-vectors,
+cone coordinates <= 12, per-p dilation uniform in 1..4), against the plain
+one-launch-per-batch form (one wavefront per p-vector, no work sharing) and
+the one-thread-per-p-vector form, with the oracle's CPU path timed on a
+bounded sample of the same p-vectors (one thread and all threads).
 
-One step = one pass of the device pipeline (preparation, frontier roots,
-search, post-processing with the exact filter) over the whole batch, inputs
-resident in HBM.  Prints one JSON line.  --workload mixed (all four
-instances of the mixed batch in one launch) and --workload large (deep
-h11 = 30 trees, reporting the exact-host fallback rate) print the same
-line for those batches.
+One step = one full coniZpM pass over the whole batch, inputs resident in
+HBM: preparation, search, the exact per-point filter, the canonical order
+and the PFV assembly in the library, and the exact host finish of what it
+leaves, ending with the final (K, M) rows on the host.  `value` is nodes
+over the wall time of a step.  Prints one JSON line.  --workload mixed
+(all four instances of the mixed batch in one launch), --workload large
+(deep h11 = 30 trees, with the exact-host fallback rate) and --workload
+serving (a Poisson stream through the continuous-batching server) print
+the same line for those workloads.  The headline line also carries
+coniPFVs/s on a workload that has PFVs (the Manwe scan).
 
-Roofline of the search kernel (k_batched): VALU issue.  Its algorithmic
-work W per launch is counted in lane-ops on the host path, which visits the
-same nodes (asserted):
+Roofline of the search kernel (k_search, batched form): VALU issue.  Its
+algorithmic work W per launch is counted in lane-ops on the host path,
+which visits the same nodes (asserted):
   W = sum_nodes [4k + 4k [i >= m0_level] + 4k r + 24]
     + sum_cand [12 + r (4 + 5 s)] + sum_leaf [4 d (d + 1) / 2 + 8]
 (k = d - 1 - i coordinates set above a node at level i, r = 1 where an H
@@ -78,18 +83,44 @@ def workload(name="headline", n_per=4096):
     raise SystemExit(f"unknown workload {name}")
 
 
+def full_pass(gpu, geoms, pf, pgeo, mode, **kw):
+    """One step: everything coniZpM does for the batch -- preparation,
+    search, the exact per-point filter, the canonical order and the PFV
+    assembly in the library, and the exact host finish of any p-vector the
+    library leaves (pstat 1 or pdone 0) -- from resident inputs to the final
+    (K, M) rows.  Returns (result, number of PFVs, p-vectors finished on the
+    host)."""
+    from pfvs_amd import coni
+    r = gpu.run(geoms, pf, pgeo, mode=mode, filter=True, sort=True, keep_points="survivors",
+                assemble=True, **kw)
+    n_pfv = len(r["pfv"]["pidx"])
+    left = np.flatnonzero((r["pstat"] != 0) | (r["pdone"] == 0))
+    for ip in left:                          # rare: exact host finish, one p-vector at a time
+        g = geoms[pgeo[ip]]
+        h = np.asarray(g["kappa"]).shape[0]
+        sub = {k: r[k][r["pidx"] == ip] for k in ("M", "Kn", "q", "nonsing")}
+        sub.update(pidx=np.zeros(len(sub["q"]), np.int64), pstat=r["pstat"][ip:ip + 1],
+                   pcount=r["pcount"][ip:ip + 1])
+        Ks, _ = coni.assemble(sub, pf[ip:ip + 1, 1:h], pf[ip:ip + 1, :h], g["kappa"], g["Mbasis"],
+                              g["Q"], np.array([gpu.qmax_of(g["Q"], g["dilation"])]), g["M0min"])
+        n_pfv += len(Ks)
+    return r, n_pfv, len(left)
+
+
 def time_mode(gpu, geoms, pf, pgeo, mode, steps, warmup, **kw):
-    """Each gpu.run call returns after its stream has drained (results are
-    copied back), so the wall clock brackets complete passes."""
+    """Wall time of `steps` full passes (each returns after its results are
+    on the host), after `warmup` untimed ones."""
     for _ in range(warmup):
-        gpu.run(geoms, pf, pgeo, mode=mode, keep_points=False, **kw)
+        full_pass(gpu, geoms, pf, pgeo, mode, **kw)
+    stats, walls = [], []
     t0 = time.perf_counter()
-    stats = []
     for _ in range(steps):
-        r = gpu.run(geoms, pf, pgeo, mode=mode, keep_points=False, **kw)
+        t1 = time.perf_counter()
+        r, n_pfv, n_left = full_pass(gpu, geoms, pf, pgeo, mode, **kw)
+        walls.append(time.perf_counter() - t1)
         stats.append(r["stats"])
     wall = (time.perf_counter() - t0) / steps
-    return wall, stats, r
+    return wall, stats, r, n_pfv, n_left, walls
 
 
 def search_work(gpu, geoms, pf, pgeo):
@@ -101,13 +132,15 @@ def search_work(gpu, geoms, pf, pgeo):
     return r["stats"]["work"], r["stats"]["nodes"], threads
 
 
+TRAFFIC_FILE = os.path.join("profiles", "k_batched_h20_hbm_traffic.md")
+
+
 def measured_traffic(kernel="k_batched"):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, calibrated with a copy kernel), or
-    None."""
-    path = os.path.join(ROOT, "profiles", "k_batched_h20_hbm_traffic.md")
+    """HBM bytes per launch of `kernel` as recorded in the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by
+    a copy kernel of known size), or None.  Not measured by this run."""
     try:
-        for line in open(path):
+        for line in open(os.path.join(ROOT, TRAFFIC_FILE)):
             if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
                 return float(line.split(":")[1])
     except OSError:
@@ -115,22 +148,73 @@ def measured_traffic(kernel="k_batched"):
     return None
 
 
-def cpu_baseline(items, pf, budget_s=15.0):
-    """The oracle (single thread) over p-vectors of the same batch, in a
-    fixed seeded order, until budget_s seconds have elapsed."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(items, pf, budget_s=8.0, threads=1):
+    """The oracle's CPU path (its lattice preparation and search, the points
+    and their transform) over p-vectors of the same batch in a fixed seeded
+    order, `threads` threads each taking the next p-vector, until budget_s
+    seconds have elapsed.  Returns (nodes/s, p-vectors done, seconds)."""
+    import threading
     from oracle import cpu
     order = np.random.default_rng(7).permutation(len(pf))
-    nodes = done = 0
+    lock = threading.Lock()
+    state = {"next": 0, "nodes": 0, "done": 0}
     t0 = time.perf_counter()
-    for i in order:
-        kap, Mb, Q, qmax = items[i]
-        M, K, q, pidx, st, cnt = cpu.coni_batch(kap, Mb, pf[i:i + 1, :kap.shape[0]], Q, qmax)
-        nodes += cnt[0]
-        done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
+
+    def worker():
+        while True:
+            with lock:
+                k = state["next"]
+                if k >= len(order) or time.perf_counter() - t0 > budget_s:
+                    return
+                state["next"] += 1
+            i = order[k]
+            kap, Mb, Q, qmax = items[i]
+            res = cpu.coni_batch(kap, Mb, pf[i:i + 1, :kap.shape[0]], Q, qmax)
+            with lock:
+                state["nodes"] += res[5][0]
+                state["done"] += 1
+    ths = [threading.Thread(target=worker) for _ in range(threads)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
     t = time.perf_counter() - t0
-    return nodes / t, done, t
+    return state["nodes"] / t, state["done"], t
+
+
+def pfv_rate(gpu, steps):
+    """coniPFVs/s on a workload that has PFVs: the Manwe geometry's scan
+    (every primitive p-vector of the smallest box holding 10,000 of them,
+    dilation 50, M0min 13, Q = h11 + h21 + 4; the reference's
+    tests/test_manwe.py known answer is 111 PFVs), as full coniZpM calls on
+    the device."""
+    from pfvs_amd import coniZpM
+    from pfvs_amd.cydata import CYData
+    from pfvs_amd.pvectors import pvecs
+    e = json.load(open(os.path.join(ROOT, "tests", "golden", "paper_coni.json")))[0]
+    data = CYData(e["h21"], e["kappa"], e["c2"], e["H"], coni_curve=e["coni_curve"],
+                  coni_cob=e["cob"])
+    ps = pvecs(data, min_N_pts=10_000)
+    call = lambda: coniZpM(data, ps, M0min=13, ellipsoid_dilation=50, max_N_pfvs=10_000_000,
+                           device="gpu")
+    call()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        Ks, Ms = call()
+    t = (time.perf_counter() - t0) / steps
+    return {"value": len(Ks) / t, "unit": "PFVs/s", "pfvs": len(Ks), "ms_per_call": t * 1e3,
+            "workload": f"Manwe (h11 = 8, h21 = 150) scan: {len(ps)} p-vectors, dilation 50, "
+                        "coniZpM(device='gpu') end to end"}
 
 
 def serving_bench(gpu, args):
@@ -204,7 +288,9 @@ def main():
     ap.add_argument("--rate", type=float, default=40.0, help="serving: arrivals per second")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-pfv", action="store_true", help="skip the coniPFVs/s workload")
+    ap.add_argument("--cpu-seconds", type=float, default=16.0,
+                    help="CPU baseline budget (half single-thread, half all threads)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the launch-parameter autotuner (library defaults)")
     ap.add_argument("--waves-per-cu", type=int, default=0,
@@ -242,61 +328,90 @@ def main():
         params = {k: best[k] for k in ("waves_per_cu", "budget", "stack_cap")}
         print(f"# autotuned: {params} ({best['ms_search']:.2f} ms search)", file=sys.stderr,
               flush=True)
-    wall_b, st_b, r_b = time_mode(gpu, geoms, pf, pgeo, "batched", args.steps, args.warmup,
-                                  **params)
+    wall_b, st_b, r_b, pfv_b, left_b, walls_b = time_mode(gpu, geoms, pf, pgeo, "batched",
+                                                          args.steps, args.warmup, **params)
     nodes = st_b[-1]["nodes"]
-    ms_dev = np.mean([s["ms_total"] for s in st_b])
-    ms_search = np.mean([s["ms_search"] for s in st_b])
-    print(f"# batched: {nodes:.0f} nodes, device {ms_dev:.2f} ms, search {ms_search:.2f} ms, "
-          f"wall {wall_b*1e3:.1f} ms, passes {st_b[-1]['passes']:.0f}", file=sys.stderr, flush=True)
+    mean = lambda k, st: float(np.mean([s[k] for s in st]))
+    ms_dev, ms_search = mean("ms_total", st_b), mean("ms_search", st_b)
+    print(f"# batched: {nodes:.0f} nodes, full pass {wall_b*1e3:.1f} ms (device {ms_dev:.2f} ms, "
+          f"search {ms_search:.2f} ms), {pfv_b} PFVs, host-finished {left_b}", file=sys.stderr,
+          flush=True)
     metric = {"headline": "FP nodes/s (batched persistent coniZpM, h11=20)",
               "mixed": "FP nodes/s (batched persistent coniZpM, mixed h11 = 6/10/14/20)",
               "large": "FP nodes/s (batched persistent coniZpM, deep h11 = 30 trees)"}[args.workload]
+    sb = st_b[-1]
     out = {
         "metric": metric,
-        "value": nodes / (ms_dev * 1e-3),
+        "value": nodes / wall_b,
         "unit": "nodes/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_dev,
+        "ms_per_step": wall_b * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "int64+f64", "data": "synthetic",
         "config": {"workload": desc, "parallelism": "none",
+                   "step": "one full coniZpM pass over the batch from resident inputs to the "
+                           "final (K, M) rows: preparation, search, exact filter, canonical "
+                           "order, PFV assembly, host finish of leftovers (wall clock)",
                    "search_params": params or "library defaults"},
-        "detail": {"nodes": nodes, "points": st_b[-1]["points"], "kept": st_b[-1]["kept"],
-                   "pfvs": st_b[-1]["expansions"], "search_ms": ms_search,
-                   "prep_ms": float(np.mean([s["ms_prep"] for s in st_b])),
-                   "post_ms": float(np.mean([s["ms_post"] for s in st_b])),
-                   "passes": st_b[-1]["passes"], "host_p": st_b[-1]["host_p"],
-                   "cand": st_b[-1]["cand"], "steps": st_b[-1]["steps"],
-                   "wait_frac": st_b[-1]["wait_cycles"] / max(st_b[-1]["wave_cycles"], 1),
-                   "donated": st_b[-1]["donated"], "fetches": st_b[-1]["fetches"],
-                   "coniPFVs_per_s": st_b[-1]["expansions"] / (ms_dev * 1e-3),
-                   "host_fallback_rate": st_b[-1]["host_p"] / max(len(pf), 1),
-                   "wall_ms": wall_b * 1e3},
+        "detail": {"nodes": nodes, "points": sb["points"], "kept": sb["kept"],
+                   "pfvs": pfv_b, "survivors_copied": sb["survivors"],
+                   "device_ms": ms_dev, "search_ms": ms_search,
+                   "prep_ms": mean("ms_prep", st_b), "post_ms": mean("ms_post", st_b),
+                   "finish_ms": mean("ms_finish", st_b), "d2h_ms": mean("ms_d2h", st_b),
+                   "upload_ms": mean("ms_upload", st_b), "alloc_ms": mean("ms_alloc", st_b),
+                   "library_wall_ms": mean("ms_wall", st_b),
+                   "pass_ms_min": min(walls_b) * 1e3, "pass_ms_max": max(walls_b) * 1e3,
+                   "device_nodes_per_s": nodes / (ms_dev * 1e-3),
+                   "search_nodes_per_s": nodes / (ms_search * 1e-3),
+                   "passes": sb["passes"], "host_p": sb["host_p"], "host_finished": left_b,
+                   "cand": sb["cand"], "steps": sb["steps"],
+                   "wait_frac": sb["wait_cycles"] / max(sb["wave_cycles"], 1),
+                   "donated": sb["donated"], "fetches": sb["fetches"],
+                   "coniPFVs_per_s": pfv_b / wall_b,
+                   "host_fallback_rate": sb["host_p"] / max(len(pf), 1)},
     }
     work, wnodes, wthreads = search_work(gpu, geoms, pf, pgeo)
     assert wnodes == nodes, "host count and device search visited different nodes"
     cus, clk_khz = gpu.device_facts(0)
     peak = cus * 64 * clk_khz * 1e3                 # lane-ops/s
     achieved = work / (ms_search * 1e-3)
+    traffic = measured_traffic("k_batched") if args.workload == "headline" else None
     out["roofline"] = {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
                        "unit": "T lane-op/s", "frac": achieved / peak,
-                       "traffic": measured_traffic("k_batched") if args.workload == "headline" else None,
-                       "kernel": "k_batched", "work_per_launch": work,
+                       "traffic": traffic,
+                       "traffic_source": (f"{TRAFFIC_FILE} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE;"
+                                          " recorded, not measured by this run)") if traffic else None,
+                       "kernel": "k_search<batched>", "work_per_launch": work,
                        "ms_per_launch": ms_search,
                        "peak_basis": f"{cus} CUs x 64 lanes x {clk_khz / 1e6:.2f} GHz"}
     if not args.no_plain:
-        wall_p, st_p, _ = time_mode(gpu, geoms, pf, pgeo, "plain", 1, 0)
-        ms_p = st_p[-1]["ms_total"]
-        out["detail"]["plain_ms"] = ms_p
-        out["detail"]["plain_nodes_per_s"] = st_p[-1]["nodes"] / (ms_p * 1e-3)
-        out["detail"]["batched_over_plain"] = out["value"] / out["detail"]["plain_nodes_per_s"]
-        assert st_p[-1]["nodes"] == nodes, "plain and batched forms visited different nodes"
+        for mode in ("plain", "lane"):
+            if mode == "lane" and args.workload != "headline":
+                continue
+            wall_p, st_p, _, pfv_p, _, _ = time_mode(gpu, geoms, pf, pgeo, mode, 1, 0)
+            assert st_p[-1]["nodes"] == nodes, f"{mode} and batched forms visited different nodes"
+            assert pfv_p == pfv_b
+            d = out["detail"]
+            d[f"{mode}_pass_ms"] = wall_p * 1e3
+            d[f"{mode}_search_ms"] = st_p[-1]["ms_search"]
+            d[f"{mode}_nodes_per_s"] = nodes / wall_p
+            d[f"batched_over_{mode}"] = (nodes / wall_b) / (nodes / wall_p)
+            d[f"batched_over_{mode}_search"] = st_p[-1]["ms_search"] / ms_search
+            print(f"# {mode}: full pass {wall_p*1e3:.1f} ms, search {st_p[-1]['ms_search']:.2f} ms",
+                  file=sys.stderr, flush=True)
     if not args.no_cpu:
-        v, done, t = cpu_baseline(items, pf, args.cpu_seconds)
-        out["cpu_baseline"] = {"value": v, "unit": "nodes/s", "cores": 1, "kind": "port",
-                               "sample": f"{done} p-vectors of the same batch in {t:.1f} s "
-                                         "(seeded order)"}
+        nth = max(1, min(16, len(os.sched_getaffinity(0))))
+        v1, done1, t1 = cpu_baseline(items, pf, args.cpu_seconds / 2, threads=1)
+        vn, donen, tn = cpu_baseline(items, pf, args.cpu_seconds / 2, threads=nth)
+        out["cpu_baseline"] = {"value": vn, "unit": "nodes/s", "cores": nth, "kind": "port",
+                               "sample": f"{donen} p-vectors of the same batch in {tn:.1f} s on "
+                                         f"{nth} threads (seeded order; the oracle's C "
+                                         f"preparation and search)",
+                               "cpu": cpu_model(),
+                               "single_thread": {"value": v1, "cores": 1,
+                                                 "sample": f"{done1} p-vectors in {t1:.1f} s"}}
+    if args.workload == "headline" and not args.no_pfv:
+        out["coniPFVs"] = pfv_rate(gpu, max(1, min(args.steps, 5)))
     print(json.dumps(out))
 
 

@@ -70,14 +70,20 @@ void pfg_release(int device);
 
 /* ------------------------------------------------------------ part 2 */
 
-enum { PFA_MODE_PLAIN = 0, PFA_MODE_BATCHED = 1, PFA_MODE_HOST = 2 };
+enum { PFA_MODE_PLAIN = 0, PFA_MODE_BATCHED = 1, PFA_MODE_HOST = 2, PFA_MODE_LANE = 3 };
 
 /* Geometry/p-vector layout as pfg_input.  mode: PFA_MODE_PLAIN (one
- * launch per p-vector batch, one thread per p-vector), PFA_MODE_BATCHED
- * (persistent wavefronts, LIFO frontier stacks, global frontier queue),
- * PFA_MODE_HOST (the same search on host threads).  Tunables <= 0 take
- * defaults.  filter != 0 also evaluates the exact per-point filter
- * (K-integrality count and det != 0). */
+ * launch per p-vector batch, one wavefront per p-vector: wave-cooperative
+ * candidate expansion of that p-vector's tree, no work sharing),
+ * PFA_MODE_BATCHED (persistent wavefronts, LIFO frontier deques, a global
+ * frontier ring with work stealing across p-vectors), PFA_MODE_LANE (one
+ * thread per p-vector, the reference kernel's shape), PFA_MODE_HOST (the
+ * same search on host threads).  Tunables <= 0 take defaults.  filter != 0
+ * also evaluates the exact per-point filter (K-integrality count and
+ * det != 0).  keep_points: 0 counters only; 1 every lattice point; 2 (with
+ * filter) only the points that can yield PFVs (n_exp > 0, det != 0 or
+ * undecided), compacted on the device -- what coniZpM's post-processing
+ * reads. */
 typedef struct {
     int n_geo;
     const int32_t *h;
@@ -95,7 +101,7 @@ typedef struct {
     int host_threads;     /* host mode */
     int filter;
     int sort;             /* 1: group by p and sort by (M, Kn) */
-    int keep_points;      /* 0: return counters only (no point arrays) */
+    int keep_points;      /* 0 counters only, 1 all points, 2 the filter's survivors */
     int verbose;
     int assemble;         /* with filter and sort: also the finished PFVs (K, M) of every
                              p-vector the library can finish (see pfa_run_output.pdone) */
@@ -115,6 +121,9 @@ enum {
     PFA_ST_MS_ALLOC,      /* device: host ms spent growing device and pinned buffers */
     PFA_ST_MS_D2H,        /* device: host ms of copying the results back */
     PFA_ST_ATTEMPTS,      /* device: searches run (more than 1 after a point-buffer overflow) */
+    PFA_ST_MS_UPLOAD,     /* device: host ms of the uploads (pageable inputs) */
+    PFA_ST_WAVES,         /* device: wavefronts of the search launch */
+    PFA_ST_SURVIVORS,     /* device, keep_points 2: points copied back */
     PFA_ST_N
 };
 
@@ -137,10 +146,37 @@ typedef struct {
     int64_t n_pfv;
     int64_t *Ks, *Ms, *pfv_pidx;
     int8_t *pdone;        /* n_p */
+    int64_t *pcount;      /* n_p: lattice points of each p-vector (every keep_points mode) */
 } pfa_run_output;
 
 int pfa_coni_run(const pfa_run_input *in, pfa_run_output *out);
 void pfa_run_output_free(pfa_run_output *out);
+
+/* coniZpM's post-processing of given lattice points (pfvs/coni.py:275-412,
+ * restated natively; the same code finishes pfa_coni_run's assemble) for
+ * one geometry (h11 = h, tadpole bound Q): points with rows M and Kn (h
+ * int64 each), q = c^T mat c, key = the p-vector index (0 <= key <
+ * n_keys) and the verdict of det((kappa M)[1:, 1:]) (1 nonsingular, 0
+ * singular, 2 undecided), in any order.  The library orders them by
+ * (key, M, Kn), expands each point into its (g, K0) rows (g = 1 ..
+ * (Q Kgcd - 1) / q, K0 on the tadpole with K' > 0) and keeps the rows of
+ * nonsingular points: by key, then g, then point, then K0.  Rows (n_rows x
+ * h) and their keys go to malloc'd arrays; done[k] = 0 leaves key k to the
+ * caller (an undecided point, or a value beyond int64).  threads <= 0:
+ * all cores.  Returns 0, or 1 with a message in out->err (a tadpole
+ * violation, bad sizes).  Replaces the numpy post-processing of
+ * pfvs/coni.py:275-412 in this library's coniZpM. */
+typedef struct {
+    int64_t n_rows;
+    int64_t *Ks, *Ms, *keys;
+    int8_t *done;         /* n_keys */
+    char err[256];
+} pfa_assemble_output;
+
+int pfa_assemble(int h, int64_t Q, int64_t n_keys, int64_t n, const int64_t *key,
+                 const int64_t *M, const int64_t *Kn, const int64_t *q, const int8_t *nonsing,
+                 int threads, pfa_assemble_output *out);
+void pfa_assemble_output_free(pfa_assemble_output *out);
 
 /* Single-problem kernel batch (the contract of conipfv_kernel,
  * pfvs/fp_kernel/fp_kernel.pyx:258-318 / fp_kernel.h:25-63): for problem

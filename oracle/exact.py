@@ -8,8 +8,8 @@ any decision) used to pin the C oracle and the product on small cases:
   contract pfvs/fp_kernel/fp_kernel.h:25-63).
 - `hnf`: canonical row HNF (flint's fmpz_mat_hnf form, which
   pfvs/coni.py:172-202 uses).
-- `pfv_checks`: the 7 coniPFV constraints (reference README.md:38-45,
-  pfvs/pfv.py:651-715).
+- `pfv_checks`: the checks of a coni PFV (reference README.md:38-45,
+  pfvs/pfv.py:606-715), with p derived exactly from N^-1 K.
 - `is_singular`: exact rank test (pfvs/util.py:224-243).
 - `coni_points`: the exact per-p point set of the batched pipeline
   (fp_kernel.pyx:480-622) computed from scratch with Python ints: the
@@ -168,52 +168,156 @@ def is_singular(N):
     return rank(N) < len(N)
 
 
-def pfv_checks(kappa, c2, h11, h21, K, M, p):
-    """The 7 coniPFV constraints for (K, M) with p-vector p (p[0] = 0), all
-    exact.  Returns a dict name -> bool (pfvs/pfv.py:651-715 and README
-    constraints 1-7; a and b as pfvs/cydata.py:252-303)."""
-    h = h11
+def solve_exact(A, y):
+    """x with A x = y for an invertible integer matrix A, in Fractions."""
+    n = len(A)
+    M = [[Fraction(int(v)) for v in row] + [Fraction(int(t))] for row, t in zip(A, y)]
+    for col in range(n):
+        piv = next(i for i in range(col, n) if M[i][col] != 0)
+        M[col], M[piv] = M[piv], M[col]
+        for i in range(n):
+            if i != col and M[i][col] != 0:
+                f = M[i][col] / M[col][col]
+                M[i] = [u - f * w for u, w in zip(M[i], M[col])]
+    return [M[i][n] / M[i][i] for i in range(n)]
+
+
+def pfv_checks(kappa, c2, H, h21, K, M):
+    """The checks of a coni PFV (K, M) (PFV.check_all, pfvs/pfv.py:606-715;
+    README constraints 1-7), all exact.  kappa, c2 and H in the coni basis
+    (H: the Kahler cone rows on coordinates 1..h-1).  With N = (kappa M)
+    [1:, 1:] invertible, p = (0, N^-1 K[1:]) (pfv.py:497-541) and
+    pgrading its primitive integer multiple:
+      a_even        a M is even (a_xy = kappa_xxy for x >= y, else kappa_xyy)
+      b_24          (c2 + 2 e_0) . M = 0 mod 24
+      tadpole       0 <= -K . M <= h11 + h21 + 4
+      K_nonzero     K != 0
+      N_invertible  det N != 0
+      p_in_cone     H pgrading[1:] >= 1
+      NpK           N p[1:] = K[1:]
+      orthogonal    pgrading . K = 0
+      Kprime        K' = -K_0 + (M kappa p)_0 > 0
+    Returns a dict name -> bool (p-dependent checks False when N is
+    singular)."""
+    h = len(M)
     K = [int(x) for x in K]
     M = [int(x) for x in M]
-    p = [int(x) for x in p]
-    a = [[int(kappa[x][x][y]) if x >= y else int(kappa[x][y][y]) for y in range(h)] for x in range(h)][1:]
+    kap = [[[int(kappa[i][j][k]) for k in range(h)] for j in range(h)] for i in range(h)]
+    a = [[kap[x][x][y] if x >= y else kap[x][y][y] for y in range(h)] for x in range(h)]
     b = [int(x) for x in c2]
     b[0] += 2
-    N = [[sum(int(kappa[i][j][k]) * M[k] for k in range(h)) for j in range(h)] for i in range(h)]
-    Nr = [row[1:] for row in N[1:]]
+    N = [[sum(kap[i][j][k] * M[k] for k in range(h)) for j in range(1, h)] for i in range(1, h)]
     KM = sum(x * y for x, y in zip(K, M))
     out = {
         "a_even": all(sum(ai * m for ai, m in zip(row, M)) % 2 == 0 for row in a),
         "b_24": sum(bi * m for bi, m in zip(b, M)) % 24 == 0,
-        "tadpole": 0 <= -KM <= h11 + h21 + 4,
+        "tadpole": 0 <= -KM <= h + int(h21) + 4,
         "K_nonzero": any(K),
-        "N_invertible": not is_singular(Nr),
-        "orthogonal": p[0] == 0 and sum(x * y for x, y in zip(p[1:], K[1:])) == 0,
+        "N_invertible": not is_singular(N),
     }
-    if out["N_invertible"]:
-        Np = [sum(Nr[i][j] * p[1 + j] for j in range(h - 1)) for i in range(h - 1)]
-        out["NpK"] = Np == K[1:]
-    else:
-        out["NpK"] = False
+    if not out["N_invertible"]:
+        out.update(p_in_cone=False, NpK=False, orthogonal=False, Kprime=False)
+        return out
+    p = [Fraction(0)] + solve_exact(N, K[1:])
+    den = 1
+    for x in p:
+        den = den * x.denominator // math.gcd(den, x.denominator)
+    num = [int(x * den) for x in p]
+    g = 0
+    for x in num:
+        g = math.gcd(g, x)
+    pgrading = [x // g for x in num] if g else num
+    out["p_in_cone"] = all(sum(int(r[j]) * pgrading[j + 1] for j in range(h - 1)) >= 1 for r in H)
+    out["NpK"] = [sum(N[i][j] * p[j + 1] for j in range(h - 1)) for i in range(h - 1)] == K[1:]
+    out["orthogonal"] = sum(x * y for x, y in zip(pgrading, K)) == 0
+    MKp0 = sum(M[i] * kap[i][0][k] * p[k] for i in range(h) for k in range(h))
+    out["Kprime"] = -K[0] + MKp0 > 0
     return out
+
+
+def lll_reduce(rows, ip, delta=Fraction(3, 4)):
+    """LLL reduction (Lenstra, Lenstra, Lovasz 1982) of linearly independent
+    integer vectors `rows` under the inner product ip(a, b), in exact
+    rationals with the incremental Gram-Schmidt updates of H. Cohen, "A
+    Course in Computational Algebraic Number Theory", Algorithm 2.6.3.
+    Returns the reduced rows (same lattice)."""
+    b = [list(r) for r in rows]
+    n = len(b)
+    if n <= 1:
+        return b
+    mu = [[Fraction(0)] * n for _ in range(n)]
+    B = [Fraction(0)] * n
+    B[0] = Fraction(ip(b[0], b[0]))
+    k, kmax = 1, 0
+
+    def red(k, l):
+        if abs(mu[k][l]) > Fraction(1, 2):
+            q = round(mu[k][l])
+            b[k] = [x - q * y for x, y in zip(b[k], b[l])]
+            mu[k][l] -= q
+            for i in range(l):
+                mu[k][i] -= q * mu[l][i]
+
+    while k < n:
+        if k > kmax:
+            kmax = k
+            for j in range(k):
+                mu[k][j] = (Fraction(ip(b[k], b[j]))
+                            - sum(mu[j][i] * mu[k][i] * B[i] for i in range(j))) / B[j]
+            B[k] = ip(b[k], b[k]) - sum(mu[k][j] ** 2 * B[j] for j in range(k))
+        red(k, k - 1)
+        if B[k] < (delta - mu[k][k - 1] ** 2) * B[k - 1]:
+            b[k], b[k - 1] = b[k - 1], b[k]
+            for j in range(k - 1):
+                mu[k][j], mu[k - 1][j] = mu[k - 1][j], mu[k][j]
+            m = mu[k][k - 1]
+            Bn = B[k] + m * m * B[k - 1]
+            mu[k][k - 1] = m * B[k - 1] / Bn
+            B[k] = B[k - 1] * B[k] / Bn
+            B[k - 1] = Bn
+            for i in range(k + 1, kmax + 1):
+                t = mu[i][k]
+                mu[i][k] = mu[i][k - 1] - m * t
+                mu[i][k - 1] = t + mu[k][k - 1] * mu[i][k]
+            k = max(1, k - 1)
+            continue
+        for l in range(k - 2, -1, -1):
+            red(k, l)
+        k += 1
+    return b
+
+
+def lll_gram(G, delta=Fraction(3, 4)):
+    """Rows T of a unimodular matrix whose rows, as coefficient vectors, are
+    an LLL-reduced basis for the positive definite Gram matrix G."""
+    n = len(G)
+    G = [[int(x) for x in r] for r in G]
+
+    def ip(a, c):
+        return sum(a[i] * G[i][j] * c[j] for i in range(n) if a[i] for j in range(n) if c[j])
+    return lll_reduce([[int(i == j) for j in range(n)] for i in range(n)], ip, delta)
 
 
 def coni_points(kappa, Mbasis, p_full, Q, qmax, linmin):
     """Exact point set of one p-vector as a set of (M, Kn, q) tuples, with no
     floating point: the same lattice, ellipsoid and cuts as the pipeline,
-    but in a basis of our own (so only the set -- not the order -- is
-    comparable).  Small cases only."""
-    from itertools import product as _p  # noqa: F401
+    but in a basis of our own (the complement of (Z p) M = 0 by Euclid
+    steps, then LLL-reduced for the form exactly), so only the set -- not
+    the order -- is comparable."""
     h = len(p_full)
     p = [int(x) for x in p_full]
     Mb = [[int(x) for x in r] for r in Mbasis]
     Z = [[sum(int(kappa[i][j][k]) * p[k] for k in range(h)) for j in range(h)] for i in range(h)]
     T = [sum(Z[i][j] * p[j] for j in range(h)) for i in range(h)]
     v = [sum(T[i] * Mb[i][j] for i in range(h)) for j in range(h)]
-    O = _orth(v)                                  # rows: coefficient vectors
+    O = lll_reduce(_orth(v), lambda x, y: sum(a * c for a, c in zip(x, y)))   # rows: coefficients
     B = [[sum(Mb[i][l] * O[a][l] for l in range(h)) for i in range(h)] for a in range(len(O))]
     d = len(B)
     ZB = [[sum(Z[i][j] * B[a][j] for j in range(h)) for i in range(h)] for a in range(d)]
+    mat = [[-sum(B[a][i] * ZB[b][i] for i in range(h)) for b in range(d)] for a in range(d)]
+    T = lll_gram(mat)                              # new basis vector a = sum_b T[a][b] B[b]
+    B = [[sum(T[a][b] * B[b][i] for b in range(d)) for i in range(h)] for a in range(d)]
+    ZB = [[sum(T[a][b] * ZB[b][i] for b in range(d)) for i in range(h)] for a in range(d)]
     mat = [[-sum(B[a][i] * ZB[b][i] for i in range(h)) for b in range(d)] for a in range(d)]
     out = set()
     for c in ellipsoid_points(mat, qmax):

@@ -612,6 +612,11 @@ typedef struct {
     long max_out;
     double eps;
     const double *fac;    /* optional exact factor: U, m0N, m0V, m0Va */
+    /* optional: rows (nleaf, dim) int64 whose gcd at a leaf is the exact g
+     * (used when some rows of H do not fit int64: H then prunes with the
+     * rows that fit, and the leaf decides with the complete gcd) */
+    const int64_t *leafrows;
+    int nleaf;
 } or_problem;
 
 typedef struct {
@@ -836,8 +841,18 @@ int or_search(const or_problem *P, or_out *out)
         i128 q;
         if (exact_q(mat, c, dim, &q)) return 1;
         if (q > P->qmax) continue;
-        if (use_gcd && g[0] != 0 && (g[0] >> 64) == 0) {
-            i128 Qg = (i128)P->Q * (i128)(uint64_t)g[0];
+        u128 gl = use_gcd ? g[0] : 0;
+        if (P->leafrows) {
+            /* gcd(H c) = gcd(L c) for L with the row lattice of H (|L c| < 2^100) */
+            gl = 0;
+            for (int r = 0; r < P->nleaf; ++r) {
+                i128 s_ = 0;
+                for (int j = 0; j < dim; ++j) s_ += (i128)P->leafrows[r * dim + j] * c[j];
+                gl = gcd128(gl, oabs(s_));
+            }
+        }
+        if ((use_gcd || P->leafrows) && gl != 0 && (gl >> 64) == 0) {
+            i128 Qg = (i128)P->Q * (i128)(uint64_t)gl;
             if (P->strict ? (Qg <= q) : (Qg < q)) continue;
         }
         int rc = push(out, c, dim, (int64_t)q, P->max_out);
@@ -856,7 +871,8 @@ int or_kernel(int dim, const int64_t *mat, int64_t Q, int64_t qmax, int nrows,
               long max_out, double eps, const double *fac, int32_t **pts, int64_t **qs,
               long *n, long long *counts)
 {
-    or_problem P = {dim, strict, nrows, -1, Q, qmax, linmin, mat, linvec, H, max_out, eps, fac};
+    or_problem P = {dim, strict, nrows, -1, Q, qmax, linmin, mat, linvec, H, max_out, eps, fac,
+                    NULL, 0};
     or_out o;
     memset(&o, 0, sizeof o);
     int st = or_search(&P, &o);
@@ -884,14 +900,17 @@ long or_coni_batch(int h, const int64_t *kappa, const int64_t *Mbasis, const int
         status[ip] = 0;
         int st = or_build(h, kappa, Mbasis, &ps[ip * h], R);
         if (st) { status[ip] = 1; continue; }
-        int bad = 0;
-        for (int k = 0; k < d * d; ++k) {
-            if (!fits64(R->H[k])) { bad = 1; break; }
-            H64[k] = (int64_t)R->H[k];
+        /* HNF rows beyond int64 leave the pruning; the leaf then takes the
+         * exact gcd of Kn[1:] = ZB[1:] c, whose rows span the same lattice */
+        int bad = 0, wide = 0;
+        for (int r = 0; r < d; ++r) {
+            int fit = 1;
+            for (int k = 0; k < d; ++k) fit &= fits64(R->H[r * d + k]);
+            for (int k = 0; k < d; ++k) H64[r * d + k] = fit ? (int64_t)R->H[r * d + k] : 0;
+            wide |= !fit;
         }
-        if (bad) { status[ip] = 1; continue; }
         or_problem P = {d, 1, d, -1, Q, qmax, linmin, R->mat, &R->Binter[0], H64, max_out, 1e-4,
-                        NULL};
+                        NULL, wide ? &R->ZB[d] : NULL, wide ? d : 0};
         or_out o;
         memset(&o, 0, sizeof o);
         st = or_search(&P, &o);

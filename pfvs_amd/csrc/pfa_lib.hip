@@ -6,9 +6,11 @@
 //               per p-vector, one launch per h11 size class: lattice, HNF,
 //               FP64 Cholesky with its error margin, record.
 //   k_roots     one lane per p-vector: top-level node -> frontier ring.
-//   search      PLAIN: k_plain, one lane per p-vector runs the whole
-//               depth-first search (the one-launch-per-batch form).
-//               BATCHED: k_batched, persistent wavefronts each holding a
+//   search      PLAIN: k_search<true>, one launch per batch, one
+//               wavefront per p-vector (the same wave-cooperative step,
+//               no work sharing between p-vectors).  LANE: k_lane, one
+//               thread per p-vector runs the whole depth-first search.
+//               BATCHED: k_search<false>, persistent wavefronts each holding a
 //               LIFO deque of nodes (its top 128 entries cached in LDS);
 //               every step expands up to 64 sibling candidates of the top
 //               nodes at once (prefix sum over their ranges), pushes
@@ -128,6 +130,24 @@ __device__ static inline PfaProb rec_view_uni(const int64_t *r, const RecL &L)
 }
 
 PFA_HD static inline int32_t &rec_status(int64_t *r) { return ((int32_t *)r)[0]; }
+
+// A search that may meet several events in one problem (the batched form,
+// whose wavefronts reach them in no fixed order) records them with a fixed
+// precedence instead of last-writer-wins: PFA_HOST (the exact host path
+// searches the problem again and decides it) over a kernel status (PFA_I32)
+// over PFA_OK.  The result then does not depend on timing.
+PFA_HD static inline int status_rank(int32_t s) { return s == PFA_OK ? 0 : (s == PFA_HOST ? 2 : 1); }
+
+__device__ static inline void rec_raise_status(int64_t *r, int32_t s)
+{
+    int32_t *p = (int32_t *)r;
+    int32_t old = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (status_rank(s) > status_rank(old)) {
+        const int32_t prev = atomicCAS(p, old, s);
+        if (prev == old) return;
+        old = prev;
+    }
+}
 
 // scalar constants of the search for a problem of dimension d with factor
 // margin phi (slack and Kerr as fpk_search_consts, fpk_search_impl.h:30-39)
@@ -379,7 +399,7 @@ __global__ void k_roots(int64_t n, int64_t *recs, RecL L, QueueArgs Qa, EntL E, 
     pfa_root(P, nd);
     int32_t c[PFA_DMAX];
     for (int j = 0; j < PFA_DMAX; ++j) c[j] = 0;
-    if (pfa_enter(P, c, nd)) { rec_status(rec) = PFA_I32; return; }
+    if (pfa_enter(P, c, nd)) { rec_raise_status(rec, PFA_I32); return; }
     atomicAdd(&C.c[0], 1ULL);
     atomicAdd(&Qa.ctl[32], 1ULL);                        // one outstanding unit
     unsigned long long t = atomicAdd(&Qa.ctl[16], 1ULL);
@@ -392,8 +412,10 @@ __global__ void k_roots(int64_t n, int64_t *recs, RecL L, QueueArgs Qa, EntL E, 
     atomicExch(&Qa.seq[t % Qa.cap], t + 1);
 }
 
-// ----------------------------------------------------------- plain form
-__global__ void k_plain(int64_t n, int64_t *recs, RecL L, PointArgs Pt, int D, CountArgs C)
+// ------------------------------------------------ one lane per problem
+// The reference-shaped form: a thread runs a whole problem's depth-first
+// search (PFA_MODE_LANE).
+__global__ void k_lane(int64_t n, int64_t *recs, RecL L, PointArgs Pt, int D, CountArgs C)
 {
     int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
@@ -418,16 +440,25 @@ __global__ void k_plain(int64_t n, int64_t *recs, RecL L, PointArgs Pt, int D, C
     atomicAdd(&C.c[2], cnt.leaf);
 }
 
-// --------------------------------------------------------- batched form
-// One persistent launch.  Each wavefront owns a deque of nodes (stack top
-// for depth-first expansion, bottom for donation).  A step expands up to 64
-// candidates taken from the top nodes; children are pushed with a ballot;
-// leaves are written with one atomic per wavefront.  Wavefronts whose deque
-// runs dry take a ticket of the frontier ring and wait for its entry; busy
-// wavefronts that see waiting ones donate their bottom (shallowest, largest)
-// nodes.  `work` counts queued + in-process units, so waiting wavefronts
-// leave exactly when the whole search is done; a wall-clock deadline bounds
-// every wait and the launch itself.
+// --------------------------------------------- batched and plain forms
+// k_search<false> (PFA_MODE_BATCHED): one persistent launch.  Each
+// wavefront owns a deque of nodes (stack top for depth-first expansion,
+// bottom for donation).  A step expands up to 64 candidates taken from the
+// top nodes; children are pushed with a ballot; leaves are written with one
+// atomic per wavefront.  Wavefronts whose deque runs dry take a ticket of
+// the frontier ring and wait for its entry; busy wavefronts that see
+// waiting ones donate their bottom (shallowest, largest) nodes.  `work`
+// counts queued + in-process units, so waiting wavefronts leave exactly
+// when the whole search is done; a wall-clock deadline bounds every wait
+// and the launch itself.
+//
+// k_search<true> (PFA_MODE_PLAIN): the one-launch-per-batch form, one
+// wavefront per p-vector.  Workgroup b searches p-vectors b, b + grid, ...
+// from their roots with the same step (64 candidates of its top nodes per
+// step, ballot pushes, chunked point slots), but no frontier ring, no
+// donation and no stealing: a p-vector's tree stays on the wavefront that
+// started it.  A deque that outgrows stack_cap stops the launch with error
+// 3, and the host reruns it with twice the capacity.
 struct BatchArgs {
     int64_t *recs;
     RecL L;
@@ -438,6 +469,7 @@ struct BatchArgs {
     PointArgs Pt;
     CountArgs C;
     unsigned long long deadline;      // wall-clock ticks (0: none)
+    int64_t n_p;                      // plain form: the p-vectors
 };
 
 #define PFA_CSTRIDE (PFA_DMAX + 1)        // odd stride: conflict-free LDS rows
@@ -521,7 +553,8 @@ __device__ static inline void cache_move(char *cache, char *stk, int from, int t
     __syncthreads();
 }
 
-__global__ __launch_bounds__(64) void k_batched(BatchArgs a)
+template <bool PLAIN>
+__global__ __launch_bounds__(64) void k_search(BatchArgs a)
 {
     const int lane = threadIdx.x;
     const int ES = a.E.bytes, D = a.E.D;
@@ -539,8 +572,49 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
     int bot = 0, top = 0, have_unit = 0, steps = 0;
     unsigned long long waiting = 0;
     unsigned long long pt_next = 0, pt_end = 0;        // this wavefront's point slots
+    int64_t ip_next = blockIdx.x;                       // plain form: next p-vector
     for (;;) {
-        if (top == bot) {
+        if (PLAIN && top == bot) {
+            // the next p-vector of this wavefront whose root opens: its root
+            // node to cache slot 0 (counted as a node, as k_roots does)
+            int got = 0;
+            while (!got && ip_next < a.n_p) {
+                const int64_t ip = ip_next;
+                ip_next += gridDim.x;
+                if (lane == 0) {
+                    int64_t *rec = a.recs + ip * a.L.words;
+                    int ok = 0;
+                    if (rec_status(rec) == PFA_OK) {
+                        PfaProb P = rec_view(rec, a.L);
+                        if (!pfa_trivially_empty(P)) {
+                            PfaNode nd;
+                            pfa_root(P, nd);
+                            for (int j = 0; j < PFA_DMAX; ++j) c[j] = 0;
+                            if (pfa_enter(P, c, nd)) {
+                                rec_raise_status(rec, PFA_I32);
+                            } else {
+                                char *e = ent(0);
+                                *(PfaNode *)e = nd;
+                                ((int32_t *)(e + 128))[0] = (int32_t)ip;
+                                int32_t *ec = (int32_t *)(e + 144);
+                                for (int j = 0; j < D; ++j) ec[j] = 0;
+                                n_nodes++;
+                                ok = 1;
+                            }
+                        }
+                    }
+                    s_u64[0] = ok;
+                }
+                __syncthreads();
+                got = (int)s_u64[0];
+                __syncthreads();
+            }
+            if (!got) break;
+            bot = 0;
+            top = 1;
+            cb = 0;
+        }
+        if (!PLAIN && top == bot) {
             if (lane == 0) {
                 if (have_unit) atomicAdd(&a.Q.ctl[32], ~0ULL);          // unit done
                 unsigned long long h = atomicAdd(&a.Q.ctl[0], 1ULL), slot = h % a.Q.cap;
@@ -581,14 +655,17 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
             have_unit = 1;
         }
         if ((++steps & 255) == 0 && a.deadline) {
-            if (lane == 0) s_u64[0] = (wall_clock64() - t0 > a.deadline || ld_agent(&a.Q.ctl[80])) ? 1 : 0;
+            // stop at the deadline (error 2) or once another wavefront has
+            // recorded an error (kept: the first error stands)
+            if (lane == 0) {
+                const int late = wall_clock64() - t0 > a.deadline;
+                if (late) atomicCAS(&a.Q.ctl[80], 0ULL, 2ULL);
+                s_u64[0] = (late || ld_agent(&a.Q.ctl[80])) ? 1 : 0;
+            }
             __syncthreads();
             const int stop = (int)s_u64[0];
             __syncthreads();
-            if (stop) {
-                if (lane == 0) atomicExch(&a.Q.ctl[80], 2ULL);
-                break;
-            }
+            if (stop) break;
         }
         // gather up to 64 candidates from the top nodes (lane l: node top-1-l),
         // all of them cached
@@ -655,17 +732,17 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
             n_cand++;
             int r = pfa_cand(P, nd, v, ch);
             if (r == PFA_CAND_OVF) {
-                rec_status((int64_t *)rec) = PFA_HOST;
+                rec_raise_status((int64_t *)rec, PFA_HOST);
             } else if (r == 1) {
                 c[nd.i] = v;
                 if (nd.i > 0) {
                     n_nodes++;
-                    if (pfa_enter(P, c, ch)) rec_status((int64_t *)rec) = PFA_I32;
+                    if (pfa_enter(P, c, ch)) rec_raise_status((int64_t *)rec, PFA_I32);
                     else has_child = ch.hi >= ch.cur;
                 } else {
                     n_leaf++;
                     int lr = pfa_leaf(P, ch, c, &q);
-                    if (lr == PFA_CAND_OVF) rec_status((int64_t *)rec) = PFA_HOST;
+                    if (lr == PFA_CAND_OVF) rec_raise_status((int64_t *)rec, PFA_HOST);
                     else emit = lr == 1;
                 }
             }
@@ -713,6 +790,12 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
                 newtop -= bot;
                 bot = 0;
             }
+            if (PLAIN && newtop + nch > a.stack_cap) {
+                // no donation in the plain form: the host reruns the launch
+                // with a larger deque
+                if (lane == 0) atomicCAS(&a.Q.ctl[80], 0ULL, 3ULL);
+                break;
+            }
             if (newtop + nch > a.stack_cap) {
                 int k = newtop / 2;
                 if (!donate(a, stk, 0, k, 1, lane, s_u64)) break;     // ring full: error set
@@ -745,13 +828,13 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
         // nodes: the largest subtrees), only from deques deep enough to keep
         // this wavefront's own steps full; the waiting count is refreshed
         // every 16 steps
-        if ((steps & 15) == 0) {
+        if (!PLAIN && (steps & 15) == 0) {
             if (lane == 0) s_u64[0] = ld_agent(&a.Q.ctl[48]);
             __syncthreads();
             waiting = s_u64[0];
             __syncthreads();
         }
-        if (waiting > 0 && top - bot >= a.donate_min) {
+        if (!PLAIN && waiting > 0 && top - bot >= a.donate_min) {
             {
                 int k = (top - bot) / 4;
                 if ((unsigned long long)k > waiting) k = (int)waiting;
@@ -778,6 +861,20 @@ __global__ __launch_bounds__(64) void k_batched(BatchArgs a)
         atomicAdd(&a.C.c[5], clock64() - c0);
         atomicAdd(&a.C.c[6], n_don);
         atomicAdd(&a.C.c[7], n_fetch);
+    }
+}
+
+// Status words of the records: op 0 saves them to buf, op 1 restores them
+// from buf (a search changes only the status word of a record), op 2
+// gathers (status, stage code) pairs for the copy back.
+__global__ void k_status_io(int64_t n, int64_t *recs, int words, int32_t *buf, int op)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t *ri = (int32_t *)(recs + i * words);
+        if (op == 0) buf[i] = ri[0];
+        else if (op == 1) ri[0] = buf[i];
+        else { buf[2 * i] = ri[0]; buf[2 * i + 1] = ri[25]; }
     }
 }
 
@@ -917,6 +1014,7 @@ struct FiltArgs {
     int64_t *modkp, *modnexp, *modM;   // list: slot, n_exp, M row (stride MH)
     unsigned long long *modcount;
     int64_t *keepkp, *keepnexp;   // points with n_exp > 0 (k_post -> k_filter)
+    unsigned long long *pcount;   // per p-vector: lattice points (k_post; null: not counted)
     int64_t *keepmeta;            // their p-vector << 32 | geometry << 8 | h
     int n_geo;
     unsigned long long *keepcount;
@@ -1073,6 +1171,16 @@ __global__ __launch_bounds__(64) void k_post(FiltArgs a)
         if (a.filter && valid) ne = ok ? pfa_count_expansions_fast(M0, Kn0, kg, a.q[kp], rec[2]) : -1;
         if (a.nexp && valid) a.nexp[kp] = ne;
         if (a.nonsing && valid) a.nonsing[kp] = 0;
+        if (a.pcount) {
+            // lattice points per p-vector: one atomic per wavefront when the
+            // wavefront's slots belong to one p-vector
+            const unsigned long long vm0 = __ballot(valid);
+            if (uni) {
+                if (lane == 0) atomicAdd(&a.pcount[p0], (unsigned long long)__popcll(vm0));
+            } else if (valid) {
+                atomicAdd(&a.pcount[pid], 1ULL);
+            }
+        }
         // append n_exp > 0 to the list
         const unsigned long long km = __ballot(ne > 0);
         const int cnt = __popcll(km);
@@ -1644,8 +1752,77 @@ __global__ __launch_bounds__(256) void k_det_mod(FiltArgs a)
     }
 }
 
+// ---------------------------------------------------------- k_compact
+// The points that can yield PFVs -- n_exp > 0 (the filter's list) and a
+// verdict other than singular (1 nonsingular, 2 undecided) -- gathered into
+// dense arrays for the copy back: p-vector, q, n_exp, verdict, and the rows
+// M = Binter c and Kn = ZB c recomputed in int128 (a value beyond int64 has
+// already marked its p-vector for the exact host path in k_post; such a
+// row is stored truncated and dropped with its p-vector).  One atomic per
+// wavefront reserves the output slots; slots beyond `cap` are counted, not
+// written, and the host reruns with room for all.
+struct CompactArgs {
+    const int64_t *keepkp, *keepnexp;
+    const unsigned long long *keepcount;
+    const int8_t *nonsing;
+    const int32_t *pidx;
+    const int64_t *q;
+    const int32_t *c;
+    int D, HS;
+    const int64_t *recs;
+    RecL L;
+    int32_t *o_pidx;
+    int64_t *o_q, *o_M, *o_Kn, *o_nexp;
+    int8_t *o_ns;
+    unsigned long long *o_count;
+    unsigned long long cap;
+};
+
+__global__ __launch_bounds__(64) void k_compact(CompactArgs a)
+{
+    const int lane = threadIdx.x;
+    const int64_t nk = (int64_t)*a.keepcount;
+    for (int64_t base = (int64_t)blockIdx.x * 64; base < nk; base += (int64_t)gridDim.x * 64) {
+        const int64_t t = base + lane;
+        int64_t kp = -1;
+        int8_t ns = 0;
+        if (t < nk) {
+            kp = a.keepkp[t];
+            ns = a.nonsing[kp];
+        }
+        const bool take = kp >= 0 && ns != 0;
+        const unsigned long long m = __ballot(take);
+        if (!m) continue;
+        unsigned long long o0 = 0;
+        if (lane == 0) o0 = atomicAdd(a.o_count, (unsigned long long)__popcll(m));
+        o0 = shfl_u64(o0, 0);
+        if (!take) continue;
+        const unsigned long long o = o0 + __popcll(m & ((1ull << lane) - 1));
+        if (o >= a.cap) continue;
+        const int32_t ip = a.pidx[kp];
+        const int64_t *rec = a.recs + (int64_t)ip * a.L.words;
+        const int d = ((const int32_t *)rec)[1];
+        const int32_t *cc = a.c + kp * a.D;
+        const int64_t *Bi = rec + a.L.Bi, *ZB = rec + a.L.ZB;
+        for (int i = 0; i < a.HS; ++i) {
+            pfa_i128 sm = 0, sk = 0;
+            if (i <= d)
+                for (int j = 0; j < d; ++j) {
+                    sm += (pfa_i128)Bi[(int64_t)i * d + j] * cc[j];
+                    sk += (pfa_i128)ZB[(int64_t)i * d + j] * cc[j];
+                }
+            a.o_M[o * a.HS + i] = (int64_t)sm;
+            a.o_Kn[o * a.HS + i] = (int64_t)sk;
+        }
+        a.o_pidx[o] = ip;
+        a.o_q[o] = a.q[kp];
+        a.o_nexp[o] = a.keepnexp[t];
+        a.o_ns[o] = ns;
+    }
+}
+
 // ============================================================ host side
-#define HIPCHK(x)                                                                      \
+#define HIPCHK(x)                                                                    \
     do {                                                                               \
         hipError_t e_ = (x);                                                           \
         if (e_ != hipSuccess) {                                                        \
@@ -1682,6 +1859,9 @@ struct DevCtx {
     unsigned long long pts_reserved = 0;   // point slots pfa_reserve set aside (0: none)
     double ppp_hint = 0;                   // largest points per p-vector seen (n_p >= 64)
     double ms_alloc = 0;                   // this run's buffer growth (host ms)
+    unsigned long long sv_hint = 0;        // largest survivor count seen (keep_points 2)
+    hipEvent_t ev[8] = {};                 // stage timings (created once)
+    int wclk_khz = 0;                      // wall-clock counter rate
 };
 
 static std::mutex g_ctx_mu;
@@ -1707,6 +1887,8 @@ static void ctx_init(DevCtx *C, int device, char *errbuf)
     C->lds_max = prop.sharedMemPerBlock;
     C->lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
                                                       : prop.sharedMemPerBlock;
+    for (auto &e : C->ev) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipDeviceGetAttribute(&C->wclk_khz, hipDeviceAttributeWallClockRate, device));
 }
 
 enum {
@@ -1714,7 +1896,10 @@ enum {
     B_RECS, B_RING, B_SEQ, B_STACKS, B_CTL, B_PT_PIDX, B_PT_Q, B_PT_C, B_CNT, B_M, B_KN, B_NEXP, B_BAD,
     B_PRIMES, B_KT, B_KTOFS, B_KTMAX, B_MODCNT, B_MODLIST, B_MODNEXP, B_MODM, B_NONSING, B_PREPCOLD,
     B_TOT, B_KEEPCNT, B_KEEPKP, B_KEEPNEXP, B_KEEPMETA, B_PREPORDER,
-    B_SPR, B_SPKV, B_SPROFS, B_SPKVOFS, B_SPINTS, B_NBUF
+    B_SPR, B_SPKV, B_SPROFS, B_SPKVOFS, B_SPINTS,
+    B_SV_PIDX, B_SV_Q, B_SV_M, B_SV_KN, B_SV_NEXP, B_SV_NS, B_SV_CNT, B_STATSAVE, B_STATOUT,
+    B_PCOUNT,
+    B_NBUF
 };
 
 static void *dev_get(DevCtx *C, int slot, size_t bytes, char *errbuf)
@@ -1739,7 +1924,7 @@ static void *dev_get(DevCtx *C, int slot, size_t bytes, char *errbuf)
 
 // Grow-only pinned host buffers, per device, for the results' copy back
 // (no zero fill, no page faults after the first use, full-rate DMA).
-enum { H_PIDX, H_Q, H_M, H_KN, H_NEXP, H_NS, H_NBUF };
+enum { H_PIDX, H_Q, H_M, H_KN, H_NEXP, H_NS, H_STAT, H_NBUF };
 
 static void *host_get(DevCtx *C, Stage &S, int slot, size_t bytes, char *errbuf)
 {
@@ -2096,46 +2281,69 @@ static void filter_run(DevCtx *C, hipStream_t s, const pfa_run_input *gin,
 #undef PFA_FILTER_LAUNCH
 }
 
-// coniZpM's post-processing (pfvs/coni.py:313-412) for the points out
-// holds (sorted canonically): per p-vector, for its kept (n_exp > 0)
-// nonsingular points with Kgcd = gcd(Kn[1:]) (0 -> 1), raw = (q + M0 Kn0)
-// / Kgcd and g = 1 .. (Q Kgcd - 1) / q, every K0 from
-// ceil((g raw - Q) / M0) to min(floor((g raw - Q) / M0), ceil(Kn0 g / Kgcd) - 1)
-// gives K = (K0, g Kn[1:] / Kgcd) with the point's M; rows by g, then the
-// points in their order, then K0.  The tadpole -K.M <= Q is checked (an
-// error, as coni.py:374-386 raises).  A p-vector with an undecided point, a
-// status, or a value beyond int64 is left to the caller (pdone 0).
-// Returns 0, or 1 with a message.
-static int assemble_pfvs(const pfa_run_input *in, pfa_run_output *out, int nth, char *errbuf)
+// coniZpM's post-processing (pfvs/coni.py:313-412) over points grouped by
+// key (ascending) and in canonical (M, Kn) order within a key: for the kept
+// (n_exp > 0, or all when nexp is null) nonsingular points of a key, with
+// Kgcd = gcd(Kn[1:]) (0 -> 1), raw = (q + M0 Kn0) / Kgcd and
+// g = 1 .. (Q Kgcd - 1) / q, every K0 from ceil((g raw - Q) / M0) to
+// min(floor((g raw - Q) / M0), ceil(Kn0 g / Kgcd) - 1) gives
+// K = (K0, g Kn[1:] / Kgcd) with the point's M; rows by g, then the points
+// in their order, then K0.  The tadpole -K.M <= Q is checked (an error, as
+// coni.py:374-386 raises).  A key with an undecided point (verdict 2), a
+// value beyond int64, g beyond 10^6, or skip[key] set is left to the caller
+// (done[key] = 0).  When nexp is given, each point's row count must equal
+// it (the device's count checked against this expansion).
+struct AsmIn {
+    int64_t n_keys, n;
+    const int64_t *key, *M, *Kn, *q, *nexp;
+    const int8_t *nonsing;
+    int stride;                       // of M, Kn and of the output rows
+    const int32_t *kh;                // per key: h11
+    const int64_t *kQ;                // per key: Q
+    const int8_t *skip;               // per key (null: none)
+};
+
+struct AsmOut {
+    int64_t n_rows = 0;
+    int64_t *Ks = nullptr, *Ms = nullptr, *keys = nullptr;
+    int8_t *done = nullptr;
+};
+
+// Returns 0, or 1 with a message (tadpole violation, count mismatch).
+static int assemble_core(const AsmIn &A, AsmOut &O, int nth, char *errbuf)
 {
-    const int MH = PFA_MAX_H11;
-    const int64_t n_p = in->n_p, n = out->n;
-    out->pdone = (int8_t *)calloc((size_t)std::max<int64_t>(n_p, 1), 1);
-    std::vector<int64_t> pb(n_p + 1, 0), rows(n_p, -1), base(n_p, 0);
-    for (int64_t t = 0; t < n; ++t) pb[out->pidx[t] + 1]++;
-    for (int64_t i = 0; i < n_p; ++i) pb[i + 1] += pb[i];
+    const int S = A.stride;
+    const int64_t nk = A.n_keys, n = A.n;
+    O.done = (int8_t *)calloc((size_t)std::max<int64_t>(nk, 1), 1);
+    std::vector<int64_t> pb(nk + 1, 0), rows(nk, -1), base(nk, 0);
+    for (int64_t t = 0; t < n; ++t) pb[A.key[t] + 1]++;
+    for (int64_t i = 0; i < nk; ++i) pb[i + 1] += pb[i];
     std::mutex emu;
     std::string emsg;
-    // rows of p-vector ip (-1: left to the caller); writes them from row
-    // `at` when write is set
+    // rows of key ip (-1: left to the caller); writes them from row `at`
+    // when write is set
     auto one = [&](int64_t ip, bool write, int64_t at) -> int64_t {
-        if (out->pstat[ip] != 0) return -1;
-        const int geo = in->pgeo[ip], h = in->h[geo];
-        const pfa_i128 Q = in->Q[geo];
+        if (A.skip && A.skip[ip]) return -1;
+        const int h = A.kh[ip];
+        const pfa_i128 Q = A.kQ[ip];
         struct Pt { int64_t t; pfa_i128 kg, raw; int64_t gmax; };
         std::vector<Pt> pts;
         int64_t G = 0;
         for (int64_t t = pb[ip]; t < pb[ip + 1]; ++t) {
-            if (out->nexp[t] <= 0) continue;
-            if (out->nonsing[t] == 2) return -1;
-            if (out->nonsing[t] != 1) continue;
-            const int64_t *Kn = out->Kn + t * MH, *M = out->M + t * MH;
+            if (A.nexp && A.nexp[t] <= 0) continue;
+            if (A.nonsing[t] == 2) return -1;
+            if (A.nonsing[t] != 1) continue;
+            const int64_t *Kn = A.Kn + t * S, *M = A.M + t * S;
             uint64_t kg = 0;
             for (int i = 1; i < h; ++i)
                 kg = pfa_box_gcd(kg, (uint64_t)(Kn[i] < 0 ? -(pfa_i128)Kn[i] : (pfa_i128)Kn[i]));
             if (kg == 0) kg = 1;
-            const pfa_i128 q = out->q[t], QK = Q * (pfa_i128)kg, M0 = M[0];
-            if (!(q < QK) || q <= 0 || M0 <= 0) return -1;
+            const pfa_i128 q = A.q[t], QK = Q * (pfa_i128)kg, M0 = M[0];
+            if (!(q < QK)) {                             // no g: the point gives no row
+                if (A.nexp) return -1;                   // (the filter counted some)
+                continue;
+            }
+            if (q <= 0 || M0 <= 0) return -1;
             const pfa_i128 num = q + M0 * (pfa_i128)Kn[0];
             if (num % (pfa_i128)kg != 0) return -1;
             const pfa_i128 gm = (QK - 1) / q;
@@ -2149,7 +2357,7 @@ static int assemble_pfvs(const pfa_run_input *in, pfa_run_output *out, int nth, 
             for (size_t k = 0; k < pts.size(); ++k) {
                 const Pt &P = pts[k];
                 if (g > P.gmax) continue;
-                const int64_t *Kn = out->Kn + P.t * MH, *M = out->M + P.t * MH;
+                const int64_t *Kn = A.Kn + P.t * S, *M = A.M + P.t * S;
                 const pfa_i128 M0 = M[0], a = (pfa_i128)g * P.raw - Q;
                 const pfa_i128 lo = pfa_cdiv_pos(a, M0);
                 pfa_i128 up = pfa_fdiv128(a, M0);
@@ -2159,7 +2367,7 @@ static int assemble_pfvs(const pfa_run_input *in, pfa_run_output *out, int nth, 
                 if (!pfa_fits64(lo) || !pfa_fits64(up)) return -1;
                 for (pfa_i128 K0 = lo; K0 <= up; ++K0) {
                     pfa_i128 tad = -(K0 * (pfa_i128)M[0]);
-                    int64_t *Ko = write ? out->Ks + (at + cnt) * MH : nullptr;
+                    int64_t *Ko = write ? O.Ks + (at + cnt) * S : nullptr;
                     if (Ko) Ko[0] = (int64_t)K0;
                     for (int i = 1; i < h; ++i) {
                         const pfa_i128 Ki = (pfa_i128)g * ((pfa_i128)Kn[i] / P.kg);
@@ -2178,25 +2386,25 @@ static int assemble_pfvs(const pfa_run_input *in, pfa_run_output *out, int nth, 
                         return -1;
                     }
                     if (write) {
-                        int64_t *Mo = out->Ms + (at + cnt) * MH;
-                        for (int i = 0; i < MH; ++i) Mo[i] = i < h ? M[i] : 0;
-                        for (int i = h; i < MH; ++i) out->Ks[(at + cnt) * MH + i] = 0;
-                        out->pfv_pidx[at + cnt] = ip;
+                        int64_t *Mo = O.Ms + (at + cnt) * S;
+                        for (int i = 0; i < S; ++i) Mo[i] = i < h ? M[i] : 0;
+                        for (int i = h; i < S; ++i) O.Ks[(at + cnt) * S + i] = 0;
+                        O.keys[at + cnt] = ip;
                     }
                     ++cnt;
                     ++per[k];
                 }
             }
-        // the device's expansion count of every point must agree
-        for (size_t k = 0; k < pts.size(); ++k)
-            if (per[k] != out->nexp[pts[k].t]) {
-                std::lock_guard<std::mutex> lk(emu);
-                if (emsg.empty()) emsg = "expansion count differs from the filter's";
-                return -1;
-            }
+        if (A.nexp)
+            for (size_t k = 0; k < pts.size(); ++k)
+                if (per[k] != A.nexp[pts[k].t]) {
+                    std::lock_guard<std::mutex> lk(emu);
+                    if (emsg.empty()) emsg = "expansion count differs from the filter's";
+                    return -1;
+                }
         return cnt;
     };
-    parallel_chunks(n_p, 64, nth, [&](int64_t lo, int64_t hi) {
+    parallel_chunks(nk, 64, nth, [&](int64_t lo, int64_t hi) {
         for (int64_t ip = lo; ip < hi; ++ip) rows[ip] = one(ip, false, 0);
     });
     if (!emsg.empty()) {
@@ -2204,30 +2412,58 @@ static int assemble_pfvs(const pfa_run_input *in, pfa_run_output *out, int nth, 
         return 1;
     }
     int64_t tot = 0;
-    for (int64_t ip = 0; ip < n_p; ++ip) {
+    for (int64_t ip = 0; ip < nk; ++ip) {
         base[ip] = tot;
         if (rows[ip] >= 0) tot += rows[ip];
     }
     const size_t nb = (size_t)std::max<int64_t>(tot, 1);
-    out->n_pfv = tot;
-    out->Ks = (int64_t *)malloc(8 * nb * MH);
-    out->Ms = (int64_t *)malloc(8 * nb * MH);
-    out->pfv_pidx = (int64_t *)malloc(8 * nb);
-    parallel_chunks(n_p, 64, nth, [&](int64_t lo, int64_t hi) {
+    O.n_rows = tot;
+    O.Ks = (int64_t *)malloc(8 * nb * S);
+    O.Ms = (int64_t *)malloc(8 * nb * S);
+    O.keys = (int64_t *)malloc(8 * nb);
+    parallel_chunks(nk, 64, nth, [&](int64_t lo, int64_t hi) {
         for (int64_t ip = lo; ip < hi; ++ip)
             if (rows[ip] >= 0) {
                 one(ip, true, base[ip]);
-                out->pdone[ip] = 1;
+                O.done[ip] = 1;
             }
     });
     return 0;
 }
 
-// Final assembly shared by the device and host runs: statuses, sort, copy.
+// The finish of a run: the core over the run's sorted output, keys = the
+// p-vectors, skipping those with a status.
+static int assemble_pfvs(const pfa_run_input *in, pfa_run_output *out, int nth, char *errbuf)
+{
+    const int64_t n_p = in->n_p;
+    std::vector<int32_t> kh(std::max<int64_t>(n_p, 1));
+    std::vector<int64_t> kQ(std::max<int64_t>(n_p, 1));
+    std::vector<int8_t> skip(std::max<int64_t>(n_p, 1));
+    for (int64_t ip = 0; ip < n_p; ++ip) {
+        kh[ip] = in->h[in->pgeo[ip]];
+        kQ[ip] = in->Q[in->pgeo[ip]];
+        skip[ip] = out->pstat[ip] != 0;
+    }
+    AsmIn A{n_p, out->n, out->pidx, out->M, out->Kn, out->q, out->nexp, out->nonsing,
+            PFA_MAX_H11, kh.data(), kQ.data(), skip.data()};
+    AsmOut O;
+    int rc = assemble_core(A, O, nth, errbuf);
+    out->pdone = O.done;
+    out->n_pfv = O.n_rows;
+    out->Ks = O.Ks;
+    out->Ms = O.Ms;
+    out->pfv_pidx = O.keys;
+    return rc;
+}
+
+// survivors_only: the arrays hold every point (the host path) and only
+// those the filter keeps for the PFVs (n_exp > 0, not singular) are
+// returned.  pcount: lattice points per p-vector.
 static int finish_run(const pfa_run_input *in, pfa_run_output *out, int64_t npts, int HS,
                       const int32_t *pidx32, const int64_t *q, const int64_t *M, const int64_t *Kn,
                       const int64_t *nexp, const int8_t *nonsing, const int32_t *status,
-                      const int32_t *code, const int8_t *bad, char *errbuf)
+                      const int32_t *code, const int8_t *bad, const int64_t *pcount,
+                      bool survivors_only, char *errbuf)
 {
     const int MH = PFA_MAX_H11;
     const int64_t n_p = in->n_p;
@@ -2244,7 +2480,11 @@ static int finish_run(const pfa_run_input *in, pfa_run_output *out, int64_t npts
     std::vector<int64_t> idx;
     idx.reserve(npts);
     for (int64_t k = 0; k < npts; ++k)
-        if (pidx32[k] >= 0 && out->pstat[pidx32[k]] == 0) idx.push_back(k);
+        if (pidx32[k] >= 0 && out->pstat[pidx32[k]] == 0 &&
+            !(survivors_only && nexp && nonsing && (nexp[k] <= 0 || nonsing[k] == 0)))
+            idx.push_back(k);
+    out->pcount = (int64_t *)malloc(sizeof(int64_t) * std::max<int64_t>(n_p, 1));
+    for (int64_t i = 0; i < n_p; ++i) out->pcount[i] = pcount ? pcount[i] : 0;
     std::vector<int64_t> pidx64(npts);
     for (int64_t k = 0; k < npts; ++k) pidx64[k] = pidx32[k];
     int64_t n_valid = 0;
@@ -2272,10 +2512,12 @@ static int finish_run(const pfa_run_input *in, pfa_run_output *out, int64_t npts
             out->nonsing[t] = nonsing ? nonsing[k] : 2;
         }
     });
+    // filter totals over every point of a finished p-vector (the returned
+    // ones, or all of them when only the survivors are returned)
     int64_t kept = 0, nex = 0, nns = 0, und = 0, nhost = 0;
     if (nexp)
-        for (int64_t t = 0; t < n; ++t) {
-            int64_t k = idx[t];
+        for (int64_t k = 0; k < npts; ++k) {
+            if (pidx32[k] < 0 || out->pstat[pidx32[k]] != 0) continue;
             if (nexp[k] > 0) {
                 kept++;
                 if (nonsing && nonsing[k] == 1) { nex += nexp[k]; nns++; }
@@ -2298,6 +2540,152 @@ static int finish_run(const pfa_run_input *in, pfa_run_output *out, int64_t npts
 }
 
 // --------------------------------------------------------- device run
+// The search over the prepared records (statuses PFA_OK where searchable)
+// in the chosen form, growing and rerunning on overflow: the point buffer
+// (all forms) and the plain form's deques.  A rerun restores the record
+// statuses the previous attempt changed (saved on the device first).
+struct SearchTune {
+    int mode, waves_per_cu, budget, stack_cap;
+};
+
+struct SearchRun {
+    unsigned long long npts = 0, queue_peak = 0;
+    double ms_roots = 0, ms_search = 0, wasted_ms = 0;
+    int passes = 0, launches = 0, attempts = 0, nwaves = 0, stack_cap = 0;
+    int32_t *d_pidx = nullptr;
+    int64_t *d_q = nullptr;
+    int32_t *d_c = nullptr;
+    unsigned long long *d_cnt = nullptr;     // nodes, cand, leaf, steps, wait, cycles, donated, fetches
+};
+
+static void search_device(DevCtx *C, hipStream_t s, int64_t *d_recs, const RecL &L, int64_t n_p,
+                          int D, const SearchTune &tn, unsigned long long pcap, SearchRun &R,
+                          char *errbuf)
+{
+    const EntL E = ent_layout(D);
+    // the batched and plain forms keep PFA_SL deque entries per wavefront in
+    // LDS: no more wavefronts per CU than LDS holds resident
+    const size_t bat_lds = (size_t)PFA_SL * E.bytes;
+    const size_t bat_lds_all = bat_lds + 64 * 4 + 16 + 64 * PFA_CSTRIDE * 4;
+    int waves_per_cu = tn.waves_per_cu > 0 ? tn.waves_per_cu : 4;
+    waves_per_cu = std::max(1, std::min<int>(waves_per_cu, (int)(C->lds_cu / bat_lds_all)));
+    const int donate_max = tn.budget > 0 ? tn.budget : 8;
+    const int donate_min = 64;               // deque depth before a wavefront donates
+    const int pt_chunk = 256;                // point slots reserved per atomic
+    int stack_cap = tn.stack_cap > 0 ? std::max(tn.stack_cap, 128) : 64 * (D + 2);
+    const bool plain = tn.mode == PFA_MODE_PLAIN, lane = tn.mode == PFA_MODE_LANE;
+    // plain: one workgroup per p-vector up to 8192, each p-vector's tree on
+    // the wavefront that starts it
+    const int nwaves = plain ? (int)std::max<int64_t>(1, std::min<int64_t>(n_p, 8192))
+                             : std::max(1, C->cus * waves_per_cu);
+    // room for every root plus the donation reserve kept by donate()
+    const unsigned long long ring_cap = (unsigned long long)std::max<int64_t>(2 * n_p + 64, 1 << 20) +
+                                        2 * 64ULL * 4096ULL;
+    const char *dl_env = getenv("PFVS_SEARCH_DEADLINE_S");
+    const double deadline_s = dl_env ? atof(dl_env) : 120.0;
+    const unsigned long long deadline =
+        (unsigned long long)(deadline_s * 1000.0 * (double)std::max(C->wclk_khz, 1));
+    hipEvent_t *ev = C->ev;
+    unsigned long long *d_ctl = (unsigned long long *)dev_get(C, B_CTL, CTL_BYTES, errbuf);
+    int *d_ovf = (int *)(d_ctl + 16 * 8);
+    R.d_cnt = (unsigned long long *)dev_get(C, B_CNT, 64, errbuf);
+    int32_t *d_save = (int32_t *)dev_get(C, B_STATSAVE, 4 * (size_t)std::max<int64_t>(n_p, 1), errbuf);
+    const int sgrid = (int)std::min<int64_t>((n_p + 255) / 256 + 1, 4096);
+    R.wasted_ms = 0;
+    for (int attempt = 0;; ++attempt) {
+        R.attempts = attempt + 1;
+        R.d_pidx = (int32_t *)dev_get(C, B_PT_PIDX, 4 * pcap, errbuf);
+        R.d_q = (int64_t *)dev_get(C, B_PT_Q, 8 * pcap, errbuf);
+        R.d_c = (int32_t *)dev_get(C, B_PT_C, 4 * pcap * D, errbuf);
+        HIPCHK(hipMemsetAsync(d_ctl, 0, CTL_BYTES, s));
+        HIPCHK(hipMemsetAsync(R.d_cnt, 0, 64, s));
+        if (n_p > 0) {
+            hipLaunchKernelGGL(k_status_io, dim3(sgrid), dim3(256), 0, s, n_p, d_recs, L.words, d_save,
+                               attempt == 0 ? 0 : 1);
+            HIPCHK(hipGetLastError());
+        }
+        PointArgs Pt{R.d_pidx, R.d_q, R.d_c, pcap, d_ctl + 16 * 6, d_ovf};
+        CountArgs Cn{R.d_cnt};
+        R.passes = 1;
+        R.launches = 0;
+        unsigned long long err = 0;
+        if (lane) {
+            HIPCHK(hipEventRecord(ev[2], s));
+            HIPCHK(hipEventRecord(ev[3], s));
+            if (n_p > 0) {
+                hipLaunchKernelGGL(k_lane, dim3((n_p + 63) / 64), dim3(64), 0, s, n_p, d_recs, L, Pt, D, Cn);
+                HIPCHK(hipGetLastError());
+                R.launches = 1;
+            }
+            HIPCHK(hipEventRecord(ev[4], s));
+        } else {
+            char *ring = nullptr;
+            unsigned long long *seq = nullptr;
+            if (!plain) {
+                ring = (char *)dev_get(C, B_RING, (size_t)ring_cap * E.bytes, errbuf);
+                seq = (unsigned long long *)dev_get(C, B_SEQ, 8 * ring_cap, errbuf);
+                HIPCHK(hipMemsetAsync(seq, 0, 8 * ring_cap, s));
+            }
+            char *stacks = (char *)dev_get(C, B_STACKS, (size_t)nwaves * stack_cap * E.bytes, errbuf);
+            QueueArgs Qa{ring, seq, ring_cap, d_ctl};
+            HIPCHK(hipEventRecord(ev[2], s));
+            if (!plain && n_p > 0) {
+                hipLaunchKernelGGL(k_roots, dim3((n_p + 255) / 256), dim3(256), 0, s, n_p, d_recs, L, Qa, E, Cn);
+                HIPCHK(hipGetLastError());
+            }
+            HIPCHK(hipEventRecord(ev[3], s));
+            BatchArgs A{d_recs, L, E, Qa, stacks, stack_cap, donate_max, donate_min, pt_chunk,
+                        Pt, Cn, deadline, n_p};
+            auto kern = plain ? k_search<true> : k_search<false>;
+            HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)bat_lds));
+            if (n_p > 0 || !plain) {
+                hipLaunchKernelGGL(kern, dim3(nwaves), dim3(64), bat_lds, s, A);
+                HIPCHK(hipGetLastError());
+                R.launches = 1;
+            }
+            HIPCHK(hipEventRecord(ev[4], s));
+        }
+        unsigned long long hctl[16 * 8];
+        HIPCHK(hipMemcpyAsync(hctl, d_ctl, sizeof hctl, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (!lane) {
+            R.queue_peak = hctl[16];             // tickets issued: roots + donations
+            err = hctl[80];
+            if (err == 1) {
+                snprintf(errbuf, 256, "frontier ring full (capacity %llu)", ring_cap);
+                throw 1;
+            }
+            if (err == 2) {
+                snprintf(errbuf, 256, "batched search exceeded its %.0f s deadline", deadline_s);
+                throw 1;
+            }
+            if (!plain && hctl[32] != 0) {
+                snprintf(errbuf, 256, "batched search ended with %lld units open", (long long)hctl[32]);
+                throw 1;
+            }
+        }
+        R.npts = hctl[16 * 6];
+        const int ovf = (int)(((const int *)(hctl + 16 * 8))[0]);
+        R.ms_roots = ms_between(ev[2], ev[3]);
+        R.ms_search = ms_between(ev[3], ev[4]);
+        C->pcap_hint = std::max(C->pcap_hint, R.npts);
+        if (!ovf && err != 3) {
+            R.npts = std::min<unsigned long long>(R.npts, pcap);   // trailing reserved slots
+            break;
+        }
+        R.wasted_ms += R.ms_roots + R.ms_search;
+        if (ovf) pcap = std::max<unsigned long long>(2 * pcap, R.npts + R.npts / 4 + 1024);
+        if (err == 3) stack_cap *= 2;
+        if (attempt > 8) {
+            snprintf(errbuf, 256, ovf ? "point buffer overflow" : "plain search deque overflow");
+            throw 1;
+        }
+    }
+    R.nwaves = nwaves;
+    R.stack_cap = stack_cap;
+}
+
 static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
 {
     auto t_wall = std::chrono::steady_clock::now();
@@ -2309,7 +2697,6 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     const int MH = PFA_MAX_H11, D = batch_dmax(in);
     const int64_t n_p = in->n_p;
     const RecL L = rec_layout(D);
-    const EntL E = ent_layout(D);
     std::vector<int64_t> kofs, mofs;
     int64_t ktot, mtot;
     fill_geo_offsets(in, kofs, mofs, ktot, mtot);
@@ -2324,6 +2711,7 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
         if (bytes) HIPCHK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, s));
         return d;
     };
+    auto t_up = std::chrono::steady_clock::now();
     GeoArgs G;
     G.h = (const int32_t *)up(B_GEO_H, in->h, 4 * in->n_geo);
     G.Q = (const int64_t *)up(B_GEO_Q, in->Q, 8 * in->n_geo);
@@ -2336,10 +2724,10 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     const int64_t *d_ps = (const int64_t *)up(B_PS, in->ps, 8 * n_p * MH);
     const int32_t *d_pgeo = (const int32_t *)up(B_PGEO, in->pgeo, 4 * n_p);
     int64_t *d_recs = (int64_t *)dev_get(C, B_RECS, 8 * (size_t)L.words * std::max<int64_t>(n_p, 1), errbuf);
-    unsigned long long *d_cnt = (unsigned long long *)dev_get(C, B_CNT, 64, errbuf);
+    const double ms_upload =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_up).count();
 
-    hipEvent_t ev[8];
-    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    hipEvent_t *ev = C->ev;
     HIPCHK(hipEventRecord(ev[0], s));
     if (n_p > 0) {
         // one workgroup per p-vector up to 16384 (each with its own cold
@@ -2383,168 +2771,106 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     // reservation (a search that overflows grows the buffer and reruns)
     if (C->pts_reserved > 0) est = std::min(est, std::max(C->pts_reserved, pcap));
     pcap = std::max(pcap, est);
-    // defaults measured on MI355X (h11 = 20 batch): fewer, deeper wavefronts
-    // keep the stacks' hot tops in L2
-    int waves_per_cu = in->waves_per_cu > 0 ? in->waves_per_cu : 4;
-    // the batched search keeps PFA_SL deque entries per wavefront in LDS:
-    // launch no more wavefronts per CU than LDS holds resident
-    const size_t bat_lds = (size_t)PFA_SL * E.bytes;
-    const size_t bat_lds_all = bat_lds + 64 * 4 + 16 + 64 * PFA_CSTRIDE * 4;
-    waves_per_cu = std::max(1, std::min<int>(waves_per_cu, (int)(C->lds_cu / bat_lds_all)));
-    const int donate_max = in->budget > 0 ? in->budget : 8;
-    const int donate_min = 64;               // deque depth before a wavefront donates
-    const int pt_chunk = 256;                // point slots reserved per atomic
-    const int stack_cap = in->stack_cap > 0 ? std::max(in->stack_cap, 128) : 64 * (D + 2);
-    const int nwaves = std::max(1, C->cus * waves_per_cu);
-    // room for every root plus the donation reserve kept by donate()
-    unsigned long long ring_cap = (unsigned long long)std::max<int64_t>(2 * n_p + 64, 1 << 20) +
-                                  2 * 64ULL * 4096ULL;
-    int wclk_khz = 0;
-    HIPCHK(hipDeviceGetAttribute(&wclk_khz, hipDeviceAttributeWallClockRate, in->device));
-    const char *dl_env = getenv("PFVS_SEARCH_DEADLINE_S");
-    const double deadline_s = dl_env ? atof(dl_env) : 120.0;
-    const unsigned long long deadline =
-        (unsigned long long)(deadline_s * 1000.0 * (double)std::max(wclk_khz, 1));
-    double ms_roots = 0, ms_search = 0, attempts_wasted_ms = 0;
-    int passes = 0, launches = 0;
-    unsigned long long npts = 0, queue_peak = 0;
-    int32_t *d_pidx = nullptr;
-    int64_t *d_q = nullptr;
-    int32_t *d_c = nullptr;
-    int *d_ovf = nullptr;
-    std::vector<int64_t> hdr0;
-    int attempts = 0;
-    for (int attempt = 0;; ++attempt) {
-        attempts = attempt + 1;
-        d_pidx = (int32_t *)dev_get(C, B_PT_PIDX, 4 * pcap, errbuf);
-        d_q = (int64_t *)dev_get(C, B_PT_Q, 8 * pcap, errbuf);
-        d_c = (int32_t *)dev_get(C, B_PT_C, 4 * pcap * D, errbuf);
-        unsigned long long *d_ctl = (unsigned long long *)dev_get(C, B_CTL, CTL_BYTES, errbuf);
-        d_ovf = (int *)(d_ctl + 16 * 8);
-        HIPCHK(hipMemsetAsync(d_ctl, 0, CTL_BYTES, s));
-        HIPCHK(hipMemsetAsync(d_cnt, 0, 64, s));
-        if (n_p > 0) {
-            // the search may change statuses: keep (first attempt) or restore
-            // (later attempts) the headers as the preparation left them
-            if (attempt == 0) {
-                hdr0.resize((size_t)n_p * PFA_HDR);
-                HIPCHK(hipMemcpy2DAsync(hdr0.data(), PFA_HDR * 8, d_recs, (size_t)L.words * 8, PFA_HDR * 8,
-                                        n_p, hipMemcpyDeviceToHost, s));
-            } else {
-                HIPCHK(hipMemcpy2DAsync(d_recs, (size_t)L.words * 8, hdr0.data(), PFA_HDR * 8, PFA_HDR * 8,
-                                        n_p, hipMemcpyHostToDevice, s));
-            }
-        }
-        PointArgs Pt{d_pidx, d_q, d_c, pcap, d_ctl + 16 * 6, d_ovf};
-        CountArgs Cn{d_cnt};
-        passes = 0;
-        launches = 0;
-        if (in->mode == PFA_MODE_PLAIN) {
-            HIPCHK(hipEventRecord(ev[2], s));
-            HIPCHK(hipEventRecord(ev[3], s));
-            if (n_p > 0) {
-                hipLaunchKernelGGL(k_plain, dim3((n_p + 63) / 64), dim3(64), 0, s, n_p, d_recs, L, Pt, D, Cn);
-                HIPCHK(hipGetLastError());
-                launches = 1;
-            }
-            HIPCHK(hipEventRecord(ev[4], s));
-            passes = 1;
-        } else {
-            char *ring = (char *)dev_get(C, B_RING, (size_t)ring_cap * E.bytes, errbuf);
-            unsigned long long *seq = (unsigned long long *)dev_get(C, B_SEQ, 8 * ring_cap, errbuf);
-            char *stacks = (char *)dev_get(C, B_STACKS, (size_t)nwaves * stack_cap * E.bytes, errbuf);
-            HIPCHK(hipMemsetAsync(seq, 0, 8 * ring_cap, s));
-            QueueArgs Qa{ring, seq, ring_cap, d_ctl};
-            HIPCHK(hipEventRecord(ev[2], s));
-            if (n_p > 0) {
-                hipLaunchKernelGGL(k_roots, dim3((n_p + 255) / 256), dim3(256), 0, s, n_p, d_recs, L, Qa, E, Cn);
-                HIPCHK(hipGetLastError());
-            }
-            HIPCHK(hipEventRecord(ev[3], s));
-            BatchArgs A{d_recs, L, E, Qa, stacks, stack_cap, donate_max, donate_min, pt_chunk,
-                        Pt, Cn, deadline};
-            HIPCHK(hipFuncSetAttribute((const void *)k_batched,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bat_lds));
-            hipLaunchKernelGGL(k_batched, dim3(nwaves), dim3(64), bat_lds, s, A);
-            HIPCHK(hipGetLastError());
-            launches = passes = 1;
-            HIPCHK(hipEventRecord(ev[4], s));
-            unsigned long long hctl[16 * 8];
-            HIPCHK(hipMemcpyAsync(hctl, d_ctl, sizeof hctl, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            queue_peak = hctl[16];             // tickets issued: roots + donations
-            if (hctl[80] == 1) {
-                snprintf(errbuf, 256, "frontier ring full (capacity %llu)", ring_cap);
-                return 1;
-            }
-            if (hctl[80] == 2) {
-                snprintf(errbuf, 256, "batched search exceeded its %.0f s deadline", deadline_s);
-                return 1;
-            }
-            if (hctl[32] != 0) {
-                snprintf(errbuf, 256, "batched search ended with %lld units open", (long long)hctl[32]);
-                return 1;
-            }
-        }
-        HIPCHK(hipMemcpyAsync(&npts, d_ctl + 16 * 6, 8, hipMemcpyDeviceToHost, s));
-        int ovf = 0;
-        HIPCHK(hipMemcpyAsync(&ovf, d_ovf, 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        ms_roots = ms_between(ev[2], ev[3]);
-        ms_search = ms_between(ev[3], ev[4]);
-        C->pcap_hint = std::max(C->pcap_hint, npts);
-        if (!ovf) {
-            npts = std::min<unsigned long long>(npts, pcap);   // trailing reserved slots
-            break;
-        }
-        attempts_wasted_ms += ms_roots + ms_search;
-        pcap = std::max<unsigned long long>(2 * pcap, npts + npts / 4 + 1024);
-        if (attempt > 6) {
-            snprintf(errbuf, 256, "point buffer overflow");
-            return 1;
-        }
-    }
-    // post: M, Kn, the expansion count and the det filter (k_filter); the
-    // rows are stored only when the points are kept
-    const int64_t n = (int64_t)npts;
+    SearchRun R;
+    SearchTune tn{in->mode, in->waves_per_cu, in->budget, in->stack_cap};
+    search_device(C, s, d_recs, L, n_p, D, tn, pcap, R, errbuf);
+    // post: M, Kn, the expansion count and the det filter (k_filter).
+    // keep_points 1: every point's rows are stored and copied back;
+    // 2 (with the filter): only the points that can yield PFVs, gathered
+    // by k_compact; 0: counters only
+    const int64_t n = (int64_t)R.npts;
     const int HS = D + 1;                    // row stride of M, Kn on the device
-    const bool keep = in->keep_points != 0;
-    int64_t *d_M = keep ? (int64_t *)dev_get(C, B_M, 8 * std::max<int64_t>(n, 1) * HS, errbuf) : nullptr;
-    int64_t *d_Kn = keep ? (int64_t *)dev_get(C, B_KN, 8 * std::max<int64_t>(n, 1) * HS, errbuf) : nullptr;
-    int64_t *d_nexp = keep && in->filter ? (int64_t *)dev_get(C, B_NEXP, 8 * std::max<int64_t>(n, 1), errbuf) : nullptr;
+    const int keep = in->keep_points == 2 && !in->filter ? 1 : in->keep_points;
+    const bool rows = keep == 1;
+    int64_t *d_M = rows ? (int64_t *)dev_get(C, B_M, 8 * std::max<int64_t>(n, 1) * HS, errbuf) : nullptr;
+    int64_t *d_Kn = rows ? (int64_t *)dev_get(C, B_KN, 8 * std::max<int64_t>(n, 1) * HS, errbuf) : nullptr;
+    int64_t *d_nexp = rows && in->filter ? (int64_t *)dev_get(C, B_NEXP, 8 * std::max<int64_t>(n, 1), errbuf) : nullptr;
     int8_t *d_ns = keep && in->filter ? (int8_t *)dev_get(C, B_NONSING, std::max<int64_t>(n, 1), errbuf) : nullptr;
     int8_t *d_bad = (int8_t *)dev_get(C, B_BAD, std::max<int64_t>(n_p, 1), errbuf);
     unsigned long long *d_tot = (unsigned long long *)dev_get(C, B_TOT, 64, errbuf);
+    unsigned long long *d_pcount =
+        (unsigned long long *)dev_get(C, B_PCOUNT, 8 * (size_t)std::max<int64_t>(n_p, 1), errbuf);
+    HIPCHK(hipMemsetAsync(d_pcount, 0, 8 * (size_t)std::max<int64_t>(n_p, 1), s));
     HIPCHK(hipMemsetAsync(d_bad, 0, std::max<int64_t>(n_p, 1), s));
     HIPCHK(hipMemsetAsync(d_tot, 0, 64, s));
     HIPCHK(hipEventRecord(ev[5], s));
-    {
-        FiltArgs Fa{};
-        Fa.n = n;
-        Fa.pidx = d_pidx;
-        Fa.q = d_q;
-        Fa.c = d_c;
-        Fa.Min = nullptr;
-        Fa.D = D;
-        Fa.MH = HS;
-        Fa.hl = HS;
-        Fa.recs = d_recs;
-        Fa.L = L;
-        Fa.filter = in->filter;
-        Fa.M = d_M;
-        Fa.Kn = d_Kn;
-        Fa.nexp = d_nexp;
-        Fa.nonsing = d_ns;
-        Fa.bad = d_bad;
-        Fa.tot = d_tot;
-        Fa.G = G;
-        filter_run(C, s, in, kofs, Fa, errbuf);
+    FiltArgs Fa{};
+    Fa.n = n;
+    Fa.pidx = R.d_pidx;
+    Fa.q = R.d_q;
+    Fa.c = R.d_c;
+    Fa.Min = nullptr;
+    Fa.D = D;
+    Fa.MH = HS;
+    Fa.hl = HS;
+    Fa.recs = d_recs;
+    Fa.L = L;
+    Fa.filter = in->filter;
+    Fa.M = d_M;
+    Fa.Kn = d_Kn;
+    Fa.nexp = d_nexp;
+    Fa.nonsing = d_ns;
+    Fa.bad = d_bad;
+    Fa.tot = d_tot;
+    Fa.pcount = d_pcount;
+    Fa.G = G;
+    filter_run(C, s, in, kofs, Fa, errbuf);
+    // the survivors (keep 2): compacted; their count read back, with a
+    // second pass if they outgrew the buffer
+    int64_t nsv = 0;
+    int32_t *d_sv_pidx = nullptr;
+    int64_t *d_sv_q = nullptr, *d_sv_M = nullptr, *d_sv_Kn = nullptr, *d_sv_nexp = nullptr;
+    int8_t *d_sv_ns = nullptr;
+    if (keep == 2 && n > 0) {
+        unsigned long long *d_svc = (unsigned long long *)dev_get(C, B_SV_CNT, 64, errbuf);
+        unsigned long long cap = std::max<unsigned long long>(1 << 16, C->sv_hint + C->sv_hint / 4);
+        for (int pass = 0; pass < 2; ++pass) {
+            CompactArgs Ca{};
+            // the filter's list of points with n_exp > 0 (filled by k_post)
+            Ca.keepkp = (const int64_t *)dev_get(C, B_KEEPKP, 8 * n, errbuf);
+            Ca.keepnexp = (const int64_t *)dev_get(C, B_KEEPNEXP, 8 * n, errbuf);
+            Ca.keepcount = (const unsigned long long *)dev_get(C, B_KEEPCNT, 8, errbuf);
+            Ca.nonsing = d_ns;
+            Ca.pidx = R.d_pidx;
+            Ca.q = R.d_q;
+            Ca.c = R.d_c;
+            Ca.D = D;
+            Ca.HS = HS;
+            Ca.recs = d_recs;
+            Ca.L = L;
+            Ca.o_pidx = d_sv_pidx = (int32_t *)dev_get(C, B_SV_PIDX, 4 * cap, errbuf);
+            Ca.o_q = d_sv_q = (int64_t *)dev_get(C, B_SV_Q, 8 * cap, errbuf);
+            Ca.o_M = d_sv_M = (int64_t *)dev_get(C, B_SV_M, 8 * cap * HS, errbuf);
+            Ca.o_Kn = d_sv_Kn = (int64_t *)dev_get(C, B_SV_KN, 8 * cap * HS, errbuf);
+            Ca.o_nexp = d_sv_nexp = (int64_t *)dev_get(C, B_SV_NEXP, 8 * cap, errbuf);
+            Ca.o_ns = d_sv_ns = (int8_t *)dev_get(C, B_SV_NS, cap, errbuf);
+            Ca.o_count = d_svc;
+            Ca.cap = cap;
+            HIPCHK(hipMemsetAsync(d_svc, 0, 8, s));
+            const int cgrid = (int)std::min<int64_t>((n + 63) / 64, 4096);
+            hipLaunchKernelGGL(k_compact, dim3(cgrid), dim3(64), 0, s, Ca);
+            HIPCHK(hipGetLastError());
+            unsigned long long cnt = 0;
+            HIPCHK(hipMemcpyAsync(&cnt, d_svc, 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            nsv = (int64_t)cnt;
+            C->sv_hint = std::max(C->sv_hint, cnt);
+            if (cnt <= cap) break;
+            cap = cnt + cnt / 4 + 1024;
+        }
+    }
+    // statuses and stage codes of the p-vectors, gathered for one copy
+    int32_t *d_stat = (int32_t *)dev_get(C, B_STATOUT, 8 * (size_t)std::max<int64_t>(n_p, 1), errbuf);
+    if (n_p > 0) {
+        hipLaunchKernelGGL(k_status_io, dim3((int)std::min<int64_t>((n_p + 255) / 256, 4096)), dim3(256), 0,
+                           s, n_p, d_recs, L.words, d_stat, 2);
+        HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(ev[6], s));
     // results back, into the pinned staging buffers
     HIPCHK(hipEventSynchronize(ev[6]));
     auto t_d2h = std::chrono::steady_clock::now();
-    const int64_t nh = keep ? n : 0;          // points copied back
+    const int64_t nh = keep == 1 ? n : (keep == 2 ? nsv : 0);   // points copied back
     const size_t nb1 = (size_t)std::max<int64_t>(nh, 1);
     Stage &S = C->stage[C->next_stage];
     C->next_stage ^= 1;
@@ -2555,58 +2881,67 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     int64_t *h_Kn = (int64_t *)host_get(C, S, H_KN, 8 * nb1 * HS, errbuf);
     int64_t *h_nexp = (int64_t *)host_get(C, S, H_NEXP, 8 * nb1, errbuf);
     int8_t *h_ns = (int8_t *)host_get(C, S, H_NS, nb1, errbuf);
+    int32_t *h_stat = (int32_t *)host_get(C, S, H_STAT, 8 * (size_t)std::max<int64_t>(n_p, 1), errbuf);
     std::vector<int32_t> h_status(n_p), h_code(n_p);
     std::vector<int8_t> h_bad(n_p);
-    std::vector<int64_t> hdr((size_t)n_p * PFA_HDR);
+    std::vector<int64_t> h_pcount(std::max<int64_t>(n_p, 1), 0);
     unsigned long long htot[8] = {0};
     HIPCHK(hipMemcpyAsync(htot, d_tot, 64, hipMemcpyDeviceToHost, s));
-    if (nh > 0) {
-        HIPCHK(hipMemcpyAsync(h_pidx, d_pidx, 4 * n, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(h_q, d_q, 8 * n, hipMemcpyDeviceToHost, s));
+    if (nh > 0 && keep == 1) {
+        HIPCHK(hipMemcpyAsync(h_pidx, R.d_pidx, 4 * n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h_q, R.d_q, 8 * n, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(h_M, d_M, 8 * n * HS, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(h_Kn, d_Kn, 8 * n * HS, hipMemcpyDeviceToHost, s));
         if (in->filter) {
             HIPCHK(hipMemcpyAsync(h_nexp, d_nexp, 8 * n, hipMemcpyDeviceToHost, s));
             HIPCHK(hipMemcpyAsync(h_ns, d_ns, n, hipMemcpyDeviceToHost, s));
         }
+    } else if (nh > 0) {
+        HIPCHK(hipMemcpyAsync(h_pidx, d_sv_pidx, 4 * nh, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h_q, d_sv_q, 8 * nh, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h_M, d_sv_M, 8 * nh * HS, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h_Kn, d_sv_Kn, 8 * nh * HS, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h_nexp, d_sv_nexp, 8 * nh, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h_ns, d_sv_ns, nh, hipMemcpyDeviceToHost, s));
     }
     if (n_p > 0) {
-        HIPCHK(hipMemcpy2DAsync(hdr.data(), PFA_HDR * 8, d_recs, (size_t)L.words * 8, PFA_HDR * 8, n_p,
-                                hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h_stat, d_stat, 8 * n_p, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(h_bad.data(), d_bad, n_p, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h_pcount.data(), d_pcount, 8 * n_p, hipMemcpyDeviceToHost, s));
     }
     unsigned long long hc[8] = {0};
-    HIPCHK(hipMemcpyAsync(hc, d_cnt, 64, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hc, R.d_cnt, 64, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     for (int64_t ip = 0; ip < n_p; ++ip) {
-        const int32_t *ri = (const int32_t *)&hdr[ip * PFA_HDR];
-        h_status[ip] = ri[0];
-        h_code[ip] = ri[25];
+        h_status[ip] = h_stat[2 * ip];
+        h_code[ip] = h_stat[2 * ip + 1];
     }
     out->stats[PFA_ST_NODES] = (double)hc[0];
     out->stats[PFA_ST_CAND] = (double)hc[1];
     out->stats[PFA_ST_LEAF] = (double)hc[2];
-    out->stats[PFA_ST_PASSES] = passes;
-    out->stats[PFA_ST_SEARCH_LAUNCHES] = launches;
+    out->stats[PFA_ST_PASSES] = R.passes;
+    out->stats[PFA_ST_SEARCH_LAUNCHES] = R.launches;
     out->stats[PFA_ST_MS_PREP] = ms_between(ev[0], ev[1]);
-    out->stats[PFA_ST_MS_ROOTS] = ms_roots;
-    out->stats[PFA_ST_MS_SEARCH] = ms_search;
+    out->stats[PFA_ST_MS_ROOTS] = R.ms_roots;
+    out->stats[PFA_ST_MS_SEARCH] = R.ms_search;
     out->stats[PFA_ST_MS_POST] = ms_between(ev[5], ev[6]);
     // (a search rerun after the point buffer overflowed counts too)
-    out->stats[PFA_ST_MS_TOTAL] = out->stats[PFA_ST_MS_PREP] + ms_roots + ms_search +
-                                  out->stats[PFA_ST_MS_POST] + attempts_wasted_ms;
-    out->stats[PFA_ST_QUEUE_PEAK] = (double)queue_peak;
+    out->stats[PFA_ST_MS_TOTAL] = out->stats[PFA_ST_MS_PREP] + R.ms_roots + R.ms_search +
+                                  out->stats[PFA_ST_MS_POST] + R.wasted_ms;
+    out->stats[PFA_ST_QUEUE_PEAK] = (double)R.queue_peak;
     out->stats[PFA_ST_MS_D2H] =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_d2h).count();
-    out->stats[PFA_ST_ATTEMPTS] = (double)attempts;
+    out->stats[PFA_ST_ATTEMPTS] = (double)R.attempts;
     out->stats[PFA_ST_MS_ALLOC] = C->ms_alloc;
+    out->stats[PFA_ST_MS_UPLOAD] = ms_upload;
     out->stats[PFA_ST_STEPS] = (double)hc[3];
     out->stats[PFA_ST_WAIT_CYC] = (double)hc[4];
     out->stats[PFA_ST_WAVE_CYC] = (double)hc[5];
     out->stats[PFA_ST_DONATED] = (double)hc[6];
     out->stats[PFA_ST_FETCHES] = (double)hc[7];
-    for (auto &e : ev) (void)hipEventDestroy(e);
-    if (n_p >= 64) C->ppp_hint = std::max(C->ppp_hint, (double)npts / (double)n_p);
+    out->stats[PFA_ST_WAVES] = R.nwaves;
+    out->stats[PFA_ST_SURVIVORS] = (double)nsv;
+    if (n_p >= 64) C->ppp_hint = std::max(C->ppp_hint, (double)R.npts / (double)n_p);
     lk.unlock();                              // the device is free; assemble from the stage
     // counts-only totals from the device (htot: points, kept, expansions,
     // nonsingular, undecided)
@@ -2614,8 +2949,9 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
                          (double)htot[0]};
     int rc = finish_run(in, out, nh, HS, h_pidx, h_q, h_M, h_Kn,
                         in->filter ? h_nexp : nullptr, in->filter ? h_ns : nullptr,
-                        h_status.data(), h_code.data(), h_bad.data(), errbuf);
-    if (!in->keep_points) {
+                        h_status.data(), h_code.data(), h_bad.data(), h_pcount.data(), false,
+                        errbuf);
+    if (keep != 1) {
         out->stats[PFA_ST_POINTS] = st_keep[4];
         out->stats[PFA_ST_KEPT] = st_keep[0];
         out->stats[PFA_ST_EXPANSIONS] = st_keep[1];
@@ -2645,6 +2981,7 @@ static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
     std::vector<PfaCounts> tc(nth, PfaCounts{0, 0, 0});
     std::vector<PfaWork> tw(nth, PfaWork{0, 0, 0, 0, 0, 0});
     std::vector<int64_t> tn(nth, 0);                 // points found per thread
+    std::vector<int64_t> pcount(std::max<int64_t>(n_p, 1), 0);   // points per p-vector
     const bool store = in->keep_points || in->filter; // else counts only
     std::atomic<int64_t> next{0};
     auto worker = [&](int t) {
@@ -2679,6 +3016,7 @@ static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
                 hp.c.resize(mark * D);
             } else {
                 tn[t] += found;
+                pcount[ip] = found;
             }
         }
     };
@@ -2756,7 +3094,8 @@ static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
     out->stats[PFA_ST_MS_SEARCH] = std::chrono::duration<double, std::milli>(t1 - t0).count();
     int rc = finish_run(in, out, n, HS, all.pidx.data(), all.q.data(), M.data(), Kn.data(),
                         in->filter ? nexp.data() : nullptr, in->filter ? ns.data() : nullptr,
-                        status.data(), code.data(), bad.data(), errbuf);
+                        status.data(), code.data(), bad.data(), pcount.data(),
+                        in->keep_points == 2 && in->filter, errbuf);
     if (!store) {
         int64_t np = 0;
         for (int64_t x : tn) np += x;
@@ -2802,12 +3141,64 @@ extern "C" void pfa_run_output_free(pfa_run_output *out)
 {
     free(out->M); free(out->Kn); free(out->q); free(out->pidx);
     free(out->pstat); free(out->pcode); free(out->nexp); free(out->nonsing);
-    free(out->Ks); free(out->Ms); free(out->pfv_pidx); free(out->pdone);
-    out->M = out->Kn = out->q = out->pidx = out->nexp = nullptr;
+    free(out->Ks); free(out->Ms); free(out->pfv_pidx); free(out->pdone); free(out->pcount);
+    out->M = out->Kn = out->q = out->pidx = out->nexp = out->pcount = nullptr;
     out->Ks = out->Ms = out->pfv_pidx = nullptr;
     out->pstat = nullptr; out->nonsing = nullptr; out->pcode = nullptr; out->pdone = nullptr;
     out->n = 0;
     out->n_pfv = 0;
+}
+
+extern "C" int pfa_assemble(int h, int64_t Q, int64_t n_keys, int64_t n, const int64_t *key,
+                            const int64_t *M, const int64_t *Kn, const int64_t *q,
+                            const int8_t *nonsing, int threads, pfa_assemble_output *out)
+{
+    memset(out, 0, sizeof(*out));
+    char *errbuf = out->err;
+    if (h < 2 || h > PFA_MAX_H11 || n < 0 || n_keys < 0) {
+        snprintf(errbuf, 256, "pfa_assemble: bad sizes (h %d, n %lld, keys %lld)", h, (long long)n,
+                 (long long)n_keys);
+        return 1;
+    }
+    for (int64_t t = 0; t < n; ++t)
+        if (key[t] < 0 || key[t] >= n_keys) {
+            snprintf(errbuf, 256, "pfa_assemble: key out of range");
+            return 1;
+        }
+    const int nth = threads > 0 ? threads : std::max(1, (int)std::thread::hardware_concurrency());
+    std::vector<int64_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    canonical_order(idx, key, M, Kn, h, n_keys, nth);
+    std::vector<int64_t> sk(n), sM((size_t)n * h), sK((size_t)n * h), sq(n);
+    std::vector<int8_t> sns(n);
+    for (int64_t t = 0; t < n; ++t) {
+        const int64_t k = idx[t];
+        sk[t] = key[k];
+        sq[t] = q[k];
+        sns[t] = nonsing[k];
+        memcpy(&sM[t * h], M + k * h, 8 * h);
+        memcpy(&sK[t * h], Kn + k * h, 8 * h);
+    }
+    std::vector<int32_t> kh(std::max<int64_t>(n_keys, 1), h);
+    std::vector<int64_t> kQ(std::max<int64_t>(n_keys, 1), Q);
+    AsmIn A{n_keys, n, sk.data(), sM.data(), sK.data(), sq.data(), nullptr, sns.data(), h,
+            kh.data(), kQ.data(), nullptr};
+    AsmOut O;
+    int rc = assemble_core(A, O, nth, errbuf);
+    out->n_rows = O.n_rows;
+    out->Ks = O.Ks;
+    out->Ms = O.Ms;
+    out->keys = O.keys;
+    out->done = O.done;
+    return rc;
+}
+
+extern "C" void pfa_assemble_output_free(pfa_assemble_output *out)
+{
+    free(out->Ks); free(out->Ms); free(out->keys); free(out->done);
+    out->Ks = out->Ms = out->keys = nullptr;
+    out->done = nullptr;
+    out->n_rows = 0;
 }
 
 // ----------------------------------------------- reference-compatible ABI
@@ -2833,7 +3224,7 @@ extern "C" int pfg_coni_batch(const pfg_input *in, pfg_output *out)
     out->pstat = o.pstat;
     for (int64_t i = 0; i < in->n_p; ++i) if (out->pstat[i] != 0) out->pstat[i] = 1;
     out->seconds_gpu = o.stats[PFA_ST_MS_TOTAL] * 1e-3;
-    free(o.pcode); free(o.nexp); free(o.nonsing);
+    free(o.pcode); free(o.nexp); free(o.nonsing); free(o.pcount);
     return 0;
 }
 
@@ -3275,30 +3666,35 @@ extern "C" int pfa_kernel_batch(const pfa_kernel_input *in, pfa_kernel_output *o
                 hipLaunchKernelGGL(k_factor, dim3((unsigned)std::min<int64_t>(n, 65536)), dim3(64),
                                    (size_t)2 * D * D * 8 + 64, s, n, d_recs, L);
             unsigned long long pcap = std::max<int64_t>(1 << 14, 64 * n);
-            unsigned long long *d_cnt = (unsigned long long *)dev_get(C, B_CNT, 64, errbuf);
-            unsigned long long *d_ctl = (unsigned long long *)dev_get(C, B_CTL, CTL_BYTES, errbuf);
-            int32_t *d_pidx = (int32_t *)dev_get(C, B_PT_PIDX, 4 * pcap, errbuf);
-            int64_t *d_q = (int64_t *)dev_get(C, B_PT_Q, 8 * pcap, errbuf);
-            int32_t *d_c = (int32_t *)dev_get(C, B_PT_C, 4 * pcap * D, errbuf);
-            HIPCHK(hipMemsetAsync(d_ctl, 0, CTL_BYTES, s));
-            HIPCHK(hipMemsetAsync(d_cnt, 0, 64, s));
-            PointArgs Pt{d_pidx, d_q, d_c, pcap, d_ctl + 16 * 6, (int *)(d_ctl + 16 * 8)};
-            CountArgs Cn{d_cnt};
-            if (n > 0) hipLaunchKernelGGL(k_plain, dim3((n + 63) / 64), dim3(64), 0, s, n, d_recs, L, Pt, D, Cn);
-            HIPCHK(hipGetLastError());
-            unsigned long long npts = 0;
-            int ovf = 0;
-            HIPCHK(hipMemcpyAsync(&npts, d_ctl + 16 * 6, 8, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipMemcpyAsync(&ovf, d_ctl + 16 * 8, 4, hipMemcpyDeviceToHost, s));
+            // the same search forms as the pipeline (lane, plain, batched),
+            // with its overflow reruns
+            SearchRun R;
+            SearchTune tn{in->mode, 0, 0, 0};
+            search_device(C, s, d_recs, L, n, D, tn, pcap, R, errbuf);
+            const unsigned long long npts = R.npts;
             HIPCHK(hipMemcpyAsync(recs.data(), d_recs, 8 * recs.size(), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
-            if (ovf || npts > pcap) { snprintf(errbuf, 256, "point buffer overflow"); return 1; }
+            int32_t *d_pidx = R.d_pidx;
+            int64_t *d_q = R.d_q;
+            int32_t *d_c = R.d_c;
             pidx.resize(npts); qs.resize(npts); cs.resize(npts * D);
             if (npts) {
                 HIPCHK(hipMemcpy(pidx.data(), d_pidx, 4 * npts, hipMemcpyDeviceToHost));
                 HIPCHK(hipMemcpy(qs.data(), d_q, 8 * npts, hipMemcpyDeviceToHost));
                 HIPCHK(hipMemcpy(cs.data(), d_c, 4 * npts * D, hipMemcpyDeviceToHost));
             }
+        }
+        // the chunked forms leave unused point slots (pidx -1): drop them
+        {
+            int64_t w = 0;
+            for (int64_t t = 0; t < (int64_t)qs.size(); ++t) {
+                if (pidx[t] < 0) continue;
+                pidx[w] = pidx[t];
+                qs[w] = qs[t];
+                memmove(&cs[w * D], &cs[t * D], 4 * D);
+                w++;
+            }
+            pidx.resize(w); qs.resize(w); cs.resize(w * D);
         }
         // per-problem depth-first order: lexicographic in (c[d-1], ..., c[0])
         const int64_t np = (int64_t)qs.size();

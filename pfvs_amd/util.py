@@ -1,4 +1,7 @@
-"""Exact integer helpers of the coniZpM drop-in (pfvs/util.py:41-259)."""
+"""Exact integer helpers of the coniZpM drop-in: the error of an incomplete
+search (the reference's IncompleteSearchError, pfvs/util.py:41-45), the
+integral-input check, and products and singularity decisions on Python
+integers wherever int64 could wrap."""
 
 from __future__ import annotations
 
@@ -6,68 +9,76 @@ import numpy as np
 
 from . import lattice
 
+_I64_LIMIT = 1 << 63
+
 
 class IncompleteSearchError(RuntimeError):
     """A p-vector could not be searched exactly and completely (overflow, a
-    non-positive-definite ellipsoid, or more than max_N_pfvs outputs);
-    pfvs/util.py:41-45."""
+    non-positive-definite ellipsoid, or more than max_N_pfvs outputs); a
+    p-vector is never skipped silently."""
 
 
-def absmax(A) -> int:
+def magnitude(A) -> int:
+    """max |a| over the entries, as a Python int (0 when empty)."""
+    flat = np.asarray(A).ravel()
+    if flat.size == 0:
+        return 0
+    if flat.dtype == object:
+        return max(abs(int(x)) for x in flat)
+    lo, hi = int(flat.min()), int(flat.max())
+    return max(-lo, hi)
+
+
+def fits_int64(A) -> bool:
+    return magnitude(A) < _I64_LIMIT
+
+
+def to_pyints(A):
+    """An object array of Python ints with A's shape (exact arithmetic)."""
     A = np.asarray(A)
-    return max(-int(A.min()), int(A.max())) if A.size else 0
-
-
-def as_exact(*arrays):
-    """Object arrays of Python ints (no wraparound)."""
-    return tuple(np.array(np.asarray(a).tolist(), dtype=object).reshape(np.shape(a))
-                 for a in arrays)
+    out = np.empty(A.shape, dtype=object)
+    out.ravel()[:] = [int(x) for x in A.ravel()]
+    return out
 
 
 def as_integral(A):
-    """A as an integer (or object) array; raises on non-integral entries
-    (pfvs/util.py:170-179)."""
+    """A as an integer array: int64, or Python ints (object) when an entry
+    needs more.  Floats must be integral and below 2^53 in magnitude, where
+    every float is an exact integer; anything else raises ValueError."""
     A = np.asarray(A)
-    if A.dtype.kind in "iuO":
-        return A
-    if not np.all(np.isfinite(A)) or not np.all(A == np.round(A)):
+    if A.dtype.kind in "iu":
+        return A if fits_int64(A) else to_pyints(A)
+    if A.dtype == object:
+        vals = list(A.ravel())
+        if any(int(x) != x for x in vals):
+            raise ValueError("expected an integral array")
+        P = to_pyints(A)
+        return P.astype(np.int64) if fits_int64(P) else P
+    if A.size and not np.all(np.isfinite(A)):
         raise ValueError("expected an integral array")
-    if np.max(np.abs(A), initial=0) >= 2.0**53:
+    if A.size and np.any(A != np.rint(A)):
+        raise ValueError("expected an integral array")
+    if A.size and float(np.max(np.abs(A))) >= 2.0**53:
         raise ValueError("float input too large to be exactly integral")
     return A.astype(np.int64)
 
 
 def exact_matmul(A, B):
-    """Integer A @ B without int64 wraparound (pfvs/util.py:246-259)."""
+    """A @ B on integers with no wraparound: int64 when the bound
+    inner * max|A| * max|B| keeps every partial sum inside int64, else on
+    Python ints (returned as int64 when the result fits)."""
     A, B = as_integral(A), as_integral(B)
-    if A.dtype != object and B.dtype != object:
-        k = A.shape[-1] if A.ndim else 1
-        if k * absmax(A) * absmax(B) < 2**63:
-            return A.astype(np.int64, copy=False) @ B.astype(np.int64, copy=False)
-    Ao = np.array(A.tolist(), dtype=object).reshape(A.shape)
-    Bo = np.array(B.tolist(), dtype=object).reshape(B.shape)
-    R = Ao @ Bo
-    if R.size == 0 or all(-2**63 <= int(x) < 2**63 for x in R.ravel()):
-        return np.array(R.tolist(), dtype=np.int64).reshape(R.shape)
-    return R
-
-
-def colsum_prod(A, B):
-    """sum(A * B, axis=0) exactly (pfvs/util.py:214-221)."""
-    A, B = np.asarray(A), np.asarray(B)
+    inner = A.shape[-1] if A.ndim else 1
     if A.dtype != object and B.dtype != object and \
-            (A.shape[0] if A.ndim else 1) * absmax(A) * absmax(B) < 2**62:
-        return np.sum(A * B, axis=0)
-    A, B = as_exact(A, B)
-    return np.sum(A * B, axis=0)
+            inner * magnitude(A) * magnitude(B) < _I64_LIMIT:
+        return A @ B
+    R = to_pyints(A) @ to_pyints(B)
+    return R.astype(np.int64) if fits_int64(R) else R
 
 
 def singular_mask(Ns) -> np.ndarray:
-    """Exactly which integer matrices of the stack Ns (n, m, m) are singular
-    (pfvs/util.py:224-243): a fraction-free determinant on Python ints."""
+    """Which integer matrices of the stack Ns (n, m, m) are singular, by a
+    fraction-free determinant on Python ints (exact)."""
     Ns = np.asarray(Ns)
-    n = Ns.shape[0]
-    out = np.zeros(n, dtype=bool)
-    for k in range(n):
-        out[k] = lattice.det_bareiss([[int(x) for x in r] for r in Ns[k]]) == 0
-    return out
+    return np.array([lattice.det_bareiss([[int(x) for x in row] for row in N]) == 0 for N in Ns],
+                    dtype=bool).reshape(len(Ns))

@@ -74,3 +74,45 @@ def oracle_pfvs(data, ps, Q, dilation, M0min=13):
     Ks = np.array([r[0] for r in rows], dtype=np.int64).reshape(-1, h)
     Ms = np.array([r[1] for r in rows], dtype=np.int64).reshape(-1, h)
     return Ks, Ms
+
+
+def _packed_keys(cols):
+    """Exact lexicographic keys for integer columns: each column shifted to
+    start at 0 and packed, most significant first, into as few uint64 words
+    as its value range allows (the order of the packed words is the
+    lexicographic order of the columns)."""
+    words, cur, used = [], None, 0
+    for c in cols:
+        c = np.asarray(c, dtype=np.int64)
+        lo = int(c.min()) if c.size else 0
+        span = (int(c.max()) - lo) if c.size else 0
+        bits = max(1, span.bit_length())
+        v = (c - lo).astype(np.uint64)
+        if cur is None or used + bits > 64:
+            if cur is not None:
+                words.append(cur)
+            cur, used = np.zeros(len(c), dtype=np.uint64), 0
+        cur = (cur << np.uint64(bits)) | v
+        used += bits
+    if cur is not None:
+        words.append(cur)
+    return words
+
+
+def canon_arrays(M, Kn, q, pidx, h):
+    """(pidx, M[:, :h], Kn[:, :h], q) sorted by (pidx, M): a canonical order
+    of a point set (M determines the lattice point, so no two points of one
+    p-vector share M), computed exactly and vectorised for millions of
+    points."""
+    M, Kn = np.asarray(M)[:, :h], np.asarray(Kn)[:, :h]
+    q, pidx = np.asarray(q, dtype=np.int64), np.asarray(pidx, dtype=np.int64)
+    if len(q) == 0:
+        return pidx, M, Kn, q
+    keys = _packed_keys([pidx] + [M[:, i] for i in range(h)])
+    order = np.lexsort(keys[::-1])
+    return pidx[order], M[order], Kn[order], q[order]
+
+
+def same_points(a, b):
+    """Two canon_arrays results are identical (every array equal)."""
+    return all(np.array_equal(x, y) for x, y in zip(a, b))

@@ -49,18 +49,18 @@ def test_cpu_matches_oracle_postprocessing():
     np.testing.assert_array_equal(Ms, Mo)
 
 
-def test_extras_pass_the_seven_checks():
+def test_every_output_passes_the_pfv_checks():
+    """Every PFV coniZpM returns (the dataset's and the extras) passes the
+    oracle's restatement of PFV.check_all (pfv.py:606-715)."""
     g = max(SMALL, key=lambda g: len(g["K"]))
     data = cydata_of(g)
     ps = box_enum(g["B"], data.H_cob, 1, True)
     Ks, Ms = coniZpM(data, ps, ellipsoid_dilation=g["D"], device="cpu")
-    kap = data.kappa_cob.tolist()
-    for K, M in list(zip(Ks, Ms))[:40]:
-        # p from K[1:] = N p (N invertible): solve exactly
-        from fractions import Fraction
-        h = data.h11
-        N = [[sum(kap[i][j][k] * int(M[k]) for k in range(h)) for j in range(1, h)] for i in range(1, h)]
-        assert not exact.is_singular(N)
+    assert len(Ks) > 0
+    for K, M in zip(Ks, Ms):
+        c = exact.pfv_checks(data.kappa_cob.tolist(), data.c2_cob.tolist(), data.H_cob.tolist(),
+                             data.h21, K, M)
+        assert all(c.values()), c
 
 
 def test_api_errors():
@@ -87,18 +87,49 @@ GPU_GEOMS = [g for g in GEOMS if 3 <= g["h11"] <= 11][::3]
 
 
 @pytest.mark.gpu
+def test_gpu_reproduces_dataset(need_gpu):
+    """The reference's dataset regression on the device, all 71
+    geometries (tests/test_dataset.py:62-86): every primitive p in the
+    recorded box at the recorded dilation gives exactly the dataset's PFVs
+    on primitive K[1:], and every extra has gcd(K[1:]) >= 2 and passes the
+    PFV checks."""
+    n = 0
+    for g in GEOMS:
+        data = cydata_of(g)
+        ps = box_enum(g["B"], data.H_cob, 1, True)
+        if len(ps) == 0:
+            assert not g["K"]
+            continue
+        Kg, Mg = coniZpM(data, ps, ellipsoid_dilation=g["D"], device="gpu")
+        _check_against_dataset(g, Kg, Mg, data)
+        for K, M in as_set(Kg, Mg) - as_set(g["K"], g["M"]):
+            c = exact.pfv_checks(data.kappa_cob.tolist(), data.c2_cob.tolist(),
+                                 data.H_cob.tolist(), data.h21, K, M)
+            assert all(c.values())
+        n += 1
+    print(f"{n} geometries with p-vectors", flush=True)
+    assert n >= 60
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("dil", [1, 150])
 @pytest.mark.parametrize("g", GPU_GEOMS, ids=lambda g: f"h11={g['h11']}-poly{g['polyID']}-coni{g['coniID']}")
-def test_gpu_matches_cpu(need_gpu, g, dil):
+def test_gpu_matches_oracle_dataset(need_gpu, g, dil):
+    """device='gpu' against the oracle alone (its points and exact
+    post-processing), arrays in the reference's order, at the dilations of
+    the reference's tests/test_gpu.py:51-63; a geometry without p-vectors
+    must have no dataset PFVs."""
     data = cydata_of(g)
     ps = box_enum(g["B"], data.H_cob, 1, True)
     if len(ps) == 0:
-        pytest.skip("no p-vectors")
+        assert not g["K"]
+        return
     print(f"h11={g['h11']} {len(ps)} p-vectors dil {dil}", flush=True)
-    Kc, Mc = coniZpM(data, ps, ellipsoid_dilation=dil, device="cpu")
-    Kg, Mg = coniZpM(data, ps, ellipsoid_dilation=dil, device="gpu")
-    np.testing.assert_array_equal(Kg, Kc)
-    np.testing.assert_array_equal(Mg, Mc)
+    Q = data.h11 + data.h21 + 4
+    Kg, Mg = coniZpM(data, ps, Q=Q, ellipsoid_dilation=dil, device="gpu")
+    Ko, Mo = oracle_pfvs(data, ps, Q, dil)
+    np.testing.assert_array_equal(Kg, Ko)
+    np.testing.assert_array_equal(Mg, Mo)
 
 
 @pytest.mark.gpu
@@ -231,3 +262,49 @@ def test_library_assembly_matches_numpy_gpu(need_gpu, dil):
     print(f"G(10,150,3) 64 p-vectors dil {dil}", flush=True)
     frac, n = _both_assemblies(inst.data, ps, inst.Q, dil, "batched")
     assert n > 0 and frac > 0.5
+
+
+def _points_of(data, ps, Q, dil):
+    """The host pipeline's points of ps (all of them, with the device-form
+    verdicts left undecided) and the p-vector index of each."""
+    from pfvs_amd import gpu
+    p_full = np.hstack([np.zeros((len(ps), 1), np.int64), np.asarray(ps, np.int64)])
+    geo = dict(kappa=data.kappa_cob, Mbasis=data.M_lattice(), Q=Q, dilation=dil, M0min=13)
+    r = gpu.run([geo], p_full, np.zeros(len(ps), np.int32), mode="host", filter=False, sort=True)
+    assert np.all(r["pstat"] == 0)
+    h = data.h11
+    return r["M"][:, :h], r["Kn"][:, :h], r["q"], r["pidx"]
+
+
+@pytest.mark.parametrize("case", ["dataset", "G10"])
+def test_post_processing_native_python_and_oracle_agree(monkeypatch, case):
+    """The post-processing of given points three ways: the library's
+    assembly (pfa_assemble), the Python-int form it leaves keys to (forced
+    for every key here), and the oracle's per-point restatement -- the same
+    rows in the same order."""
+    from pfvs_amd import coni, gpu
+    if case == "dataset":
+        g = max(SMALL, key=lambda g: len(g["K"]))
+        data, dil = cydata_of(g), g["D"]
+        ps = box_enum(g["B"], data.H_cob, 1, True)
+        Q = data.h11 + data.h21 + 4
+    else:
+        inst = make_G(10, 150, 3)
+        data, dil, Q = inst.data, 150, inst.Q
+        ps = random_pvecs(inst, 24, 6, seed=77)[:6]
+    M, K, q, pidx = _points_of(data, ps, Q, dil)
+    h = data.h11
+    Kn, Mn, kn = coni._pfvs_from_points(M.T, K.T, q, pidx, data.kappa_cob, h, Q, 13)
+    real = gpu.assemble_points
+    monkeypatch.setattr(gpu, "assemble_points",
+                        lambda *a, **k: (lambda r: (r[0][:0], r[1][:0], r[2][:0],
+                                                    np.zeros_like(r[3])))(real(*a, **k)))
+    Kp, Mp, kp = coni._pfvs_from_points(M.T, K.T, q, pidx, data.kappa_cob, h, Q, 13)
+    rows = exact.pfvs_from_points(zip(pidx, M, K, q), data.kappa_cob.tolist(), h, Q, 13)
+    assert len(Kn) > 0
+    np.testing.assert_array_equal(Kn, Kp.astype(np.int64))
+    np.testing.assert_array_equal(Mn, Mp.astype(np.int64))
+    np.testing.assert_array_equal(kn, kp)
+    np.testing.assert_array_equal(Kn, np.array([r[0] for r in rows], dtype=np.int64).reshape(-1, h))
+    np.testing.assert_array_equal(Mn, np.array([r[1] for r in rows], dtype=np.int64).reshape(-1, h))
+    np.testing.assert_array_equal(kn, np.array([r[2] for r in rows], dtype=np.int64))
