@@ -2646,7 +2646,8 @@ static void search_device(DevCtx *C, hipStream_t s, int64_t *d_recs, const RecL 
             }
             HIPCHK(hipEventRecord(ev[4], s));
         }
-        unsigned long long hctl[16 * 8];
+        // control words up to the overflow flag (word 16 * 8)
+        unsigned long long hctl[16 * 9];
         HIPCHK(hipMemcpyAsync(hctl, d_ctl, sizeof hctl, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         if (!lane) {
