@@ -219,6 +219,25 @@ void pfa_kernel_output_free(pfa_kernel_output *out);
 int pfa_det_filter(int n_geo, const int32_t *h, const int64_t *kappa, int64_t count,
                    const int32_t *geo, const int64_t *M, int device, int8_t *out, char *err);
 
+/* pfa_det_filter on device-resident arrays, allocating nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  kappa (sum of h[g]^3 int64), geo
+ * (count int32, each in 0..n_geo-1: not checked, the array is on the
+ * device), M (count rows of stride hstride >= max h, int64) and out (count
+ * int8: 1 nonsingular, 0 singular, 2 undecided) are the caller's device
+ * memory; h (n_geo int32) is host memory.  Work is queued on `stream` (a
+ * hipStream_t of the HIP runtime the library is linked against, or null);
+ * the call returns without waiting.  Every other buffer lives in one
+ * fixed device array of about 27 MB per device, held until the process
+ * ends; calls that overlap on one device share it and race (serialize
+ * them, e.g. on one stream).  Limits: n_geo <= 16, 2 <= h <= 41,
+ * count <= 65536.  Returns 0, or 1 with a message in err (256 bytes) for
+ * sizes beyond the limits. */
+int pfa_det_filter_noalloc(int device, int n_geo, const int32_t *h, const int64_t *kappa,
+                           int64_t count, const int32_t *geo, const int64_t *M, int hstride,
+                           int8_t *out, void *stream, char *err);
+
 /* Copies `bytes` (rounded down to 16) between two fresh device buffers
  * with a plain copy kernel, `reps` timed times after one untimed;
  * *ms_per_copy gets the mean.  The reference point for HBM traffic

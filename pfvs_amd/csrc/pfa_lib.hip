@@ -3559,6 +3559,217 @@ extern "C" int pfa_det_filter(int n_geo, const int32_t *h, const int64_t *kappa,
     }
 }
 
+// ------------------------------------------- allocation-free det filter
+// pfa_det_filter_noalloc: the same kernels (k_filter, k_det_mod) on
+// caller-owned device arrays, with every other buffer carved from one
+// file-scope device array (`scratch`).  Nothing is allocated per call: the
+// per-geometry tables (record headers, transposed kappa, its magnitude)
+// are built by k_na_setup from the caller's kappa, and the geometry sizes
+// and the prime table travel as a kernel argument.  The array's device
+// address is looked up once per device (hipGetSymbolAddress) and only
+// that address is handed to kernels.  Calls that overlap on one device
+// share the array and race.
+#define PFA_NA_MAX_GEO 16
+#define PFA_NA_MAX_POINTS 65536
+#define PFA_NA_NPRIMES 96
+#define PFA_NA_RECW 32                         // record words: the header only
+namespace na {
+constexpr size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t RECS = 0;
+constexpr size_t KOFS = RECS + up256((size_t)PFA_NA_MAX_GEO * PFA_NA_RECW * 8);
+constexpr size_t TOFS = KOFS + up256(PFA_NA_MAX_GEO * 8);
+constexpr size_t TMAX = TOFS + up256(PFA_NA_MAX_GEO * 8);
+constexpr size_t BAD = TMAX + up256(PFA_NA_MAX_GEO * 8);
+constexpr size_t MODCNT = BAD + up256(PFA_NA_MAX_GEO);
+constexpr size_t PRIMES = MODCNT + up256(8);
+constexpr size_t KT = PRIMES + up256(PFA_NA_NPRIMES * sizeof(PfaModP));
+constexpr size_t KT_PER_GEO = (size_t)PFA_MAX_H11 * PFA_DMAX * PFA_DMAX;     // int32
+constexpr size_t MODKP = KT + up256(PFA_NA_MAX_GEO * KT_PER_GEO * 4);
+constexpr size_t MODNEXP = MODKP + up256((size_t)PFA_NA_MAX_POINTS * 8);
+constexpr size_t MODM = MODNEXP + up256((size_t)PFA_NA_MAX_POINTS * 8);
+constexpr size_t BYTES = MODM + up256((size_t)PFA_NA_MAX_POINTS * PFA_MAX_H11 * 8);
+}  // namespace na
+
+// about 27 MB per device, held until the process ends
+static __device__ __align__(256) char scratch[na::BYTES];
+
+struct NaTables {
+    int n_geo;
+    int32_t h[PFA_NA_MAX_GEO];
+    int64_t kofs[PFA_NA_MAX_GEO];             // into the caller's kappa
+    int64_t tofs[PFA_NA_MAX_GEO];             // into the scratch kT
+    PfaModP primes[PFA_NA_NPRIMES];
+};
+
+// Block g < n_geo: geometry g's record header (h, geometry), offsets, and
+// kT[(k n + i) n + j] = kappa[i+1][j+1][k] with max |kappa| (infinity from
+// 2^31 on: the int32 copy is then not exact).  Block n_geo: the prime
+// table and the counters that start at zero.
+__global__ __launch_bounds__(256) void k_na_setup(NaTables T, const int64_t *kappa, char *scr)
+{
+    __shared__ double s_mx[256];
+    const int g = blockIdx.x;
+    if (g == T.n_geo) {
+        PfaModP *pr = (PfaModP *)(scr + na::PRIMES);
+        for (int i = threadIdx.x; i < PFA_NA_NPRIMES; i += blockDim.x) pr[i] = T.primes[i];
+        if (threadIdx.x == 0) *(unsigned long long *)(scr + na::MODCNT) = 0;
+        for (int i = threadIdx.x; i < PFA_NA_MAX_GEO; i += blockDim.x) ((int8_t *)(scr + na::BAD))[i] = 0;
+        return;
+    }
+    const int h = T.h[g], n = h - 1;
+    int64_t *rec = (int64_t *)(scr + na::RECS) + (size_t)g * PFA_NA_RECW;
+    for (int w = threadIdx.x; w < PFA_NA_RECW; w += blockDim.x) rec[w] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ((int32_t *)rec)[24] = h;
+        ((int32_t *)rec)[26] = g;
+        ((int64_t *)(scr + na::KOFS))[g] = T.kofs[g];
+        ((int64_t *)(scr + na::TOFS))[g] = T.tofs[g];
+    }
+    const int64_t *kap = kappa + T.kofs[g];
+    int32_t *kt = (int32_t *)(scr + na::KT) + T.tofs[g];
+    double mx = 0.0;
+    for (int e = threadIdx.x; e < h * n * n; e += blockDim.x) {
+        const int k = e / (n * n), r = e - k * n * n, i = r / n, j = r - i * n;
+        const int64_t v = kap[((int64_t)(i + 1) * h + (j + 1)) * h + k];
+        kt[e] = (int32_t)v;
+        mx = fmax(mx, fabs((double)v));
+    }
+    s_mx[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s_mx[threadIdx.x] = fmax(s_mx[threadIdx.x], s_mx[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        ((double *)(scr + na::TMAX))[g] = s_mx[0] < 2147483648.0 ? s_mx[0] : INFINITY;
+}
+
+static std::mutex g_na_mu;
+static std::vector<char *> g_na_ptr;         // per device: the scratch array's address
+
+// the device address of `scratch` on `device` (looked up once; call with
+// the device current)
+static char *na_scratch(int device, char *errbuf)
+{
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if ((int)g_na_ptr.size() <= device) g_na_ptr.resize(device + 1, nullptr);
+    if (!g_na_ptr[device]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_na_ptr[device] = (char *)p;
+    }
+    return g_na_ptr[device];
+}
+
+extern "C" int pfa_det_filter_noalloc(int device, int n_geo, const int32_t *h,
+                                      const int64_t *kappa, int64_t count, const int32_t *geo,
+                                      const int64_t *M, int hstride, int8_t *out, void *stream,
+                                      char *err)
+{
+    char errbuf[256] = {0};
+    try {
+        if (n_geo < 1 || n_geo > PFA_NA_MAX_GEO) {
+            snprintf(err, 256, "pfa_det_filter_noalloc: n_geo = %d outside 1..%d", n_geo,
+                     PFA_NA_MAX_GEO);
+            return 1;
+        }
+        if (count < 0 || count > PFA_NA_MAX_POINTS) {
+            snprintf(err, 256, "pfa_det_filter_noalloc: count = %lld outside 0..%d",
+                     (long long)count, PFA_NA_MAX_POINTS);
+            return 1;
+        }
+        NaTables T;
+        memset(&T, 0, sizeof T);
+        T.n_geo = n_geo;
+        int hmax = 2;
+        int64_t ko = 0, to = 0;
+        for (int g = 0; g < n_geo; ++g) {
+            if (h[g] < 2 || h[g] > PFA_MAX_H11) {
+                snprintf(err, 256, "pfa_det_filter_noalloc: h11 = %d outside 2..%d", h[g], PFA_MAX_H11);
+                return 1;
+            }
+            T.h[g] = h[g];
+            T.kofs[g] = ko;
+            T.tofs[g] = to;
+            ko += (int64_t)h[g] * h[g] * h[g];
+            to += (int64_t)h[g] * (h[g] - 1) * (h[g] - 1);
+            hmax = std::max(hmax, (int)h[g]);
+        }
+        if (hstride < hmax || hstride > PFA_MAX_H11) {
+            snprintf(err, 256, "pfa_det_filter_noalloc: row stride %d outside %d..%d", hstride, hmax,
+                     PFA_MAX_H11);
+            return 1;
+        }
+        static const std::vector<PfaModP> pr = primes31(PFA_NA_NPRIMES);
+        for (int i = 0; i < PFA_NA_NPRIMES; ++i) T.primes[i] = pr[i];
+        HIPCHK(hipSetDevice(device));
+        char *scr = na_scratch(device, errbuf);
+        hipStream_t s = (hipStream_t)stream;
+        hipLaunchKernelGGL(k_na_setup, dim3(n_geo + 1), dim3(256), 0, s, T, kappa, scr);
+        HIPCHK(hipGetLastError());
+        if (count == 0) return 0;
+        const int MH = hstride, D = std::max(hmax - 1, 1), nl = MH - 1;
+        RecL L{};
+        L.D = D;
+        L.words = PFA_NA_RECW;
+        FiltArgs a{};
+        a.n = count;
+        a.pidx = geo;
+        a.Min = M;
+        a.D = D;
+        a.MH = MH;
+        a.hl = MH;
+        a.recs = (const int64_t *)(scr + na::RECS);
+        a.L = L;
+        a.filter = 1;
+        a.nonsing = out;
+        a.bad = (int8_t *)(scr + na::BAD);
+        a.G.kappa = kappa;
+        a.G.kofs = (const int64_t *)(scr + na::KOFS);
+        a.kT = (const int32_t *)(scr + na::KT);
+        a.kTofs = (const int64_t *)(scr + na::TOFS);
+        a.kTmax = (const double *)(scr + na::TMAX);
+        a.n_geo = n_geo;
+        a.primes = (const PfaModP *)(scr + na::PRIMES);
+        a.np = PFA_NA_NPRIMES;
+        a.modcap = count;
+        a.modkp = (int64_t *)(scr + na::MODKP);
+        a.modnexp = (int64_t *)(scr + na::MODNEXP);
+        a.modM = (int64_t *)(scr + na::MODM);
+        a.modcount = (unsigned long long *)(scr + na::MODCNT);
+        const int nmax = nl <= 8 ? 8 : nl <= 12 ? 12 : nl <= 16 ? 16 : nl <= 20 ? 20 : nl <= 24 ? 24 :
+                         nl <= 32 ? 32 : 40;
+        int lds_floats = (nmax + 1) * nmax * nmax;
+        if (lds_floats > 64 * 1024 / 4) lds_floats = 0;     // kappa read from the global table
+        a.lds_floats = lds_floats;
+        const int R = 64 / a.hl;
+        const int grid = (int)std::min<int64_t>((count + 4 * R - 1) / (4 * R), 16384);
+        const size_t lds = (size_t)((lds_floats + 3) & ~3) * 4 + (size_t)4 * R * nmax * 20;
+#define PFA_NA_LAUNCH(NM)                                                                      \
+    do {                                                                                        \
+        HIPCHK(hipFuncSetAttribute((const void *)k_filter<NM, false>,                          \
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));      \
+        hipLaunchKernelGGL((k_filter<NM, false>), dim3(grid), dim3(256), lds, s, a);           \
+        HIPCHK(hipGetLastError());                                                              \
+        hipLaunchKernelGGL(k_det_mod<NM>, dim3(512), dim3(256), 0, s, a);                       \
+        HIPCHK(hipGetLastError());                                                              \
+    } while (0)
+        if (nmax == 8) PFA_NA_LAUNCH(8);
+        else if (nmax == 12) PFA_NA_LAUNCH(12);
+        else if (nmax == 16) PFA_NA_LAUNCH(16);
+        else if (nmax == 20) PFA_NA_LAUNCH(20);
+        else if (nmax == 24) PFA_NA_LAUNCH(24);
+        else if (nmax == 32) PFA_NA_LAUNCH(32);
+        else PFA_NA_LAUNCH(40);
+#undef PFA_NA_LAUNCH
+        return 0;
+    } catch (...) {
+        snprintf(err, 256, "%s", errbuf);
+        return 1;
+    }
+}
+
 extern "C" int pfa_kernel_batch(const pfa_kernel_input *in, pfa_kernel_output *out)
 {
     memset(out, 0, sizeof(*out));

@@ -98,6 +98,7 @@ SIGNATURES = {
     "pfa_kernel_batch": ([ctypes.POINTER(_KerIn), ctypes.POINTER(_KerOut)], _I),
     "pfa_kernel_output_free": ([ctypes.POINTER(_KerOut)], None),
     "pfa_det_filter": ([_I, _P32, _P64, _c_i64, _P32, _P64, _I, _VP, _CP], _I),
+    "pfa_det_filter_noalloc": ([_I, _I, _P32, _VP, _c_i64, _VP, _VP, _I, _VP, _VP, _CP], _I),
     "pfa_copy_probe": ([_I, _c_i64, _I, ctypes.POINTER(_c_f64)], _I),
     "pfa_box_enum": ([_P32, _I, _I, _P64, _I, _I, _I, _c_i64, _P64, _P64, _P64,
                       ctypes.POINTER(_c_f64), _CP], _I),
@@ -477,6 +478,18 @@ def det_filter(kappas, M, geo=None, device=0):
                           _ptr(rows), int(device), verdict.ctypes.data_as(ctypes.c_void_p), err):
         raise RuntimeError("det filter failed: " + err.value.decode(errors="replace"))
     return verdict[:len(rows)]
+
+
+def det_filter_noalloc(device, h, kappa_dev, count, geo_dev, M_dev, hstride, out_dev, stream=0):
+    """pfa_det_filter_noalloc on device arrays (addresses as ints, from the
+    same HIP runtime the library uses): kappa (sum h^3 int64), geo (count
+    int32), M (count x hstride int64), out (count int8).  Allocates
+    nothing; returns when the work is queued on `stream`."""
+    hs = np.ascontiguousarray(h, dtype=np.int32)
+    err = ctypes.create_string_buffer(256)
+    if require().pfa_det_filter_noalloc(int(device), len(hs), _ptr(hs, _P32), kappa_dev, int(count),
+                                        geo_dev, M_dev, int(hstride), out_dev, stream, err) != 0:
+        raise RuntimeError("det filter (no allocation) failed: " + err.value.decode(errors="replace"))
 
 
 # -------------------------------------------------- single-problem searches
