@@ -106,50 +106,14 @@ PFA_HD static inline int pfa_bounds(double R, double off, double uinv, int32_t *
     return 0;
 }
 
-// Enter level i (nd.i = i): nd.rem, err, m0p, g (of level i+1) and qp are
-// set by the caller; computes the center, the prefixes and [cur, hi].
-// Returns 0 or PFA_I32.
-PFA_HD static inline int pfa_enter(const PfaProb &P, const int32_t *c, PfaNode &nd)
+// Enter level i, second half: with the center nd.off and its magnitude
+// nd.aoff set, the radius (with the GCD narrowing) and the integer interval
+// [cur, hi] of c[i], cut exactly by the M0 bound at its level.  Returns 0
+// or PFA_I32.
+PFA_HD static inline int pfa_enter_finish(const PfaProb &P, PfaNode &nd)
 {
-    const int d = P.d, i = nd.i;
-    const double *Ui = P.U + (size_t)i * d;
-    const int64_t *Mi = P.mat + (size_t)i * d;
-    const int64_t *Hi = P.H + (size_t)i * d;
-    const int hrow = (int)((P.hmask >> i) & 1);
-    double o = 0.0, a = 0.0;
-    // the exact prefixes in int64 when |mat|, |H| < 2^31 and every partial
-    // sum is bounded by max(|mat|, |H|) max|c_j| (d - 1 - i) < 2^62
-    const int64_t B31 = (int64_t)1 << 31, bm = P.matmax > P.hmax ? P.matmax : P.hmax;
-    if (bm < B31 && (double)bm * (double)nd.cmax * (double)(d - 1 - i) < 4.0e18) {
-        int64_t mr = 0, sp = 0;
-        for (int j = i + 1; j < d; ++j) {
-            const int32_t cj = c[j];
-            double t = Ui[j] * (double)cj;
-            o += t;
-            a += fabs(t);
-            mr += (int64_t)(int32_t)Mi[j] * cj;
-            sp += (int64_t)(int32_t)Hi[j] * cj;
-        }
-        nd.mrow = mr;
-        nd.pre = hrow ? sp : 0;
-    } else {
-        for (int j = i + 1; j < d; ++j) {
-            double t = Ui[j] * (double)c[j];
-            o += t;
-            a += fabs(t);
-        }
-        pfa_i128 mr = 0;
-        for (int j = i + 1; j < d; ++j) mr += (pfa_i128)Mi[j] * c[j];
-        nd.mrow = mr;
-        nd.pre = 0;
-        if (hrow) {
-            pfa_i128 s = 0;
-            for (int j = i + 1; j < d; ++j) s += (pfa_i128)Hi[j] * c[j];
-            nd.pre = s;
-        }
-    }
-    nd.off = o;
-    nd.aoff = a;
+    const int i = nd.i;
+    const double o = nd.off, a = nd.aoff;
     const double slack = P.slack, Kerr = P.Kerr;
     double R = sqrt(fmax(nd.rem + slack + nd.err, 0.0));
     R += Kerr * (R + 2.0 * a) + 1e-300;
@@ -186,6 +150,60 @@ PFA_HD static inline int pfa_enter(const PfaProb &P, const int32_t *c, PfaNode &
     nd.cur = lo;
     nd.hi = hi;
     return 0;
+}
+
+// 1 when the exact prefixes of level i fit the int64 form: |mat|, |H| <
+// 2^31 and every partial sum bounded by max(|mat|, |H|) max|c_j| (d - 1 - i)
+// < 2^62
+PFA_HD static inline int pfa_prefix_small(const PfaProb &P, const PfaNode &nd)
+{
+    const int64_t B31 = (int64_t)1 << 31, bm = P.matmax > P.hmax ? P.matmax : P.hmax;
+    return bm < B31 && (double)bm * (double)nd.cmax * (double)(P.d - 1 - nd.i) < 4.0e18;
+}
+
+// Enter level i (nd.i = i): nd.rem, err, m0p, g (of level i+1) and qp are
+// set by the caller; computes the center off = sum_{j > i} U_ij c_j (j
+// ascending), aoff, the exact prefixes mrow and pre, then the interval.
+// Returns 0 or PFA_I32.
+PFA_HD static inline int pfa_enter(const PfaProb &P, const int32_t *c, PfaNode &nd)
+{
+    const int d = P.d, i = nd.i;
+    const double *Ui = P.U + (size_t)i * d;
+    const int64_t *Mi = P.mat + (size_t)i * d;
+    const int64_t *Hi = P.H + (size_t)i * d;
+    const int hrow = (int)((P.hmask >> i) & 1);
+    double o = 0.0, a = 0.0;
+    if (pfa_prefix_small(P, nd)) {
+        int64_t mr = 0, sp = 0;
+        for (int j = i + 1; j < d; ++j) {
+            const int32_t cj = c[j];
+            double t = Ui[j] * (double)cj;
+            o += t;
+            a += fabs(t);
+            mr += (int64_t)(int32_t)Mi[j] * cj;
+            sp += (int64_t)(int32_t)Hi[j] * cj;
+        }
+        nd.mrow = mr;
+        nd.pre = hrow ? sp : 0;
+    } else {
+        for (int j = i + 1; j < d; ++j) {
+            double t = Ui[j] * (double)c[j];
+            o += t;
+            a += fabs(t);
+        }
+        pfa_i128 mr = 0;
+        for (int j = i + 1; j < d; ++j) mr += (pfa_i128)Mi[j] * c[j];
+        nd.mrow = mr;
+        nd.pre = 0;
+        if (hrow) {
+            pfa_i128 s = 0;
+            for (int j = i + 1; j < d; ++j) s += (pfa_i128)Hi[j] * c[j];
+            nd.pre = s;
+        }
+    }
+    nd.off = o;
+    nd.aoff = a;
+    return pfa_enter_finish(P, nd);
 }
 
 // Test candidate c[i] = v of node nd; on success fill the child's inherited
