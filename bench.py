@@ -368,6 +368,8 @@ def main():
                    "wait_frac": sb["wait_cycles"] / max(sb["wave_cycles"], 1),
                    "donated": sb["donated"], "fetches": sb["fetches"],
                    "coniPFVs_per_s": pfv_b / wall_b,
+                   "graph_segments_held": sb["graphs"], "graph_replays": sb["graph_hits"],
+                   "graph_captures": sb["graph_captures"],
                    "host_fallback_rate": sb["host_p"] / max(len(pf), 1)},
     }
     work, wnodes, wthreads = search_work(gpu, geoms, pf, pgeo)

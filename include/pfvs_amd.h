@@ -124,6 +124,9 @@ enum {
     PFA_ST_MS_UPLOAD,     /* device: host ms of the uploads (pageable inputs) */
     PFA_ST_WAVES,         /* device: wavefronts of the search launch */
     PFA_ST_SURVIVORS,     /* device, keep_points 2: points copied back */
+    PFA_ST_GRAPHS,        /* device: instantiated segment graphs held (0: graphs off) */
+    PFA_ST_GRAPH_HITS,    /* device: segment launches replayed from a held graph (cumulative) */
+    PFA_ST_GRAPH_CAPTURES,/* device: segments captured and instantiated (cumulative) */
     PFA_ST_N
 };
 

@@ -864,6 +864,21 @@ __global__ __launch_bounds__(64) void k_search(BatchArgs a)
     }
 }
 
+// n bytes at p set to v (the buffer clears of a captured segment: a graph
+// gets kernel nodes, not memset nodes); 16-byte stores where aligned
+__global__ void k_fill(uint8_t *p, size_t n, uint8_t v)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t head = (16 - ((uintptr_t)p & 15)) & 15, h = head < n ? head : n;
+    const size_t n16 = (n - h) / 16;
+    const uint32_t w = 0x01010101u * v;
+    uint4 *q = (uint4 *)(p + h);
+    for (size_t i = t0; i < n16; i += stride) q[i] = make_uint4(w, w, w, w);
+    for (size_t i = t0; i < h; i += stride) p[i] = v;
+    for (size_t i = h + 16 * n16 + t0; i < n; i += stride) p[i] = v;
+}
+
 // Status words of the records: op 0 saves them to buf, op 1 restores them
 // from buf (a search changes only the status word of a record), op 2
 // gathers (status, stage code) pairs for the copy back.
@@ -1862,7 +1877,13 @@ struct DevCtx {
     unsigned long long sv_hint = 0;        // largest survivor count seen (keep_points 2)
     hipEvent_t ev[8] = {};                 // stage timings (created once)
     int wclk_khz = 0;                      // wall-clock counter rate
+    // instantiated graphs of the pipeline's segments, by launch key (the
+    // oldest is dropped beyond PFA_GRAPHS_MAX)
+    std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
+    unsigned long long graph_hits = 0, graph_captures = 0;
 };
+
+#define PFA_GRAPHS_MAX 48
 
 static std::mutex g_ctx_mu;
 static std::vector<DevCtx *> g_ctx;
@@ -2183,6 +2204,89 @@ static void build_kT(const pfa_run_input *in, const std::vector<int64_t> &kofs,
     if (spKV.empty()) spKV.push_back(0);
 }
 
+// ---------------------------------------------------- captured segments
+// A segment is a fixed run of launches on the library's stream between two
+// host decisions (prep; the search's set-up and the search; the
+// post-processing).  run_seg first describes the body's launches (kernel,
+// grid, block, LDS bytes, argument bytes) into a key; with graphs on, a key
+// seen before replays its instantiated graph and a new key is captured
+// (hipStreamBeginCapture .. EndCapture), instantiated, kept and launched.
+// Buffer clears in a body are k_fill launches.  With graphs off
+// (PFVS_GRAPH=0) the body launches directly.
+struct Seg {
+    hipStream_t s;
+    char *errbuf;
+    int describe;                          // 1: append to key only
+    std::string key;
+    void put(const void *p, size_t n) { key.append((const char *)p, n); }
+    template <class P, class A> void put_arg(const A &x)
+    {
+        P y = (P)x;
+        put(&y, sizeof y);
+    }
+    template <class... P, class... A>
+    void k(void (*fn)(P...), dim3 g, dim3 b, size_t lds, A... args)
+    {
+        static_assert(sizeof...(P) == sizeof...(A), "argument count");
+        if (describe) {
+            put(&fn, sizeof fn);
+            put(&g, sizeof g);
+            put(&b, sizeof b);
+            put(&lds, sizeof lds);
+            (put_arg<P>(args), ...);
+            return;
+        }
+        hipLaunchKernelGGL(fn, g, b, lds, s, ((P)args)...);
+        HIPCHK(hipGetLastError());
+    }
+    void zero(void *p, size_t n)
+    {
+        if (!n) return;
+        const int grid = (int)std::min<size_t>((n / 16 + 255) / 256 + 1, 2048);
+        k(k_fill, dim3(grid), dim3(256), 0, (uint8_t *)p, n, (uint8_t)0);
+    }
+};
+
+static bool graphs_on()
+{
+    const char *e = getenv("PFVS_GRAPH");
+    return !(e && e[0] == '0');
+}
+
+template <class F>
+static void run_seg(DevCtx *C, hipStream_t s, F body, char *errbuf)
+{
+    Seg sg{s, errbuf, 0, std::string()};
+    if (!graphs_on()) {
+        body(sg);
+        return;
+    }
+    sg.describe = 1;
+    body(sg);
+    if (sg.key.empty()) return;
+    hipGraphExec_t ex = nullptr;
+    for (auto &g : C->graphs)
+        if (g.first == sg.key) { ex = g.second; break; }
+    if (ex) {
+        C->graph_hits++;
+    } else {
+        Seg cap{s, errbuf, 0, std::string()};
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        body(cap);
+        hipGraph_t gr = nullptr;
+        HIPCHK(hipStreamEndCapture(s, &gr));
+        HIPCHK(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(gr);
+        if ((int)C->graphs.size() >= PFA_GRAPHS_MAX) {
+            (void)hipGraphExecDestroy(C->graphs.front().second);
+            C->graphs.erase(C->graphs.begin());
+        }
+        C->graphs.push_back({sg.key, ex});
+        C->graph_captures++;
+    }
+    HIPCHK(hipGraphLaunch(ex, s));
+}
+
 static void *upload(DevCtx *C, int slot, const void *src, size_t bytes, hipStream_t s, char *errbuf)
 {
     void *d = dev_get(C, slot, std::max<size_t>(bytes, 16), errbuf);
@@ -2190,18 +2294,29 @@ static void *upload(DevCtx *C, int slot, const void *src, size_t bytes, hipStrea
     return d;
 }
 
-// Post-processing over n point slots: k_filter, then k_det_mod on the
-// points it listed.  Outputs go to the non-null pointers of `a` (filled in
-// by the caller: slots, inputs, outputs, bad, tot); this adds kappa in
-// float, the primes and the list.
-static void filter_run(DevCtx *C, hipStream_t s, const pfa_run_input *gin,
-                       const std::vector<int64_t> &kofs, FiltArgs a, char *errbuf)
+// Post-processing over n point slots: k_post (unless M rows are given),
+// k_filter, then k_det_mod on the points it listed.  Outputs go to the
+// non-null pointers of `a` (filled in by the caller: slots, inputs,
+// outputs, bad, tot).  filter_setup uploads kappa in the filter's forms
+// and the primes and sizes the lists and the launches; filter_body
+// launches (a segment body: no host work, no uploads).
+struct FiltLaunch {
+    FiltArgs a;
+    int nmax = 8, grid = 0, pgrid = 0;
+    size_t lds = 0, plds = 0;
+    bool any_sparse = false;
+    unsigned long long *d_mc = nullptr, *d_kc = nullptr;
+};
+
+static FiltLaunch filter_setup(DevCtx *C, hipStream_t s, const pfa_run_input *gin,
+                               const std::vector<int64_t> &kofs, FiltArgs a, char *errbuf)
 {
     static const std::vector<PfaModP> pr = primes31(96);
     std::vector<int32_t> kT, spR, spKV, spInts;
     std::vector<double> kTmax;
     std::vector<int64_t> kTofs, spRofs, spKVofs;
     build_kT(gin, kofs, kT, kTofs, kTmax, spR, spKV, spRofs, spKVofs, spInts);
+    FiltLaunch F;
     // LDS for kappa: the padded layout of the launch's NMAX if it fits, and
     // room for the largest sparse form up to 48 KB
     const int nl = a.hl - 1;
@@ -2211,8 +2326,7 @@ static void filter_run(DevCtx *C, hipStream_t s, const pfa_run_input *gin,
     if (nmax > 24)
         for (int g = 0; g < gin->n_geo; ++g)
             if (spRofs[g] >= 0 && spInts[g] <= 12 * 1024) lds_floats = std::max(lds_floats, (int)spInts[g]);
-    unsigned long long *d_mc = (unsigned long long *)dev_get(C, B_MODCNT, 8, errbuf);
-    HIPCHK(hipMemsetAsync(d_mc, 0, 8, s));
+    F.d_mc = (unsigned long long *)dev_get(C, B_MODCNT, 8, errbuf);
     a.kT = (const int32_t *)upload(C, B_KT, kT.data(), 4 * kT.size(), s, errbuf);
     a.kTofs = (const int64_t *)upload(C, B_KTOFS, kTofs.data(), 8 * kTofs.size(), s, errbuf);
     a.kTmax = (const double *)upload(C, B_KTMAX, kTmax.data(), 8 * kTmax.size(), s, errbuf);
@@ -2229,55 +2343,73 @@ static void filter_run(DevCtx *C, hipStream_t s, const pfa_run_input *gin,
     a.modkp = (int64_t *)dev_get(C, B_MODLIST, 8 * a.modcap, errbuf);
     a.modnexp = (int64_t *)dev_get(C, B_MODNEXP, 8 * a.modcap, errbuf);
     a.modM = (int64_t *)dev_get(C, B_MODM, 8 * a.modcap * a.MH, errbuf);
-    a.modcount = d_mc;
-    if (a.n <= 0) return;
-    if (!a.Min) {
-        unsigned long long *d_kc = (unsigned long long *)dev_get(C, B_KEEPCNT, 8, errbuf);
-        HIPCHK(hipMemsetAsync(d_kc, 0, 8, s));
+    a.modcount = F.d_mc;
+    if (a.n > 0 && !a.Min) {
+        F.d_kc = (unsigned long long *)dev_get(C, B_KEEPCNT, 8, errbuf);
         a.keepkp = (int64_t *)dev_get(C, B_KEEPKP, 8 * a.n, errbuf);
         a.keepnexp = (int64_t *)dev_get(C, B_KEEPNEXP, 8 * a.n, errbuf);
         a.keepmeta = (int64_t *)dev_get(C, B_KEEPMETA, 8 * a.n, errbuf);
-        a.keepcount = d_kc;
-        const int pgrid = (int)std::min<int64_t>((a.n + 63) / 64, 65536);
-        const size_t plds = (a.M ? (size_t)64 * a.MH * 8 : 0) + (size_t)64 * a.D * 4;
-        auto kpost = a.D <= 8 ? k_post<8> : a.D <= 12 ? k_post<12> : a.D <= 16 ? k_post<16> :
-                     a.D <= 20 ? k_post<20> : a.D <= 24 ? k_post<24> : a.D <= 32 ? k_post<32> :
-                     k_post<40>;
-        hipLaunchKernelGGL(kpost, dim3(pgrid), dim3(64), plds, s, a);
-        HIPCHK(hipGetLastError());
-        if (!a.filter) return;
+        a.keepcount = F.d_kc;
+        F.pgrid = (int)std::min<int64_t>((a.n + 63) / 64, 65536);
+        F.plds = (a.M ? (size_t)64 * a.MH * 8 : 0) + (size_t)64 * a.D * 4;
     }
     const int R = 64 / a.hl;
     // list length unknown on the host: size the grid by the slots
     const int64_t rounds = (a.n + 4 * R - 1) / (4 * R);
-    const int grid = (int)std::min<int64_t>(rounds, 16384);
+    F.grid = (int)std::min<int64_t>(rounds, 16384);
     // kappa, then 4 wavefronts x R groups x NMAX steps of (double2, int32)
-    const size_t lds = (size_t)((a.lds_floats + 3) & ~3) * 4 + (size_t)4 * R * nmax * 20;
+    F.lds = (size_t)((a.lds_floats + 3) & ~3) * 4 + (size_t)4 * R * nmax * 20;
     // the sparse form where the padded dense layout cannot be staged (NMAX
     // > 24); below that the unrolled dense build is faster even at 20 %
     // density
-    bool any_sparse = false;
     for (int g = 0; g < gin->n_geo; ++g)
-        any_sparse |= nmax > 24 && spRofs[g] >= 0 && spInts[g] <= a.lds_floats;
+        F.any_sparse |= nmax > 24 && spRofs[g] >= 0 && spInts[g] <= a.lds_floats;
+    F.nmax = nmax;
+    F.a = a;
+#define PFA_FILTER_ATTR(NM)                                                                    \
+    HIPCHK(hipFuncSetAttribute((const void *)(F.any_sparse ? k_filter<NM, true> : k_filter<NM, false>), \
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)F.lds))
+    switch (nmax) {
+    case 8: PFA_FILTER_ATTR(8); break;
+    case 12: PFA_FILTER_ATTR(12); break;
+    case 16: PFA_FILTER_ATTR(16); break;
+    case 20: PFA_FILTER_ATTR(20); break;
+    case 24: PFA_FILTER_ATTR(24); break;
+    case 32: PFA_FILTER_ATTR(32); break;
+    default: PFA_FILTER_ATTR(40); break;
+    }
+#undef PFA_FILTER_ATTR
+    return F;
+}
+
+static void filter_body(Seg &sg, const FiltLaunch &F)
+{
+    const FiltArgs &a = F.a;
+    sg.zero(F.d_mc, 8);
+    if (a.n <= 0) return;
+    if (!a.Min) {
+        sg.zero(F.d_kc, 8);
+        auto kpost = a.D <= 8 ? k_post<8> : a.D <= 12 ? k_post<12> : a.D <= 16 ? k_post<16> :
+                     a.D <= 20 ? k_post<20> : a.D <= 24 ? k_post<24> : a.D <= 32 ? k_post<32> :
+                     k_post<40>;
+        sg.k(kpost, dim3(F.pgrid), dim3(64), F.plds, a);
+        if (!a.filter) return;
+    }
 #define PFA_FILTER_LAUNCH(NM)                                                                  \
     do {                                                                                        \
-        auto kf = any_sparse ? k_filter<NM, true> : k_filter<NM, false>;                       \
-        HIPCHK(hipFuncSetAttribute((const void *)kf,                                           \
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));      \
-        hipLaunchKernelGGL(kf, dim3(grid), dim3(256), lds, s, a);                               \
-        HIPCHK(hipGetLastError());                                                              \
-        if (a.filter) {                                                                         \
-            hipLaunchKernelGGL(k_det_mod<NM>, dim3(512), dim3(256), 0, s, a);                   \
-            HIPCHK(hipGetLastError());                                                          \
-        }                                                                                       \
+        sg.k(F.any_sparse ? k_filter<NM, true> : k_filter<NM, false>, dim3(F.grid), dim3(256),  \
+             F.lds, a);                                                                         \
+        if (a.filter) sg.k(k_det_mod<NM>, dim3(512), dim3(256), 0, a);                         \
     } while (0)
-    if (nl <= 8) PFA_FILTER_LAUNCH(8);
-    else if (nl <= 12) PFA_FILTER_LAUNCH(12);
-    else if (nl <= 16) PFA_FILTER_LAUNCH(16);
-    else if (nl <= 20) PFA_FILTER_LAUNCH(20);
-    else if (nl <= 24) PFA_FILTER_LAUNCH(24);
-    else if (nl <= 32) PFA_FILTER_LAUNCH(32);
-    else PFA_FILTER_LAUNCH(40);
+    switch (F.nmax) {
+    case 8: PFA_FILTER_LAUNCH(8); break;
+    case 12: PFA_FILTER_LAUNCH(12); break;
+    case 16: PFA_FILTER_LAUNCH(16); break;
+    case 20: PFA_FILTER_LAUNCH(20); break;
+    case 24: PFA_FILTER_LAUNCH(24); break;
+    case 32: PFA_FILTER_LAUNCH(32); break;
+    default: PFA_FILTER_LAUNCH(40); break;
+    }
 #undef PFA_FILTER_LAUNCH
 }
 
@@ -2597,55 +2729,50 @@ static void search_device(DevCtx *C, hipStream_t s, int64_t *d_recs, const RecL 
         R.d_pidx = (int32_t *)dev_get(C, B_PT_PIDX, 4 * pcap, errbuf);
         R.d_q = (int64_t *)dev_get(C, B_PT_Q, 8 * pcap, errbuf);
         R.d_c = (int32_t *)dev_get(C, B_PT_C, 4 * pcap * D, errbuf);
-        HIPCHK(hipMemsetAsync(d_ctl, 0, CTL_BYTES, s));
-        HIPCHK(hipMemsetAsync(R.d_cnt, 0, 64, s));
-        if (n_p > 0) {
-            hipLaunchKernelGGL(k_status_io, dim3(sgrid), dim3(256), 0, s, n_p, d_recs, L.words, d_save,
-                               attempt == 0 ? 0 : 1);
-            HIPCHK(hipGetLastError());
-        }
         PointArgs Pt{R.d_pidx, R.d_q, R.d_c, pcap, d_ctl + 16 * 6, d_ovf};
         CountArgs Cn{R.d_cnt};
         R.passes = 1;
-        R.launches = 0;
+        R.launches = (n_p > 0 || !(plain || lane)) ? 1 : 0;
         unsigned long long err = 0;
-        if (lane) {
-            HIPCHK(hipEventRecord(ev[2], s));
-            HIPCHK(hipEventRecord(ev[3], s));
-            if (n_p > 0) {
-                hipLaunchKernelGGL(k_lane, dim3((n_p + 63) / 64), dim3(64), 0, s, n_p, d_recs, L, Pt, D, Cn);
-                HIPCHK(hipGetLastError());
-                R.launches = 1;
-            }
-            HIPCHK(hipEventRecord(ev[4], s));
-        } else {
-            char *ring = nullptr;
-            unsigned long long *seq = nullptr;
+        char *ring = nullptr, *stacks = nullptr;
+        unsigned long long *seq = nullptr;
+        if (!lane) {
             if (!plain) {
                 ring = (char *)dev_get(C, B_RING, (size_t)ring_cap * E.bytes, errbuf);
                 seq = (unsigned long long *)dev_get(C, B_SEQ, 8 * ring_cap, errbuf);
-                HIPCHK(hipMemsetAsync(seq, 0, 8 * ring_cap, s));
             }
-            char *stacks = (char *)dev_get(C, B_STACKS, (size_t)nwaves * stack_cap * E.bytes, errbuf);
-            QueueArgs Qa{ring, seq, ring_cap, d_ctl};
-            HIPCHK(hipEventRecord(ev[2], s));
-            if (!plain && n_p > 0) {
-                hipLaunchKernelGGL(k_roots, dim3((n_p + 255) / 256), dim3(256), 0, s, n_p, d_recs, L, Qa, E, Cn);
-                HIPCHK(hipGetLastError());
-            }
-            HIPCHK(hipEventRecord(ev[3], s));
-            BatchArgs A{d_recs, L, E, Qa, stacks, stack_cap, donate_max, donate_min, pt_chunk,
-                        Pt, Cn, deadline, n_p};
-            auto kern = plain ? k_search<true> : k_search<false>;
+            stacks = (char *)dev_get(C, B_STACKS, (size_t)nwaves * stack_cap * E.bytes, errbuf);
+        }
+        QueueArgs Qa{ring, seq, ring_cap, d_ctl};
+        BatchArgs A{d_recs, L, E, Qa, stacks, stack_cap, donate_max, donate_min, pt_chunk,
+                    Pt, Cn, deadline, n_p};
+        auto kern = plain ? k_search<true> : k_search<false>;
+        if (!lane)
             HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)bat_lds));
-            if (n_p > 0 || !plain) {
-                hipLaunchKernelGGL(kern, dim3(nwaves), dim3(64), bat_lds, s, A);
-                HIPCHK(hipGetLastError());
-                R.launches = 1;
+        // set-up: clears, the statuses saved (first attempt) or restored,
+        // the roots of the batched form; then the search
+        HIPCHK(hipEventRecord(ev[2], s));
+        run_seg(C, s, [&](Seg &sg) {
+            sg.zero(d_ctl, CTL_BYTES);
+            sg.zero(R.d_cnt, 64);
+            if (n_p > 0)
+                sg.k(k_status_io, dim3(sgrid), dim3(256), 0, n_p, d_recs, L.words, d_save,
+                     attempt == 0 ? 0 : 1);
+            if (!lane && !plain) {
+                sg.zero(seq, 8 * ring_cap);
+                if (n_p > 0) sg.k(k_roots, dim3((n_p + 255) / 256), dim3(256), 0, n_p, d_recs, L, Qa, E, Cn);
             }
-            HIPCHK(hipEventRecord(ev[4], s));
-        }
+        }, errbuf);
+        HIPCHK(hipEventRecord(ev[3], s));
+        run_seg(C, s, [&](Seg &sg) {
+            if (lane) {
+                if (n_p > 0) sg.k(k_lane, dim3((n_p + 63) / 64), dim3(64), 0, n_p, d_recs, L, Pt, D, Cn);
+            } else if (n_p > 0 || !plain) {
+                sg.k(kern, dim3(nwaves), dim3(64), bat_lds, A);
+            }
+        }, errbuf);
+        HIPCHK(hipEventRecord(ev[4], s));
         // control words up to the overflow flag (word 16 * 8)
         unsigned long long hctl[16 * 9];
         HIPCHK(hipMemcpyAsync(hctl, d_ctl, sizeof hctl, hipMemcpyDeviceToHost, s));
@@ -2750,16 +2877,15 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
         }
         HIPCHK(hipFuncSetAttribute((const void *)k_prep, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds));
-        int64_t off = 0;
-        for (const PrepClass &c : cls) {
-            const int grid = (int)std::min<int64_t>(c.n, 16384);
-            const size_t cold = pfa_ws_cold_bytes(c.h);
-            hipLaunchKernelGGL(k_prep, dim3(grid), dim3(64), pfa_ws_lds_bytes(c.h), s, G, d_ps, MH,
-                               d_pgeo, c.n, d_order ? d_order + off : nullptr, d_recs, L, d_cold,
-                               cold);
-            HIPCHK(hipGetLastError());
-            off += c.n;
-        }
+        run_seg(C, s, [&](Seg &sg) {
+            int64_t off = 0;
+            for (const PrepClass &c : cls) {
+                const int grid = (int)std::min<int64_t>(c.n, 16384);
+                sg.k(k_prep, dim3(grid), dim3(64), pfa_ws_lds_bytes(c.h), G, d_ps, MH, d_pgeo, c.n,
+                     d_order ? d_order + off : nullptr, d_recs, L, d_cold, pfa_ws_cold_bytes(c.h));
+                off += c.n;
+            }
+        }, errbuf);
     }
     HIPCHK(hipEventRecord(ev[1], s));
 
@@ -2791,10 +2917,6 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     unsigned long long *d_tot = (unsigned long long *)dev_get(C, B_TOT, 64, errbuf);
     unsigned long long *d_pcount =
         (unsigned long long *)dev_get(C, B_PCOUNT, 8 * (size_t)std::max<int64_t>(n_p, 1), errbuf);
-    HIPCHK(hipMemsetAsync(d_pcount, 0, 8 * (size_t)std::max<int64_t>(n_p, 1), s));
-    HIPCHK(hipMemsetAsync(d_bad, 0, std::max<int64_t>(n_p, 1), s));
-    HIPCHK(hipMemsetAsync(d_tot, 0, 64, s));
-    HIPCHK(hipEventRecord(ev[5], s));
     FiltArgs Fa{};
     Fa.n = n;
     Fa.pidx = R.d_pidx;
@@ -2815,7 +2937,19 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     Fa.tot = d_tot;
     Fa.pcount = d_pcount;
     Fa.G = G;
-    filter_run(C, s, in, kofs, Fa, errbuf);
+    const FiltLaunch F = filter_setup(C, s, in, kofs, Fa, errbuf);
+    // statuses and stage codes of the p-vectors, gathered for one copy
+    int32_t *d_stat = (int32_t *)dev_get(C, B_STATOUT, 8 * (size_t)std::max<int64_t>(n_p, 1), errbuf);
+    HIPCHK(hipEventRecord(ev[5], s));
+    run_seg(C, s, [&](Seg &sg) {
+        sg.zero(d_pcount, 8 * (size_t)std::max<int64_t>(n_p, 1));
+        sg.zero(d_bad, std::max<int64_t>(n_p, 1));
+        sg.zero(d_tot, 64);
+        filter_body(sg, F);
+        if (n_p > 0)
+            sg.k(k_status_io, dim3((int)std::min<int64_t>((n_p + 255) / 256, 4096)), dim3(256), 0, n_p,
+                 d_recs, L.words, d_stat, 2);
+    }, errbuf);
     // the survivors (keep 2): compacted; their count read back, with a
     // second pass if they outgrew the buffer
     int64_t nsv = 0;
@@ -2847,10 +2981,11 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
             Ca.o_ns = d_sv_ns = (int8_t *)dev_get(C, B_SV_NS, cap, errbuf);
             Ca.o_count = d_svc;
             Ca.cap = cap;
-            HIPCHK(hipMemsetAsync(d_svc, 0, 8, s));
             const int cgrid = (int)std::min<int64_t>((n + 63) / 64, 4096);
-            hipLaunchKernelGGL(k_compact, dim3(cgrid), dim3(64), 0, s, Ca);
-            HIPCHK(hipGetLastError());
+            run_seg(C, s, [&](Seg &sg) {
+                sg.zero(d_svc, 8);
+                sg.k(k_compact, dim3(cgrid), dim3(64), 0, Ca);
+            }, errbuf);
             unsigned long long cnt = 0;
             HIPCHK(hipMemcpyAsync(&cnt, d_svc, 8, hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
@@ -2859,13 +2994,6 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
             if (cnt <= cap) break;
             cap = cnt + cnt / 4 + 1024;
         }
-    }
-    // statuses and stage codes of the p-vectors, gathered for one copy
-    int32_t *d_stat = (int32_t *)dev_get(C, B_STATOUT, 8 * (size_t)std::max<int64_t>(n_p, 1), errbuf);
-    if (n_p > 0) {
-        hipLaunchKernelGGL(k_status_io, dim3((int)std::min<int64_t>((n_p + 255) / 256, 4096)), dim3(256), 0,
-                           s, n_p, d_recs, L.words, d_stat, 2);
-        HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(ev[6], s));
     // results back, into the pinned staging buffers
@@ -2941,6 +3069,9 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     out->stats[PFA_ST_DONATED] = (double)hc[6];
     out->stats[PFA_ST_FETCHES] = (double)hc[7];
     out->stats[PFA_ST_WAVES] = R.nwaves;
+    out->stats[PFA_ST_GRAPHS] = graphs_on() ? (double)C->graphs.size() : 0.0;
+    out->stats[PFA_ST_GRAPH_HITS] = (double)C->graph_hits;
+    out->stats[PFA_ST_GRAPH_CAPTURES] = (double)C->graph_captures;
     out->stats[PFA_ST_SURVIVORS] = (double)nsv;
     if (n_p >= 64) C->ppp_hint = std::max(C->ppp_hint, (double)R.npts / (double)n_p);
     lk.unlock();                              // the device is free; assemble from the stage
@@ -3258,6 +3389,10 @@ extern "C" void pfg_release(int device)
     }
     std::lock_guard<std::mutex> lk(C->mu);
     if (hipSetDevice(device) != hipSuccess) return;
+    if (C->stream) (void)hipStreamSynchronize(C->stream);
+    // graphs refer to the buffers: drop them first
+    for (auto &g : C->graphs) (void)hipGraphExecDestroy(g.second);
+    C->graphs.clear();
     for (auto &b : C->bufs) {
         if (b.p) (void)hipFree(b.p);
         b.p = nullptr;
@@ -3549,7 +3684,8 @@ extern "C" int pfa_det_filter(int n_geo, const int32_t *h, const int64_t *kappa,
         Fa.nonsing = d_ns;
         Fa.bad = d_bad;
         Fa.G = G;
-        filter_run(C, s, &gin, kofs, Fa, errbuf);
+        const FiltLaunch F = filter_setup(C, s, &gin, kofs, Fa, errbuf);
+        run_seg(C, s, [&](Seg &sg) { filter_body(sg, F); }, errbuf);
         if (count > 0) HIPCHK(hipMemcpyAsync(out, d_ns, count, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         return 0;
