@@ -297,6 +297,8 @@ def main():
                     help="fixed search launch setting instead of the autotuner's (with --budget)")
     ap.add_argument("--budget", type=int, default=0,
                     help="fixed donation size instead of the autotuner's (with --waves-per-cu)")
+    ap.add_argument("--split", type=int, default=0,
+                    help="fixed split target (nodes per wavefront; < 0 none) with --waves-per-cu")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only time a plain device copy of MIB MiB (the reference "
                          "kernel for HBM traffic counters) and exit")
@@ -321,11 +323,12 @@ def main():
           file=sys.stderr, flush=True)
     params = {}
     if args.waves_per_cu > 0 or args.budget > 0:
-        params = {"waves_per_cu": args.waves_per_cu, "budget": args.budget, "stack_cap": 0}
+        params = {"waves_per_cu": args.waves_per_cu, "budget": args.budget, "stack_cap": 0,
+                  "split": args.split}
     elif not args.no_tune:
         from pfvs_amd import autotune
         best = autotune.tune(geoms, pf, pgeo, use_cache=False)
-        params = {k: best[k] for k in ("waves_per_cu", "budget", "stack_cap")}
+        params = {k: best[k] for k in ("waves_per_cu", "budget", "stack_cap", "split")}
         print(f"# autotuned: {params} ({best['ms_search']:.2f} ms search)", file=sys.stderr,
               flush=True)
     wall_b, st_b, r_b, pfv_b, left_b, walls_b = time_mode(gpu, geoms, pf, pgeo, "batched",

@@ -97,7 +97,8 @@ typedef struct {
     int mode;
     int waves_per_cu;     /* batched: persistent wavefronts per CU (default 4) */
     int budget;           /* batched: most nodes one wavefront donates at once (default 8) */
-    int stack_cap;        /* batched: entries per wavefront deque (default 64 (d + 2)) */
+    int stack_cap;        /* batched: entries per wavefront deque (default max(64 (d + 2),
+                             2 split + 128)) */
     int host_threads;     /* host mode */
     int filter;
     int sort;             /* 1: group by p and sort by (M, Kn) */
@@ -105,6 +106,8 @@ typedef struct {
     int verbose;
     int assemble;         /* with filter and sort: also the finished PFVs (K, M) of every
                              p-vector the library can finish (see pfa_run_output.pdone) */
+    int split;            /* batched: nodes per wavefront the breadth-first split expands the
+                             roots to before the persistent search (default 2048; < 0 none) */
 } pfa_run_input;
 
 /* stats[]: see PFA_ST_* */

@@ -1,8 +1,9 @@
 """Autotuner for the batched search's launch parameters.
 
-The batched form (one persistent launch, per-wavefront deques, donation
-through a frontier ring) has three tunables: persistent wavefronts per CU,
-the most nodes one wavefront donates at once, and the deque capacity.
+The batched form (a breadth-first split of the roots, then one persistent
+launch with per-wavefront deques and donation through a frontier ring) has
+four tunables: persistent wavefronts per CU, the most nodes one wavefront
+donates at once, the deque capacity and the split's nodes per wavefront.
 Their best values depend on the tree shapes (h11, dilation, p-vector mix)
 and on the device, so `tune` times the search (its own HIP-event
 duration, `stats["ms_search"]`) on a seeded sample of the caller's batch
@@ -27,9 +28,10 @@ import numpy as np
 from . import gpu
 
 DEFAULT_GRID = {
-    "waves_per_cu": (2, 3, 4, 6, 8),
-    "budget": (4, 8, 16, 32),
+    "waves_per_cu": (2, 3, 4),
+    "budget": (4, 16),
     "stack_cap": (0,),
+    "split": (512, 2048, 8192),
 }
 
 
@@ -74,7 +76,7 @@ def sample_batch(ps, pgeo, n_sample, seed=0):
 def tune(geoms, ps, pgeo, device=0, n_sample=4096, grid=None, repeats=2, use_cache=True,
          runner=None, verbose=False):
     """Best launch parameters for this workload:
-    {"waves_per_cu", "budget", "stack_cap", "ms_search", "key", "table"}.
+    {"waves_per_cu", "budget", "stack_cap", "split", "ms_search", "key", "table"}.
 
     runner(geoms, ps, pgeo, **params) -> stats dict (default: gpu.run in
     batched mode without points); it must report "ms_search" and "nodes"."""
@@ -86,32 +88,34 @@ def tune(geoms, ps, pgeo, device=0, n_sample=4096, grid=None, repeats=2, use_cac
         if hit:
             return dict(hit, key=key, cached=True)
     sel = sample_batch(ps, pgeo, n_sample)
-    sp, sg = np.ascontiguousarray(ps[sel]), np.ascontiguousarray(pgeo[sel])
+    sp_ps, sg = np.ascontiguousarray(ps[sel]), np.ascontiguousarray(pgeo[sel])
     if runner is None:
         def runner(g, p, pg, **kw):
             return gpu.run(g, p, pg, mode="batched", device=device, keep_points=False,
                            filter=False, **kw)["stats"]
-    runner(geoms, sp, sg)                             # warm: buffers sized, code loaded
+    runner(geoms, sp_ps, sg)                          # warm: buffers sized, code loaded
     table, nodes0, best = [], None, None
     for w in grid["waves_per_cu"]:
         for b in grid["budget"]:
             for sc in grid["stack_cap"]:
-                params = {"waves_per_cu": int(w), "budget": int(b), "stack_cap": int(sc)}
-                ms = []
-                for _ in range(repeats):
-                    st = runner(geoms, sp, sg, **params)
-                    if nodes0 is None:
-                        nodes0 = st["nodes"]
-                    elif st["nodes"] != nodes0:
-                        raise RuntimeError(f"settings {params} visited {st['nodes']} nodes, "
-                                           f"not {nodes0}")
-                    ms.append(st["ms_search"])
-                row = dict(params, ms_search=min(ms))
-                table.append(row)
-                if verbose:
-                    print(f"  {params} -> {min(ms):.2f} ms", flush=True)
-                if best is None or row["ms_search"] < best["ms_search"]:
-                    best = row
+                for sp in grid["split"]:
+                    params = {"waves_per_cu": int(w), "budget": int(b), "stack_cap": int(sc),
+                              "split": int(sp)}
+                    ms = []
+                    for _ in range(repeats):
+                        st = runner(geoms, sp_ps, sg, **params)
+                        if nodes0 is None:
+                            nodes0 = st["nodes"]
+                        elif st["nodes"] != nodes0:
+                            raise RuntimeError(f"settings {params} visited {st['nodes']} nodes, "
+                                               f"not {nodes0}")
+                        ms.append(st["ms_search"])
+                    row = dict(params, ms_search=min(ms))
+                    table.append(row)
+                    if verbose:
+                        print(f"  {params} -> {min(ms):.2f} ms", flush=True)
+                    if best is None or row["ms_search"] < best["ms_search"]:
+                        best = row
     out = dict(best, table=table, signature=sig, tuned_at=time.time())
     if use_cache:
         cache = _load_cache(path)

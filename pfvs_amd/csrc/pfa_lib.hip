@@ -5,7 +5,10 @@
 //               8.9 KB at h11 = 20, the matrix products' in global memory)
 //               per p-vector, one launch per h11 size class: lattice, HNF,
 //               FP64 Cholesky with its error margin, record.
-//   k_roots     one lane per p-vector: top-level node -> frontier ring.
+//   k_roots     one lane per p-vector: top-level node -> frontier buffer.
+//   k_expand    batched form: breadth-first expansion of the frontier, one
+//               launch per level, until it holds enough nodes to start
+//               every persistent wavefront with a deep deque.
 //   search      PLAIN: k_search<true>, one launch per batch, one
 //               wavefront per p-vector (the same wave-cooperative step,
 //               no work sharing between p-vectors).  LANE: k_lane, one
@@ -367,6 +370,24 @@ struct QueueArgs {
     unsigned long long *ctl;
 };
 
+// Breadth-first split of the search before the persistent launch: the
+// roots go to frontier buffer 0, and expansion t turns the nodes of buffer
+// t % 2 (all children of expansion t - 1) into their children in the other
+// buffer, until a frontier holds `target` nodes or its children could
+// outgrow the deques (`room`) or a buffer (`cap`).  The persistent search
+// then starts every wavefront with a contiguous slice of the last frontier.
+// st[t] nodes and st[64 + t] their candidates in the frontier of expansion
+// t; st[128] stop flag, st[129] the expansion that stopped (the last
+// frontier), st[130] its node count, st[131] its buffer.
+#define PFA_SPLIT_MAX 48
+struct SplitArgs {
+    char *F0, *F1;
+    unsigned long long cap;       // entries per buffer
+    unsigned long long *st;
+    unsigned long long target;    // stop once a frontier holds this many nodes
+    unsigned long long room;      // most nodes the deques take at the start
+};
+
 struct PointArgs {
     int32_t *pidx;
     int64_t *q;
@@ -387,7 +408,9 @@ __device__ static inline void entry_copy(char *dst, const char *src, int bytes)
     for (int k = 0; k < bytes / 16; ++k) t[k] = s[k];
 }
 
-__global__ void k_roots(int64_t n, int64_t *recs, RecL L, QueueArgs Qa, EntL E, CountArgs C)
+// one lane per p-vector: its root node (entered, counted as a node)
+// appended to frontier buffer 0 with its candidate count
+__global__ void k_roots(int64_t n, int64_t *recs, RecL L, SplitArgs S, EntL E, CountArgs C)
 {
     int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
@@ -401,15 +424,14 @@ __global__ void k_roots(int64_t n, int64_t *recs, RecL L, QueueArgs Qa, EntL E, 
     for (int j = 0; j < PFA_DMAX; ++j) c[j] = 0;
     if (pfa_enter(P, c, nd)) { rec_raise_status(rec, PFA_I32); return; }
     atomicAdd(&C.c[0], 1ULL);
-    atomicAdd(&Qa.ctl[32], 1ULL);                        // one outstanding unit
-    unsigned long long t = atomicAdd(&Qa.ctl[16], 1ULL);
-    char *e = Qa.ring + (t % Qa.cap) * E.bytes;
+    if (nd.hi < nd.cur) return;                          // no candidate: a node without children
+    unsigned long long t = atomicAdd(&S.st[0], 1ULL);
+    atomicAdd(&S.st[64], (unsigned long long)((long long)nd.hi - nd.cur + 1));
+    char *e = S.F0 + t * E.bytes;
     *(PfaNode *)e = nd;
     ((int32_t *)(e + 128))[0] = (int32_t)ip;
     int32_t *ec = (int32_t *)(e + 144);
     for (int j = 0; j < E.D; ++j) ec[j] = 0;
-    __threadfence();                                    // entry before its ticket
-    atomicExch(&Qa.seq[t % Qa.cap], t + 1);
 }
 
 // ------------------------------------------------ one lane per problem
@@ -470,6 +492,7 @@ struct BatchArgs {
     CountArgs C;
     unsigned long long deadline;      // wall-clock ticks (0: none)
     int64_t n_p;                      // plain form: the p-vectors
+    SplitArgs S;                      // batched form: the split's last frontier
 };
 
 #define PFA_CSTRIDE (PFA_DMAX + 1)        // odd stride: conflict-free LDS rows
@@ -573,6 +596,23 @@ __global__ __launch_bounds__(64) void k_search(BatchArgs a)
     unsigned long long waiting = 0;
     unsigned long long pt_next = 0, pt_end = 0;        // this wavefront's point slots
     int64_t ip_next = blockIdx.x;                       // plain form: next p-vector
+    if (!PLAIN) {
+        // this wavefront's contiguous slice of the split's last frontier
+        // becomes its deque (in the global array; nothing cached yet)
+        const unsigned long long nf = a.S.st[130];
+        const char *Ff = a.S.st[131] ? a.S.F1 : a.S.F0;
+        const unsigned long long lo = nf * blockIdx.x / gridDim.x;
+        const unsigned long long hi = nf * (blockIdx.x + 1) / gridDim.x;
+        if (hi > lo) {
+            const int n = (int)(hi - lo);
+            const int4 *src = (const int4 *)(Ff + lo * ES);
+            int4 *dst = (int4 *)stk;
+            for (int k = lane; k < n * (ES / 16); k += 64) dst[k] = src[k];
+            top = n;
+            cb = n;
+            have_unit = 1;
+        }
+    }
     for (;;) {
         if (PLAIN && top == bot) {
             // the next p-vector of this wavefront whose root opens: its root
@@ -862,6 +902,191 @@ __global__ __launch_bounds__(64) void k_search(BatchArgs a)
         atomicAdd(&a.C.c[6], n_don);
         atomicAdd(&a.C.c[7], n_fetch);
     }
+}
+
+// Expansion t of the breadth-first split (SplitArgs): every node of the
+// frontier in buffer t % 2 has each of its candidates tested (pfa_cand);
+// survivors above level 0 are entered (pfa_enter) and, when they have
+// candidates, appended to the other buffer; level-0 survivors are leaves
+// (pfa_leaf) written to the point buffer.  A workgroup (one wavefront)
+// takes 64 nodes at a time, stages them in LDS and runs their candidates
+// 64 per round (lane -> candidate by a prefix sum over the nodes' ranges).
+// Every workgroup reads the same frontier sizes and so takes the same
+// decision to expand or to stop.  The same nodes, candidates and leaves are
+// counted as in the depth-first forms.
+__device__ static inline long long wave_scan_incl64(long long x, int lane)
+{
+    for (int o = 1; o < 64; o <<= 1) {
+        long long y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+__global__ __launch_bounds__(64) void k_expand(int t, int64_t *recs, RecL L, EntL E, SplitArgs S,
+                                               PointArgs Pt, CountArgs C, int pt_chunk)
+{
+    const int lane = threadIdx.x;
+    unsigned long long *st = S.st;
+    if (st[128]) return;
+    const unsigned long long n_in = st[t], cs = st[64 + t];
+    if (n_in == 0 || n_in >= S.target || cs > S.room || cs > S.cap || t + 1 >= 64) {
+        if (blockIdx.x == 0 && lane == 0) {
+            st[129] = t;
+            st[128] = 1;
+        }
+        return;
+    }
+    const int ES = E.bytes, D = E.D;
+    const char *Fin = (t & 1) ? S.F1 : S.F0;
+    char *Fout = (t & 1) ? S.F0 : S.F1;
+    extern __shared__ __align__(16) char s_nodes[];      // 64 entries
+    __shared__ long long s_incl[64];
+    __shared__ unsigned long long s_u64[2];
+    __shared__ int32_t s_c[64 * PFA_CSTRIDE];
+    int32_t *c = s_c + lane * PFA_CSTRIDE;
+    unsigned long long n_nodes = 0, n_cand = 0, n_leaf = 0;
+    unsigned long long pt_next = 0, pt_end = 0;
+    for (unsigned long long base = (unsigned long long)blockIdx.x * 64ULL; base < n_in;
+         base += (unsigned long long)gridDim.x * 64ULL) {
+        const int nb = n_in - base < 64 ? (int)(n_in - base) : 64;
+        __syncthreads();                                  // the previous block's reads done
+        {
+            const int4 *src = (const int4 *)(Fin + base * ES);
+            int4 *dst = (int4 *)s_nodes;
+            for (int k = lane; k < nb * (ES / 16); k += 64) dst[k] = src[k];
+        }
+        __syncthreads();
+        long long cnt = 0;
+        if (lane < nb) {
+            const PfaNode *nd = (const PfaNode *)(s_nodes + (size_t)lane * ES);
+            cnt = (long long)nd->hi - nd->cur + 1;
+            if (cnt < 0) cnt = 0;
+        }
+        s_incl[lane] = wave_scan_incl64(cnt, lane);
+        __syncthreads();
+        const long long total = s_incl[63];
+        for (long long r0 = 0; r0 < total; r0 += 64) {
+            const long long g = r0 + lane;
+            int j = -1;
+            if (g < total) {
+                int lo = 0, hi2 = nb - 1;
+                while (lo < hi2) {
+                    int mid = (lo + hi2) >> 1;
+                    if (s_incl[mid] > g) hi2 = mid; else lo = mid + 1;
+                }
+                j = lo;
+            }
+            PfaNode nd;
+            int32_t pidx = 0, v = 0;
+            if (j >= 0) {
+                const char *e = s_nodes + (size_t)j * ES;
+                nd = *(const PfaNode *)e;
+                pidx = ((const int32_t *)(e + 128))[0];
+                const int32_t *ec = (const int32_t *)(e + 144);
+                for (int k = 0; k < D; ++k) c[k] = k > nd.i ? ec[k] : 0;
+                v = nd.cur + (int32_t)(g - (j > 0 ? s_incl[j - 1] : 0));
+            }
+            int has_child = 0, emit = 0;
+            int64_t q = 0;
+            PfaNode ch;
+            const int32_t p0 = __builtin_amdgcn_readfirstlane(j >= 0 ? pidx : -1);
+            const bool uni = p0 >= 0 && !__ballot(j >= 0 && pidx != p0);
+            if (j >= 0) {
+                const int64_t *rec = recs + (int64_t)pidx * L.words;
+                PfaProb P = uni ? rec_view_uni(recs + (int64_t)p0 * L.words, L) : rec_view(rec, L);
+                n_cand++;
+                int r = pfa_cand(P, nd, v, ch);
+                if (r == PFA_CAND_OVF) {
+                    rec_raise_status((int64_t *)rec, PFA_HOST);
+                } else if (r == 1) {
+                    c[nd.i] = v;
+                    if (nd.i > 0) {
+                        n_nodes++;
+                        if (pfa_enter(P, c, ch)) rec_raise_status((int64_t *)rec, PFA_I32);
+                        else has_child = ch.hi >= ch.cur;
+                    } else {
+                        n_leaf++;
+                        int lr = pfa_leaf(P, ch, c, &q);
+                        if (lr == PFA_CAND_OVF) rec_raise_status((int64_t *)rec, PFA_HOST);
+                        else emit = lr == 1;
+                    }
+                }
+            }
+            const unsigned long long em = __ballot(emit);
+            if (em) {
+                const int ne = __popcll(em);
+                if (pt_next + ne > pt_end) {
+                    if (lane == 0) s_u64[0] = atomicAdd(Pt.count, (unsigned long long)pt_chunk);
+                    __syncthreads();
+                    for (unsigned long long k = pt_next + lane; k < pt_end; k += 64)
+                        if (k < Pt.cap) Pt.pidx[k] = -1;
+                    pt_next = s_u64[0];
+                    pt_end = pt_next + pt_chunk;
+                    __syncthreads();
+                }
+                if (emit) {
+                    unsigned long long k = pt_next + __popcll(em & ((1ULL << lane) - 1));
+                    if (k < Pt.cap) {
+                        Pt.pidx[k] = pidx;
+                        Pt.q[k] = q;
+                        for (int tt = 0; tt < D; ++tt) Pt.c[k * D + tt] = c[tt];
+                    } else {
+                        *Pt.overflow = 1;
+                    }
+                }
+                pt_next += ne;
+            }
+            const unsigned long long cm = __ballot(has_child);
+            if (cm) {
+                const int nch = __popcll(cm);
+                // the children's candidates, summed over the wavefront
+                long long cc = has_child ? (long long)ch.hi - ch.cur + 1 : 0;
+                for (int o = 32; o > 0; o >>= 1) cc += __shfl_xor(cc, o, 64);
+                if (lane == 0) {
+                    s_u64[0] = atomicAdd(&st[t + 1], (unsigned long long)nch);
+                    atomicAdd(&st[64 + t + 1], (unsigned long long)cc);
+                }
+                __syncthreads();
+                const unsigned long long ob = s_u64[0];
+                __syncthreads();
+                if (has_child) {
+                    // (cap >= the candidates of this frontier >= its children)
+                    char *e = Fout + (ob + __popcll(cm & ((1ULL << lane) - 1))) * (unsigned long long)ES;
+                    *(PfaNode *)e = ch;
+                    ((int32_t *)(e + 128))[0] = pidx;
+                    int32_t *ec = (int32_t *)(e + 144);
+                    for (int k = 0; k < D; ++k) ec[k] = k > ch.i ? c[k] : 0;
+                }
+            }
+        }
+    }
+    for (unsigned long long k = pt_next + lane; k < pt_end; k += 64)
+        if (k < Pt.cap) Pt.pidx[k] = -1;
+    for (int o = 32; o > 0; o >>= 1) {
+        n_nodes += __shfl_xor(n_nodes, o, 64);
+        n_cand += __shfl_xor(n_cand, o, 64);
+        n_leaf += __shfl_xor(n_leaf, o, 64);
+    }
+    if (lane == 0) {
+        if (n_nodes) atomicAdd(&C.c[0], n_nodes);
+        if (n_cand) atomicAdd(&C.c[1], n_cand);
+        if (n_leaf) atomicAdd(&C.c[2], n_leaf);
+    }
+}
+
+// After the split: the last frontier (node count, buffer) for the
+// persistent search, and one outstanding work unit per wavefront that gets
+// a nonempty slice of it (min(n, W) of the W wavefronts).
+__global__ void k_split_final(SplitArgs S, unsigned long long *ctl, int W, int n_exp)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    unsigned long long *st = S.st;
+    const int t = st[128] ? (int)st[129] : n_exp;
+    const unsigned long long n = st[t];
+    st[130] = n;
+    st[131] = (unsigned long long)(t & 1);
+    ctl[32] = n < (unsigned long long)W ? n : (unsigned long long)W;
 }
 
 // n bytes at p set to v (the buffer clears of a captured segment: a graph
@@ -1919,7 +2144,7 @@ enum {
     B_TOT, B_KEEPCNT, B_KEEPKP, B_KEEPNEXP, B_KEEPMETA, B_PREPORDER,
     B_SPR, B_SPKV, B_SPROFS, B_SPKVOFS, B_SPINTS,
     B_SV_PIDX, B_SV_Q, B_SV_M, B_SV_KN, B_SV_NEXP, B_SV_NS, B_SV_CNT, B_STATSAVE, B_STATOUT,
-    B_PCOUNT,
+    B_PCOUNT, B_F0, B_F1, B_SPLITST,
     B_NBUF
 };
 
@@ -2676,8 +2901,16 @@ static int finish_run(const pfa_run_input *in, pfa_run_output *out, int64_t npts
 // in the chosen form, growing and rerunning on overflow: the point buffer
 // (all forms) and the plain form's deques.  A rerun restores the record
 // statuses the previous attempt changed (saved on the device first).
+// nodes per wavefront the split aims at, and the deque capacity that
+// takes such a slice with room for the depth-first search below it
+#define PFA_SPLIT_DEFAULT 2048
+static int default_stack_cap(int D, int split_per_wave)
+{
+    return std::max(64 * (D + 2), split_per_wave > 0 ? 2 * std::min(split_per_wave, 1 << 16) + 128 : 0);
+}
+
 struct SearchTune {
-    int mode, waves_per_cu, budget, stack_cap;
+    int mode, waves_per_cu, budget, stack_cap, split;
 };
 
 struct SearchRun {
@@ -2704,12 +2937,21 @@ static void search_device(DevCtx *C, hipStream_t s, int64_t *d_recs, const RecL 
     const int donate_max = tn.budget > 0 ? tn.budget : 8;
     const int donate_min = 64;               // deque depth before a wavefront donates
     const int pt_chunk = 256;                // point slots reserved per atomic
-    int stack_cap = tn.stack_cap > 0 ? std::max(tn.stack_cap, 128) : 64 * (D + 2);
     const bool plain = tn.mode == PFA_MODE_PLAIN, lane = tn.mode == PFA_MODE_LANE;
+    const int split_per_wave = tn.split == 0 ? PFA_SPLIT_DEFAULT : tn.split;
+    int stack_cap = tn.stack_cap > 0 ? std::max(tn.stack_cap, 128)
+                                     : default_stack_cap(D, (plain || lane) ? -1 : split_per_wave);
     // plain: one workgroup per p-vector up to 8192, each p-vector's tree on
     // the wavefront that starts it
     const int nwaves = plain ? (int)std::max<int64_t>(1, std::min<int64_t>(n_p, 8192))
                              : std::max(1, C->cus * waves_per_cu);
+    // batched: the split's last frontier (at most the roots, or `room`
+    // nodes) starts the deques, so every slice fits with room to grow
+    if (!plain && !lane)
+        stack_cap = std::max<int64_t>(stack_cap, (n_p + nwaves - 1) / nwaves + 128);
+    const int n_exp = (plain || lane || tn.split < 0) ? 0 : std::min(D, PFA_SPLIT_MAX - 1);
+    const unsigned long long room = (unsigned long long)nwaves * (unsigned long long)(stack_cap / 2);
+    const unsigned long long fcap = std::max<unsigned long long>(room, (unsigned long long)n_p) + 64;
     // room for every root plus the donation reserve kept by donate()
     const unsigned long long ring_cap = (unsigned long long)std::max<int64_t>(2 * n_p + 64, 1 << 20) +
                                         2 * 64ULL * 4096ULL;
@@ -2744,8 +2986,15 @@ static void search_device(DevCtx *C, hipStream_t s, int64_t *d_recs, const RecL 
             stacks = (char *)dev_get(C, B_STACKS, (size_t)nwaves * stack_cap * E.bytes, errbuf);
         }
         QueueArgs Qa{ring, seq, ring_cap, d_ctl};
+        SplitArgs Sp{nullptr, nullptr, fcap, nullptr,
+                     (unsigned long long)nwaves * (unsigned long long)std::max(split_per_wave, 1), room};
+        if (!lane && !plain) {
+            Sp.F0 = (char *)dev_get(C, B_F0, (size_t)fcap * E.bytes, errbuf);
+            Sp.F1 = (char *)dev_get(C, B_F1, (size_t)fcap * E.bytes, errbuf);
+            Sp.st = (unsigned long long *)dev_get(C, B_SPLITST, 256 * 8, errbuf);
+        }
         BatchArgs A{d_recs, L, E, Qa, stacks, stack_cap, donate_max, donate_min, pt_chunk,
-                    Pt, Cn, deadline, n_p};
+                    Pt, Cn, deadline, n_p, Sp};
         auto kern = plain ? k_search<true> : k_search<false>;
         if (!lane)
             HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -2761,14 +3010,23 @@ static void search_device(DevCtx *C, hipStream_t s, int64_t *d_recs, const RecL 
                      attempt == 0 ? 0 : 1);
             if (!lane && !plain) {
                 sg.zero(seq, 8 * ring_cap);
-                if (n_p > 0) sg.k(k_roots, dim3((n_p + 255) / 256), dim3(256), 0, n_p, d_recs, L, Qa, E, Cn);
+                sg.zero(Sp.st, 256 * 8);
+                if (n_p > 0) sg.k(k_roots, dim3((n_p + 255) / 256), dim3(256), 0, n_p, d_recs, L, Sp, E, Cn);
             }
         }, errbuf);
         HIPCHK(hipEventRecord(ev[3], s));
         run_seg(C, s, [&](Seg &sg) {
             if (lane) {
                 if (n_p > 0) sg.k(k_lane, dim3((n_p + 63) / 64), dim3(64), 0, n_p, d_recs, L, Pt, D, Cn);
-            } else if (n_p > 0 || !plain) {
+            } else if (plain) {
+                if (n_p > 0) sg.k(kern, dim3(nwaves), dim3(64), bat_lds, A);
+            } else {
+                // the breadth-first split, then the persistent search from
+                // its last frontier
+                for (int t = 0; t < n_exp; ++t)
+                    sg.k(k_expand, dim3(nwaves), dim3(64), (size_t)64 * E.bytes, t, d_recs, L, E, Sp, Pt,
+                         Cn, pt_chunk);
+                sg.k(k_split_final, dim3(1), dim3(64), 0, Sp, d_ctl, nwaves, n_exp);
                 sg.k(kern, dim3(nwaves), dim3(64), bat_lds, A);
             }
         }, errbuf);
@@ -2899,7 +3157,7 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     if (C->pts_reserved > 0) est = std::min(est, std::max(C->pts_reserved, pcap));
     pcap = std::max(pcap, est);
     SearchRun R;
-    SearchTune tn{in->mode, in->waves_per_cu, in->budget, in->stack_cap};
+    SearchTune tn{in->mode, in->waves_per_cu, in->budget, in->stack_cap, in->split};
     search_device(C, s, d_recs, L, n_p, D, tn, pcap, R, errbuf);
     // post: M, Kn, the expansion count and the det filter (k_filter).
     // keep_points 1: every point's rows are stored and copied back;
@@ -3515,7 +3773,15 @@ extern "C" int pfa_reserve(int device, int64_t n_p, int64_t n_points, int h_max,
             (unsigned long long)std::max<int64_t>(2 * n_p + 64, 1 << 20) + 2 * 64ULL * 4096ULL;
         dev_get(C, B_RING, (size_t)ring_cap * E.bytes, errbuf);
         dev_get(C, B_SEQ, 8 * ring_cap, errbuf);
-        dev_get(C, B_STACKS, (size_t)std::max(1, C->cus * 4) * 64 * (D + 2) * E.bytes, errbuf);
+        // deques and the split's frontier buffers at the default settings,
+        // for up to 4 wavefronts per CU
+        const int sc = default_stack_cap(D, PFA_SPLIT_DEFAULT);
+        dev_get(C, B_STACKS, (size_t)std::max(1, C->cus * 4) * sc * E.bytes, errbuf);
+        const unsigned long long room = (unsigned long long)std::max(1, C->cus * 4) * (unsigned long long)(sc / 2);
+        const unsigned long long fcap = std::max<unsigned long long>(room, (unsigned long long)np1) + 64;
+        dev_get(C, B_F0, (size_t)fcap * E.bytes, errbuf);
+        dev_get(C, B_F1, (size_t)fcap * E.bytes, errbuf);
+        dev_get(C, B_SPLITST, 256 * 8, errbuf);
         dev_get(C, B_M, 8 * n1 * HS, errbuf);
         dev_get(C, B_KN, 8 * n1 * HS, errbuf);
         dev_get(C, B_NEXP, 8 * n1, errbuf);
@@ -4017,7 +4283,7 @@ extern "C" int pfa_kernel_batch(const pfa_kernel_input *in, pfa_kernel_output *o
             // the same search forms as the pipeline (lane, plain, batched),
             // with its overflow reruns
             SearchRun R;
-            SearchTune tn{in->mode, 0, 0, 0};
+            SearchTune tn{in->mode, 0, 0, 0, 0};
             search_device(C, s, d_recs, L, n, D, tn, pcap, R, errbuf);
             const unsigned long long npts = R.npts;
             HIPCHK(hipMemcpyAsync(recs.data(), d_recs, 8 * recs.size(), hipMemcpyDeviceToHost, s));

@@ -55,7 +55,7 @@ _RunIn = _struct("pfa_run_input", [
     ("device", ctypes.c_int), ("mode", ctypes.c_int), ("waves_per_cu", ctypes.c_int),
     ("budget", ctypes.c_int), ("stack_cap", ctypes.c_int), ("host_threads", ctypes.c_int),
     ("filter", ctypes.c_int), ("sort", ctypes.c_int), ("keep_points", ctypes.c_int),
-    ("verbose", ctypes.c_int), ("assemble", ctypes.c_int)])
+    ("verbose", ctypes.c_int), ("assemble", ctypes.c_int), ("split", ctypes.c_int)])
 _RunOut = _struct("pfa_run_output", [
     ("n", _c_i64), ("M", _P64), ("Kn", _P64), ("q", _P64), ("pidx", _P64), ("pstat", _P8),
     ("pcode", _P32), ("nexp", _P64), ("nonsing", _P8), ("stats", _c_f64 * len(STAT_NAMES)),
@@ -254,7 +254,8 @@ class _Batch:
                       _ptr(self.ps), _ptr(self.pgeo, _P32), int(device), MODES[mode],
                       int(tune.get("waves_per_cu", 0)), int(tune.get("budget", 0)),
                       int(tune.get("stack_cap", 0)), int(host_threads), int(bool(filter)),
-                      int(bool(sort)), KEEP[keep], int(bool(verbose)), int(bool(assemble)))
+                      int(bool(sort)), KEEP[keep], int(bool(verbose)), int(bool(assemble)),
+                      int(tune.get("split", 0)))
 
 
 _free = ctypes.CDLL(None).free
@@ -291,13 +292,17 @@ def _copy(ptr, n, dtype):
 
 
 def run(geoms, ps, pgeo, mode="batched", device=0, waves_per_cu=0, budget=0, stack_cap=0,
-        host_threads=0, filter=True, sort=True, keep_points=True, verbose=False, assemble=False):
+        host_threads=0, filter=True, sort=True, keep_points=True, verbose=False, assemble=False,
+        split=0):
     """One pfa_coni_run call.
 
     geoms: per geometry kappa (h, h, h), Mbasis (h, h), Q, dilation, M0min;
     ps: p-vectors with the leading coni 0; pgeo: geometry of each.
     mode: "batched" (persistent, work stealing), "plain" (one launch, one
     wavefront per p-vector), "lane" (one thread per p-vector), "host".
+    split (batched): nodes per persistent wavefront that the breadth-first
+    expansion of the roots aims at before the persistent search (0: the
+    library's default; < 0: no expansion, the roots start the search).
     keep_points: True / "all" every lattice point, "survivors" only those
     the filter keeps for the PFVs, False counters only.
     Returns a dict: M, Kn (n x max_h11()), q, pidx, nexp, nonsing per
@@ -307,7 +312,7 @@ def run(geoms, ps, pgeo, mode="batched", device=0, waves_per_cu=0, budget=0, sta
     finished)."""
     lib = require()
     b = _Batch.build(geoms, ps, pgeo)
-    tune = dict(waves_per_cu=waves_per_cu, budget=budget, stack_cap=stack_cap)
+    tune = dict(waves_per_cu=waves_per_cu, budget=budget, stack_cap=stack_cap, split=split)
     inp = b.run_input(mode, device, tune, host_threads, filter, sort, keep_points, verbose,
                       assemble)
     out = _RunOut()

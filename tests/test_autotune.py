@@ -32,7 +32,7 @@ def test_tune_picks_fastest_and_caches(tmp_path, monkeypatch):
     run, calls = _fake_runner(times)
     ps = np.zeros((10, 41), np.int64)
     pgeo = np.zeros(10, np.int32)
-    grid = {"waves_per_cu": (2, 4), "budget": (4, 8)}
+    grid = {"waves_per_cu": (2, 4), "budget": (4, 8), "split": (2048,)}
     best = autotune.tune(_geoms(), ps, pgeo, grid=grid, runner=run, repeats=1)
     assert (best["waves_per_cu"], best["budget"]) == (4, 8)
     assert best["ms_search"] == 3.0 and not best["cached"]
@@ -50,7 +50,7 @@ def test_tune_rejects_node_mismatch(tmp_path, monkeypatch):
     ps = np.zeros((4, 41), np.int64)
     with pytest.raises(RuntimeError, match="visited"):
         autotune.tune(_geoms(), ps, np.zeros(4, np.int32),
-                      grid={"waves_per_cu": (2, 4), "budget": (4, 8)}, runner=run,
+                      grid={"waves_per_cu": (2, 4), "budget": (4, 8), "split": (2048,)}, runner=run,
                       repeats=1, use_cache=False)
 
 
@@ -73,11 +73,12 @@ def test_gpu_tune_real_batch(need_gpu, tmp_path, monkeypatch):
         geoms.append(dict(kappa=kap, Mbasis=Mb, Q=inst.Q, dilation=D, M0min=13))
         pgeo[dil == D] = k
     pf = np.hstack([np.zeros((len(ps), 1), np.int64), ps])
-    best = autotune.tune(geoms, pf, pgeo, grid={"waves_per_cu": (2, 4), "budget": (4, 16)},
+    best = autotune.tune(geoms, pf, pgeo, grid={"waves_per_cu": (2, 4), "budget": (4, 16),
+                                                "split": (-1, 2048)},
                          repeats=1)
     assert best["waves_per_cu"] in (2, 4) and best["budget"] in (4, 16)
     r0 = gpu.run(geoms, pf, pgeo, mode="batched")
     r1 = gpu.run(geoms, pf, pgeo, mode="batched", waves_per_cu=best["waves_per_cu"],
-                 budget=best["budget"])
+                 budget=best["budget"], split=best["split"])
     assert r0["stats"]["nodes"] == r1["stats"]["nodes"]
     assert (r0["M"] == r1["M"]).all() and (r0["pidx"] == r1["pidx"]).all()
