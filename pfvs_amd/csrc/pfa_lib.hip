@@ -2144,7 +2144,7 @@ enum {
     B_TOT, B_KEEPCNT, B_KEEPKP, B_KEEPNEXP, B_KEEPMETA, B_PREPORDER,
     B_SPR, B_SPKV, B_SPROFS, B_SPKVOFS, B_SPINTS,
     B_SV_PIDX, B_SV_Q, B_SV_M, B_SV_KN, B_SV_NEXP, B_SV_NS, B_SV_CNT, B_STATSAVE, B_STATOUT,
-    B_PCOUNT, B_F0, B_F1, B_SPLITST,
+    B_PCOUNT, B_F0, B_F1, B_SPLITST, B_BOX, B_BOXOUT,
     B_NBUF
 };
 
@@ -3681,6 +3681,20 @@ static int box_prefix(int n, int B, int64_t target)
     return s;
 }
 
+// device form: one thread per unit, grid-stride; counts (write 0) or rows
+// at their offsets (write 1)
+__global__ __launch_bounds__(256) void k_box(PfaBox X, PfaBoxScr S0, int64_t n_units, int write,
+                                             int64_t *counts, const int64_t *offs, int64_t *out,
+                                             int64_t cap)
+{
+    PfaBoxScr S = S0;
+    S.tid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t u = S.tid; u < n_units; u += S.T) {
+        if (write) pfa_box_unit_thread(X, S, u, out, offs[u], cap);
+        else counts[u] = pfa_box_unit_thread(X, S, u, nullptr, 0, 0);
+    }
+}
+
 extern "C" int pfa_box_enum(const int32_t *H, int m, int n, const int64_t *rhs, int B, int primitive,
                             int device, int64_t cap, int64_t *out, int64_t *n_out, int64_t *n_total,
                             double *ms, char *err)
@@ -3695,8 +3709,9 @@ extern "C" int pfa_box_enum(const int32_t *H, int m, int n, const int64_t *rhs, 
         return 2;
     }
     // H^T and the reachable maxima rest[j][r] = B sum_{k >= j} |H_rk|
-    std::vector<int32_t> Ht((size_t)n * std::max(m, 1));
-    std::vector<int64_t> rest((size_t)(n + 1) * std::max(m, 1), 0);
+    const int mm = std::max(m, 1);
+    std::vector<int32_t> Ht((size_t)n * mm, 0);
+    std::vector<int64_t> rest((size_t)(n + 1) * mm, 0);
     for (int r = 0; r < m; ++r)
         for (int j = 0; j < n; ++j) Ht[(size_t)j * m + r] = H[(size_t)r * n + j];
     for (int j = n - 1; j >= 0; --j)
@@ -3704,40 +3719,81 @@ extern "C" int pfa_box_enum(const int32_t *H, int m, int n, const int64_t *rhs, 
             const int64_t a = H[(size_t)r * n + j];
             rest[(size_t)j * m + r] = rest[(size_t)(j + 1) * m + r] + (int64_t)B * (a < 0 ? -a : a);
         }
-    std::vector<int64_t> rhs_v(std::max(m, 1), 0);
+    std::vector<int64_t> rhs_v(mm, 0);
     for (int r = 0; r < m; ++r) rhs_v[r] = rhs[r];
-    if (device >= 0) {
-        snprintf(errbuf, 256, "pfa_box_enum: the device form is not enabled in this build; "
-                              "use device < 0 (host threads)");
-        return 2;
-    }
     auto t0 = std::chrono::steady_clock::now();
+    const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    const int s = box_prefix(n, B, device >= 0 ? 65536 : 64 * nth);
+    int64_t n_units = 1;
+    for (int k = 0; k < s; ++k) n_units *= 2 * (int64_t)B + 1;
+    std::vector<int64_t> counts(n_units), offs(n_units);
+    int64_t tot = 0;
+    auto prefix = [&]() {
+        tot = 0;
+        for (int64_t u = 0; u < n_units; ++u) { offs[u] = tot; tot += counts[u]; }
+    };
     try {
-        {
-            const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
-            const int s = box_prefix(n, B, 64 * nth);
+        if (device < 0) {
+            // host threads: the device's per-unit walk, each thread with its
+            // own scratch (T = 1)
             PfaBox X{m, n, B, primitive ? 1 : 0, s, Ht.data(), rest.data(), rhs_v.data()};
-            int64_t n_units = 1;
-            for (int k = 0; k < s; ++k) n_units *= 2 * (int64_t)B + 1;
-            std::vector<int64_t> counts(n_units), offs(n_units);
             auto run = [&](bool write) {
                 parallel_chunks(n_units, 16, nth, [&](int64_t lo, int64_t hi) {
-                    int32_t p[PFA_BOX_MAXN], cur[PFA_BOX_MAXN], hh[PFA_BOX_MAXN];
+                    int64_t part[PFA_BOX_MAXM];
+                    int32_t cur[PFA_BOX_MAXN], hh[PFA_BOX_MAXN];
                     uint64_t g[PFA_BOX_MAXN];
-                    const PfaBoxTeam T{0, 1};
+                    const PfaBoxScr S{part, cur, hh, g, 0, 1};
                     for (int64_t u = lo; u < hi; ++u) {
-                        if (write) pfa_box_unit<PFA_BOX_MAXM>(X, T, u, p, cur, hh, g, out, offs[u], cap);
-                        else counts[u] = pfa_box_unit<PFA_BOX_MAXM>(X, T, u, p, cur, hh, g, nullptr, 0, 0);
+                        if (write) pfa_box_unit_thread(X, S, u, out, offs[u], cap);
+                        else counts[u] = pfa_box_unit_thread(X, S, u, nullptr, 0, 0);
                     }
                 });
             };
             run(false);
-            int64_t tot = 0;
-            for (int64_t u = 0; u < n_units; ++u) { offs[u] = tot; tot += counts[u]; }
-            *n_total = tot;
+            prefix();
             run(true);
-            *n_out = std::min(tot, cap);
+        } else {
+            DevCtx *C = ctx_for(device);
+            std::lock_guard<std::mutex> lk(C->mu);
+            ctx_init(C, device, errbuf);
+            hipStream_t st = C->stream;
+            const int T = (int)std::min<int64_t>(std::max<int64_t>(n_units, 1), (int64_t)C->cus * 256);
+            const int grid = (T + 255) / 256, Tg = grid * 256;
+            // inputs and the per-thread scratch (element k of thread t at k Tg + t)
+            size_t o = 0;
+            auto take = [&](size_t bytes) { size_t at = o; o += (bytes + 255) & ~(size_t)255; return at; };
+            const size_t oHt = take(4 * Ht.size()), oRest = take(8 * rest.size()), oRhs = take(8 * rhs_v.size());
+            const size_t oPart = take(8 * (size_t)mm * Tg), oCur = take(4 * (size_t)n * Tg),
+                         oHi = take(4 * (size_t)n * Tg), oG = take(8 * (size_t)n * Tg);
+            const size_t oCnt = take(8 * (size_t)n_units), oOffs = take(8 * (size_t)n_units);
+            char *d = (char *)dev_get(C, B_BOX, o, errbuf);
+            HIPCHK(hipMemcpyAsync(d + oHt, Ht.data(), 4 * Ht.size(), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(d + oRest, rest.data(), 8 * rest.size(), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(d + oRhs, rhs_v.data(), 8 * rhs_v.size(), hipMemcpyHostToDevice, st));
+            PfaBox X{m, n, B, primitive ? 1 : 0, s, (const int32_t *)(d + oHt), (const int64_t *)(d + oRest),
+                     (const int64_t *)(d + oRhs)};
+            PfaBoxScr S{(int64_t *)(d + oPart), (int32_t *)(d + oCur), (int32_t *)(d + oHi),
+                        (uint64_t *)(d + oG), 0, Tg};
+            int64_t *d_cnt = (int64_t *)(d + oCnt), *d_offs = (int64_t *)(d + oOffs);
+            hipLaunchKernelGGL(k_box, dim3(grid), dim3(256), 0, st, X, S, n_units, 0, d_cnt,
+                               (const int64_t *)d_offs, (int64_t *)nullptr, (int64_t)0);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(counts.data(), d_cnt, 8 * (size_t)n_units, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            prefix();
+            const int64_t rows = std::min(tot, cap);
+            if (rows > 0) {
+                int64_t *d_out = (int64_t *)dev_get(C, B_BOXOUT, 8 * (size_t)rows * n, errbuf);
+                HIPCHK(hipMemcpyAsync(d_offs, offs.data(), 8 * (size_t)n_units, hipMemcpyHostToDevice, st));
+                hipLaunchKernelGGL(k_box, dim3(grid), dim3(256), 0, st, X, S, n_units, 1, d_cnt,
+                                   (const int64_t *)d_offs, d_out, rows);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(out, d_out, 8 * (size_t)rows * n, hipMemcpyDeviceToHost, st));
+            }
+            HIPCHK(hipStreamSynchronize(st));
         }
+        *n_total = tot;
+        *n_out = std::min(tot, cap);
     } catch (int) {
         return 2;
     }

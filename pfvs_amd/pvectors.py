@@ -3,9 +3,10 @@ box, the input side of coniZpM (reference: pfvs/pvectors.py:58-89, which
 takes it from the latticepts dependency; the same box_enum contract is
 called at pfvs/distributed.py:121-139).
 
-Both forms run the library's enumeration (pfa_box_enum) on host threads
-(the device form is not enabled in this build).  Points come in
-lexicographic order (p_0 outermost, ascending).
+The library's enumeration (pfa_box_enum) runs on host threads ("cpu",
+"auto") or on the GPU ("gpu", a device index: one GPU thread per unit of
+the box), with the same per-unit walk.  Points come in lexicographic order
+(p_0 outermost, ascending).
 """
 
 from __future__ import annotations
@@ -17,7 +18,7 @@ def _device(device):
     if device in ("cpu", "auto", -1):
         return -1
     if device == "gpu" or (isinstance(device, int) and device >= 0):
-        return 0 if device == "gpu" else int(device)     # the library reports it unavailable
+        return 0 if device == "gpu" else int(device)
     raise ValueError(f"device must be 'auto', 'cpu', 'gpu' or a device index, got {device!r}")
 
 

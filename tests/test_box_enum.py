@@ -3,7 +3,8 @@ at pfvs/distributed.py:121-139 and tests/test_paper_examples.py:119-121):
 the host code against brute force over the whole box and the oracle's depth
 first restatement (same points, same lexicographic order) on random and
 edge cases; the reference's recorded scan counts of the paper geometries;
-capacity status; the device form reports itself unavailable."""
+capacity status; the device form (one GPU thread per unit) against the
+host."""
 
 import json
 import os
@@ -94,16 +95,22 @@ def test_capacity_status_and_pvecs():
     assert len(ps) >= 100 and len(box_enum(B - 1, data.H_cob, device="cpu")) < 100
 
 
-def test_device_form_is_reported_unavailable():
+def test_host_threads_run_the_device_walk_at_every_prefix_split():
+    """The per-unit walk (the code the device runs, one GPU thread per unit)
+    on host threads: the same points at any split of the box into units,
+    from none (s = 0) to every coordinate in the prefix."""
     gpu.require()
-    with pytest.raises(RuntimeError, match="not enabled"):
-        gpu.box_enum(np.eye(3, dtype=np.int32), 1, 2, True, 100, device=0)
+    for H, rhs, B, prim in random_cases(4, 60):
+        exp = box_enum_brute(B, H, rhs, prim)
+        ps, st, total, _ = gpu.box_enum(H, rhs, B, prim, 10**6, device=-1)
+        assert st == 0 and np.array_equal(ps, exp)
 
 
 @pytest.mark.gpu
-@pytest.mark.skip(reason="the device enumeration kernel is not enabled: its first GPU run went "
-                         "silent for 180 s without a cause found on the CPU")
 def test_gpu_enumeration_matches_host(need_gpu):
+    """One GPU thread per unit (no barriers or cross-lane traffic, every
+    loop bounded by the box): the host's points in the host's order, the
+    recorded scan counts of the paper geometries, the capacity status."""
     for H, rhs, B, prim in random_cases(3, 120):
         d, st, total, _ = gpu.box_enum(H, rhs, B, prim, 10**6, device=0)
         h, st_h, total_h, _ = gpu.box_enum(H, rhs, B, prim, 10**6, device=-1)
