@@ -184,6 +184,20 @@ int pfa_assemble(int h, int64_t Q, int64_t n_keys, int64_t n, const int64_t *key
                  int threads, pfa_assemble_output *out);
 void pfa_assemble_output_free(pfa_assemble_output *out);
 
+/* coniZp's dilation bound (pfvs/dilation.py:152-231; the reference's C
+ * form pfvs/fp_kernel/coni_bound.h:255-399, bound through
+ * fp_kernel.pyx's _coni_mu0_batch): for each direction p_r (row i of ps,
+ * h - 1 entries; p_hat = (0, p_r)), A = kappa . p_hat, S = -A_rr^-1 and
+ * Lambda = {K : p_r . K = 0}; mu0 = min over nonzero K in Lambda of
+ * K^T S K.  status[i]: 0 mu0 = num[i] / den[i] (reduced) and K (row i,
+ * h - 1 entries) a minimizer; 1 no bound (A_rr singular, s > 0, or S not
+ * positive definite on Lambda); < 0 not decided here (a value beyond
+ * int64/int128, a form too ill-conditioned for the FP64 search): the
+ * caller's exact path decides.  Host threads (threads <= 0: all cores).
+ * Returns 0, or 1 with a message for bad sizes. */
+int pfa_coni_mu0(int h, const int64_t *kappa, int64_t n_p, const int64_t *ps, int threads,
+                 int32_t *status, int64_t *num, int64_t *den, int64_t *K, char *err);
+
 /* Single-problem kernel batch (the contract of conipfv_kernel,
  * pfvs/fp_kernel/fp_kernel.pyx:258-318 / fp_kernel.h:25-63): for problem
  * k of dimension dims[k], mat (d*d), H (nh[k] rows of d, echelon form,

@@ -15,6 +15,11 @@ any decision) used to pin the C oracle and the product on small cases:
   (fp_kernel.pyx:480-622) computed from scratch with Python ints: the
   lattice {M in Mbasis Z^h : (Z p).M = 0}, its ellipsoid, and the M0 / gcd
   cuts, independent of the basis any implementation picks.
+- `direction_and_delta`, `schur_form`, `mu0_box`: coniZp's dilation bound
+  (reference pfvs/dilation.py:1-30 and tests/test_dilation_bound.py:63-80,
+  132-148): a coni PFV's direction and dilation from its fluxes, the form
+  S = -A_rr^-1 with s, and the minimum of K^T S K over a box of K in
+  Lambda = {p_r . K = 0} (brute force).
 """
 
 from __future__ import annotations
@@ -395,3 +400,76 @@ def pfvs_from_points(points, kappa, h, Q, M0min):
         if not singular_cache[tM]:
             out.append((K, M, key))
     return out
+
+
+# ------------------------------------------------------ coniZp's bound
+def _inverse(A):
+    """A^-1 in Fractions, or None if A is singular."""
+    n = len(A)
+    M = [[Fraction(int(v)) for v in row] + [Fraction(int(i == j)) for j in range(n)]
+         for i, row in enumerate(A)]
+    for col in range(n):
+        piv = next((i for i in range(col, n) if M[i][col] != 0), None)
+        if piv is None:
+            return None
+        M[col], M[piv] = M[piv], M[col]
+        pv = M[col][col]
+        M[col] = [u / pv for u in M[col]]
+        for i in range(n):
+            if i != col and M[i][col] != 0:
+                f = M[i][col]
+                M[i] = [u - f * w for u, w in zip(M[i], M[col])]
+    return [row[n:] for row in M]
+
+
+def schur_form(kappa, p_r):
+    """(S = -A_rr^-1, s = a00 - a_r0^T A_rr^-1 a_r0) for A = kappa . (0, p_r),
+    in Fractions, or None if A_rr is singular (pfvs/dilation.py:49-68)."""
+    h = len(kappa)
+    p = [0] + [int(x) for x in p_r]
+    A = [[sum(int(kappa[a][b][c]) * p[c] for c in range(h)) for b in range(h)] for a in range(h)]
+    inv = _inverse([row[1:] for row in A[1:]])
+    if inv is None:
+        return None
+    a = [A[i][0] for i in range(1, h)]
+    s = A[0][0] - sum(a[i] * inv[i][j] * a[j] for i in range(h - 1) for j in range(h - 1))
+    return [[-x for x in row] for row in inv], s
+
+
+def form_value(S, K):
+    return sum(S[i][j] * int(K[i]) * int(K[j]) for i in range(len(K)) for j in range(len(K)))
+
+
+def mu0_box(kappa, p_r, box):
+    """min of K^T S K over nonzero K with |K_i| <= box and p_r . K = 0
+    (brute force; the reference's _brute_mu0, tests/test_dilation_bound.py:
+    132-148)."""
+    import itertools
+    f = schur_form(kappa, p_r)
+    if f is None:
+        return None
+    S, _ = f
+    best = None
+    for K in itertools.product(range(-box, box + 1), repeat=len(p_r)):
+        if not any(K) or sum(k * int(x) for k, x in zip(K, p_r)) != 0:
+            continue
+        v = form_value(S, K)
+        best = v if best is None or v < best else best
+    return best
+
+
+def direction_and_delta(kappa, K, M):
+    """(p_hat, delta) of a coni PFV from its fluxes: K_r = N_rr p_r with
+    N = kappa . M and p = p_hat / delta, p_hat primitive with p_hat_0 = 0
+    (reference tests/test_dilation_bound.py:63-80)."""
+    h = len(kappa)
+    N = [[sum(int(kappa[a][b][c]) * int(M[c]) for c in range(h)) for b in range(h)] for a in range(h)]
+    pr = solve_exact([row[1:] for row in N[1:]], [int(x) for x in K[1:]])
+    den = 1
+    for x in pr:
+        den = den * x.denominator // math.gcd(den, x.denominator)
+    ints = [int(x * den) for x in pr]
+    g = 0
+    for x in ints:
+        g = math.gcd(g, abs(x))
+    return [0] + [x // g for x in ints], Fraction(den, g)

@@ -300,7 +300,7 @@ def coniZpM(data, ps, Q=None, M0min=13, ellipsoid_dilation=1, use_gcd_lattice=Fa
 
 
 def assemble(r, ps, p_full, kappa, Mbasis, Q, qmax, M0min, max_N_pfvs=1_000_000_000,
-             verbosity=0):
+             verbosity=0, with_keys=False):
     """(Ks, Ms) of one geometry's p-vectors from a pipeline result r (its
     points grouped by the p-vector indices pidx into ps; pstat and pcount
     per p-vector): the status errors, the exact host path for every
@@ -340,4 +340,103 @@ def assemble(r, ps, p_full, kappa, Mbasis, Q, qmax, M0min, max_N_pfvs=1_000_000_
     Ms = np.vstack([p[1] for p in pieces] or [np.zeros((0, h), np.int64)])
     keys = np.concatenate([p[2] for p in pieces] or [np.zeros(0, np.int64)])
     order = np.argsort(keys, kind="stable")
+    if with_keys:
+        return Ks[order], Ms[order], keys[order]
     return Ks[order], Ms[order]
+
+
+# ------------------------------------------------------------------ coniZp
+def _grid_up(D):
+    """D rounded up onto the grid ceil(2^(k/16)) (at least D itself), so
+    that nearby dilation bounds share one geometry entry of the batched
+    search (pfvs/coni.py:650-656)."""
+    D = np.maximum(np.asarray(D, dtype=np.int64), 1)
+    g = np.ceil(2.0 ** (np.ceil(16 * np.log2(D)) / 16)).astype(np.int64)
+    return np.maximum(g, D)
+
+
+def _canonical(Ks, Ms, keys):
+    """PFVs without repeats, sorted by p-vector index, then M, then K
+    (pfvs/coni.py:489-502)."""
+    keys = np.asarray(keys, dtype=np.int64)
+    if len(keys) == 0:
+        return Ks, Ms, keys
+    h = Ms.shape[1]
+    if Ks.dtype == object or Ms.dtype == object:
+        rows = sorted({(int(k), tuple(map(int, m)), tuple(map(int, kk)))
+                       for k, kk, m in zip(keys, Ks, Ms)})
+        return (np.array([r[2] for r in rows], dtype=object),
+                np.array([r[1] for r in rows], dtype=object),
+                np.array([r[0] for r in rows], dtype=np.int64))
+    A = np.unique(np.hstack([keys[:, None], Ms, Ks]), axis=0)
+    return A[:, 1 + h:], A[:, 1:1 + h], A[:, 0]
+
+
+def coniZp(data, ps, Q=None, M0min=13, fallback_dilation=1, n_jobs=-1, device="auto",
+           cost_model=None, max_N_pfvs=1_000_000_000, return_formal_pfvs=False, verbosity=0,
+           max_dilation=None):
+    """Every coni PFV of each p-vector's direction, at any dilation
+    (pfvs/coni.py:1171-1223): (Ks, Ms, complete).
+
+    Each direction with a dilation bound b (pfvs_amd.dilation: every PFV of
+    the direction has dilation < b) is searched by coniZpM at ceil(b)
+    rounded up onto a 2^(1/16) grid; all directions go through one batched
+    search with one geometry entry per distinct dilation.  A direction
+    without a bound is searched at fallback_dilation and reported
+    incomplete.  The reference can instead split a direction with a large
+    bound into ZpM up to a dilation D0 plus a short-vector search above D0
+    (coniZpK), chosen by measured costs (cost_model); that split is not
+    provided here, so every bounded direction is searched to its bound (the
+    same PFVs).  max_dilation (optional) caps the search: a direction whose
+    bound exceeds it is searched to max_dilation and reported incomplete.
+    Output as coniZpM's rows, sorted by p-vector, then (M, K), each PFV
+    once; complete[i] whether direction i was searched exhaustively."""
+    from .dilation import coni_dilation_bound_ceils
+    if fallback_dilation <= 0:
+        raise ValueError(f"fallback_dilation must be > 0, got {fallback_dilation}.")
+    _check_inputs(data, ps, 1)
+    if cost_model is not None:
+        raise NotImplementedError("cost_model routes directions to coniZpK, which is not "
+                                  "provided: every bounded direction is searched to its bound")
+    if return_formal_pfvs:
+        raise NotImplementedError("PFV objects are not provided; use (Ks, Ms)")
+    ps = np.array(ps)
+    kappa, h11 = data.kappa_cob, data.h11
+    Mbasis = data.M_lattice()
+    Q = h11 + data.h21 + 4 if Q is None else Q
+    p_int = util.as_integral(ps)
+    if p_int.dtype == object:
+        raise IncompleteSearchError("p-vector entries exceed int64")
+    p_full = np.concatenate([np.zeros((len(ps), 1), np.int64), p_int.astype(np.int64)], axis=1)
+    threads = 0 if n_jobs in (-1, None) else max(1, int(n_jobs))
+    b = coni_dilation_bound_ceils(p_int, kappa, Q, n_jobs=threads or -1)
+    has = b > 0
+    dil = np.where(has, _grid_up(np.where(has, b, 1)), 0).astype(np.float64)
+    dil[~has] = fallback_dilation
+    complete = has.copy()
+    if max_dilation is not None:
+        over = has & (b > max_dilation)
+        dil[over] = max_dilation
+        complete &= ~over
+    Ds, pgeo = np.unique(dil, return_inverse=True)
+    geoms = [{"kappa": kappa, "Mbasis": Mbasis, "Q": Q, "dilation": float(D), "M0min": M0min}
+             for D in Ds]
+    qmax = np.array([gpu.qmax_of(Q, float(D)) for D in dil], dtype=np.int64)
+    if verbosity >= 1:
+        print(f"coniZp: {len(ps)} p-vectors, {int(np.sum(~has))} without a bound (searched at "
+              f"dilation {fallback_dilation}), {len(Ds)} distinct dilations up to {Ds.max():g}")
+    on_gpu = _use_gpu(device, h11, len(ps))
+    call = dict(filter=True, sort=True, keep_points="survivors", assemble=True,
+                host_threads=threads)
+    pgeo = pgeo.astype(np.int32)
+    try:
+        r = gpu.run(geoms, p_full, pgeo, mode="batched" if on_gpu else "host", **call)
+    except RuntimeError as e:
+        if not on_gpu or device == "gpu":
+            raise
+        warnings.warn(f"GPU path failed ({e}); using the CPU", RuntimeWarning, stacklevel=2)
+        r = gpu.run(geoms, p_full, pgeo, mode="host", **call)
+    Ks, Ms, keys = assemble(r, ps, p_full, kappa, Mbasis, Q, qmax, M0min, max_N_pfvs, verbosity,
+                            with_keys=True)
+    Ks, Ms, _ = _canonical(Ks, Ms, keys)
+    return Ks, Ms, complete

@@ -39,6 +39,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <thread>
@@ -46,6 +47,7 @@
 #include <vector>
 
 #include "pfa_box.h"
+#include "pfa_bound.h"
 #include "pfa_int.h"
 #include "pfa_post.h"
 #include "pfa_prep.h"
@@ -3799,6 +3801,86 @@ extern "C" int pfa_box_enum(const int32_t *H, int m, int n, const int64_t *rhs, 
     }
     if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return *n_total > cap ? 1 : 0;
+}
+
+// ------------------------------------------ coniZp's dilation bounds
+// mu0 for every direction (pfa_bound.h), on host threads: the exact form,
+// then every y with y^T Gr y at most Gr's smallest diagonal entry by the
+// library's Fincke-Pohst search (pfa_dfs on a record with no cuts), and the
+// smallest nonzero value.
+static int coni_mu0_one(int h, const int64_t *kappa, const int64_t *p_r, int64_t *num, int64_t *den,
+                        int64_t *K)
+{
+    std::unique_ptr<PfaBoundForm> Fp(new PfaBoundForm);
+    PfaBoundForm &F = *Fp;
+    int st = pfa_bound_form(h, kappa, p_r, F);
+    if (st) return st;
+    const int n = F.n;
+    int64_t best = INT64_MAX;
+    for (int k = 0; k < n; ++k) best = std::min(best, F.Gr[k * n + k]);
+    if (best <= 0) return 1;
+    const RecL L = rec_layout(n);
+    std::vector<int64_t> rec((size_t)L.words, 0);
+    int64_t *r = rec.data();
+    int32_t *ri = (int32_t *)r;
+    ri[0] = PFA_HOST;
+    ri[1] = n;
+    r[1] = best;
+    r[2] = 1;
+    r[3] = INT64_MIN;
+    ri[20] = -1;
+    ri[23] = 1;
+    ri[24] = n + 1;
+    int64_t mm = 0;
+    for (int e = 0; e < n * n; ++e) {
+        r[L.mat + e] = F.Gr[e];
+        mm = std::max<int64_t>(mm, F.Gr[e] < 0 ? -F.Gr[e] : F.Gr[e]);
+    }
+    r[16] = mm;
+    std::vector<char> ws((size_t)2 * n * n * 8 + 64);
+    const PfaTeam T{0, 1};
+    factor_one(T, ws.data(), r, L);
+    if (rec_status(r) != PFA_OK) return -5;                     // ill-conditioned for FP64
+    PfaProb P = rec_view(r, L);
+    std::vector<PfaNode> stk(PFA_DMAX);
+    int32_t c[PFA_DMAX], y[PFA_DMAX];
+    PfaCounts cnt{0, 0, 0};
+    int64_t qmin = INT64_MAX;
+    auto emit = [&](const int32_t *cc, int64_t q) -> int {
+        if (q > 0 && q < qmin) {
+            qmin = q;
+            for (int j = 0; j < n; ++j) y[j] = cc[j];
+        }
+        return 0;
+    };
+    const int rr = pfa_dfs(P, stk.data(), c, cnt, emit);
+    if (rr) return -6;
+    if (qmin == INT64_MAX) return -7;                           // cannot happen: the best basis vector
+    if (pfa_bound_K(F, y, K)) return -1;
+    const int64_t g = (int64_t)pfa_gcd64((uint64_t)qmin, (uint64_t)F.L);
+    *num = qmin / g;
+    *den = F.L / g;
+    return 0;
+}
+
+extern "C" int pfa_coni_mu0(int h, const int64_t *kappa, int64_t n_p, const int64_t *ps, int threads,
+                            int32_t *status, int64_t *num, int64_t *den, int64_t *K, char *err)
+{
+    if (h < 2 || h > PFA_MAX_H11 || n_p < 0) {
+        snprintf(err, 256, "pfa_coni_mu0: h11 outside 2..%d or a negative count", PFA_MAX_H11);
+        return 1;
+    }
+    const int m = h - 1;
+    const int nth = threads > 0 ? threads
+                                : std::max(1, std::min(64, (int)std::thread::hardware_concurrency()));
+    parallel_chunks(n_p, 64, nth, [&](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; ++i) {
+            num[i] = den[i] = 0;
+            for (int j = 0; j < m; ++j) K[i * m + j] = 0;
+            status[i] = coni_mu0_one(h, kappa, ps + i * m, num + i, den + i, K + i * m);
+        }
+    });
+    return 0;
 }
 
 extern "C" int pfa_reserve(int device, int64_t n_p, int64_t n_points, int h_max, char *err)

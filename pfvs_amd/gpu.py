@@ -103,6 +103,7 @@ SIGNATURES = {
     "pfa_box_enum": ([_P32, _I, _I, _P64, _I, _I, _I, _c_i64, _P64, _P64, _P64,
                       ctypes.POINTER(_c_f64), _CP], _I),
     "pfa_reserve": ([_I, _c_i64, _c_i64, _I, _CP], _I),
+    "pfa_coni_mu0": ([_I, _P64, _c_i64, _P64, _I, _P32, _P64, _P64, _P64, _CP], _I),
     "pfa_device_cus": ([_I], _I),
     "pfa_device_clock_khz": ([_I], _I),
     "pfa_arch": ([], _CP),
