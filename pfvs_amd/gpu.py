@@ -237,14 +237,20 @@ class _Batch:
             raise ValueError("pgeo: geometry index out of range")
         h_of = np.asarray(hs, dtype=np.int32)
         P = np.zeros((max(n_p, 1), width), dtype=np.int64)
-        rows = ps if isinstance(ps, np.ndarray) and ps.ndim == 2 else list(ps)
-        for i in range(n_p):
-            row = np.asarray(rows[i], dtype=np.int64).ravel()
-            tail = row[h_of[pgeo[i]]:]
-            if tail.size and np.any(tail):
+        if isinstance(ps, np.ndarray) and ps.ndim == 2 and len(ps) == n_p:
+            A = ps.astype(np.int64, copy=False)
+        else:
+            rows = [np.asarray(r, dtype=np.int64).ravel() for r in ps][:n_p]
+            A = np.zeros((n_p, max([len(r) for r in rows] + [1])), dtype=np.int64)
+            for i, r in enumerate(rows):
+                A[i, :len(r)] = r
+        if n_p:
+            # entries at or beyond a p-vector's own h11 must be 0
+            beyond = np.arange(A.shape[1])[None, :] >= h_of[pgeo][:, None]
+            if np.any((A != 0) & beyond):
                 raise ValueError("a p-vector is longer than its geometry's h11")
-            m = min(len(row), width)
-            P[i, :m] = row[:m]
+            m = min(A.shape[1], width)
+            P[:n_p, :m] = A[:, :m]
         flat = lambda parts: np.concatenate(parts) if parts else np.zeros(1, np.int64)
         return cls(h_of, np.asarray(Qs, np.int64), np.asarray(qms, np.int64),
                    np.asarray(lms, np.int64), flat(kaps), flat(mbs), P, pgeo, n_p, width)
