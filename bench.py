@@ -16,14 +16,15 @@ serving (a Poisson stream through the continuous-batching server) print
 the same line for those workloads.  The headline line also carries
 coniPFVs/s on a workload that has PFVs (the Manwe scan).
 
-Roofline of the search kernel (k_search, batched form): VALU issue.  Its
-algorithmic work W per launch is counted in lane-ops on the host path,
-which visits the same nodes (asserted):
+Roofline of the search stage (the breadth-first split's k_expand launches
+and the persistent k_search, batched form): VALU issue.  Its algorithmic
+work W per pass is counted in lane-ops on the host path, which visits the
+same nodes (asserted):
   W = sum_nodes [4k + 4k [i >= m0_level] + 4k r + 24]
     + sum_cand [12 + r (4 + 5 s)] + sum_leaf [4 d (d + 1) / 2 + 8]
 (k = d - 1 - i coordinates set above a node at level i, r = 1 where an H
 row leads at column i, s = binary-gcd iterations), and divided by the
-kernel's duration from HIP events on the stream it runs on; the peak is
+stage's duration from HIP events on the stream it runs on; the peak is
 CUs x 64 lanes x peak clock.
 """
 
@@ -132,16 +133,17 @@ def search_work(gpu, geoms, pf, pgeo):
     return r["stats"]["work"], r["stats"]["nodes"], threads
 
 
-TRAFFIC_FILE = os.path.join("profiles", "k_batched_h20_hbm_traffic.md")
+TRAFFIC_FILE = os.path.join("profiles", "search_h20_hbm_traffic.md")
 
 
-def measured_traffic(kernel="k_batched"):
-    """HBM bytes per launch of `kernel` as recorded in the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by
-    a copy kernel of known size), or None.  Not measured by this run."""
+def measured_traffic(stage="search"):
+    """HBM bytes of one pass's search stage (the split's k_expand launches
+    and k_search) as recorded in the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size), or
+    None.  Not measured by this run."""
     try:
         for line in open(os.path.join(ROOT, TRAFFIC_FILE)):
-            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+            if line.startswith(f"traffic_bytes_per_launch {stage}:"):
                 return float(line.split(":")[1])
     except OSError:
         pass
@@ -380,13 +382,14 @@ def main():
     cus, clk_khz = gpu.device_facts(0)
     peak = cus * 64 * clk_khz * 1e3                 # lane-ops/s
     achieved = work / (ms_search * 1e-3)
-    traffic = measured_traffic("k_batched") if args.workload == "headline" else None
+    traffic = measured_traffic("search") if args.workload == "headline" else None
     out["roofline"] = {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
                        "unit": "T lane-op/s", "frac": achieved / peak,
                        "traffic": traffic,
                        "traffic_source": (f"{TRAFFIC_FILE} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE;"
                                           " recorded, not measured by this run)") if traffic else None,
-                       "kernel": "k_search<batched>", "work_per_launch": work,
+                       "kernel": "search stage: k_expand (breadth-first split) + k_search<batched>",
+                       "work_per_launch": work,
                        "ms_per_launch": ms_search,
                        "peak_basis": f"{cus} CUs x 64 lanes x {clk_khz / 1e6:.2f} GHz"}
     if not args.no_plain:
