@@ -138,7 +138,8 @@ typedef struct {
     int64_t *M, *Kn;      /* n x pfg_max_h11() */
     int64_t *q, *pidx;
     int8_t *pstat;        /* n_p: 0 done, 1 exact host path, < 0 kernel status */
-    int32_t *pcode;       /* n_p: detailed code (preparation stage codes) */
+    int32_t *pcode;       /* n_p: detailed code (preparation stage codes; 1: the search
+                             tree was proven empty before the lattice, one node) */
     int64_t *nexp;        /* n: (g, K0) pairs of the point (filter), -1 unknown */
     int8_t *nonsing;      /* n: 1 det != 0, 0 det == 0, 2 undecided */
     double stats[PFA_ST_N];

@@ -176,6 +176,101 @@ PFA_HD static inline pfa_i128 pfa_tdiv128(pfa_i128 a, pfa_i128 b)
     return a / b;
 }
 
+// 256-bit two's complement integers (products of two int128), for exact
+// determinants whose entries fit int128 but whose products do not.
+struct pfa_w256 {
+    pfa_u128 lo, hi;
+};
+
+PFA_HD static inline pfa_w256 pfa_umul256(pfa_u128 a, pfa_u128 b)
+{
+    const uint64_t a0 = (uint64_t)a, a1 = (uint64_t)(a >> 64), b0 = (uint64_t)b, b1 = (uint64_t)(b >> 64);
+    const pfa_u128 p00 = (pfa_u128)a0 * b0, p01 = (pfa_u128)a0 * b1, p10 = (pfa_u128)a1 * b0,
+                   p11 = (pfa_u128)a1 * b1;
+    const pfa_u128 mid = (p00 >> 64) + (uint64_t)p01 + (uint64_t)p10;      // < 3 2^64
+    pfa_w256 r;
+    r.lo = (mid << 64) | (uint64_t)p00;
+    r.hi = p11 + (p01 >> 64) + (p10 >> 64) + (mid >> 64);
+    return r;
+}
+
+PFA_HD static inline pfa_w256 pfa_neg256(pfa_w256 x)
+{
+    pfa_w256 r;
+    r.lo = ~x.lo + 1;
+    r.hi = ~x.hi + (r.lo == 0 ? 1 : 0);
+    return r;
+}
+
+PFA_HD static inline pfa_w256 pfa_smul256(pfa_i128 a, pfa_i128 b)
+{
+    const pfa_w256 r = pfa_umul256(pfa_abs128(a), pfa_abs128(b));
+    return (a < 0) != (b < 0) ? pfa_neg256(r) : r;
+}
+
+PFA_HD static inline pfa_w256 pfa_sub256(pfa_w256 x, pfa_w256 y)
+{
+    pfa_w256 r;
+    r.lo = x.lo - y.lo;
+    r.hi = x.hi - y.hi - (x.lo < y.lo ? 1 : 0);
+    return r;
+}
+
+// x >> s, arithmetic, 0 <= s < 256
+PFA_HD static inline pfa_w256 pfa_sar256(pfa_w256 x, int s)
+{
+    if (s == 0) return x;
+    pfa_w256 r;
+    if (s < 128) {
+        r.lo = (x.lo >> s) | (x.hi << (128 - s));
+        r.hi = (pfa_u128)((pfa_i128)x.hi >> s);
+    } else {
+        r.lo = (pfa_u128)((pfa_i128)x.hi >> (s - 128));
+        r.hi = (pfa_i128)x.hi < 0 ? ~(pfa_u128)0 : (pfa_u128)0;
+    }
+    return r;
+}
+
+// x < y, both nonnegative
+PFA_HD static inline int pfa_ult256(pfa_w256 x, pfa_w256 y)
+{
+    return x.hi < y.hi || (x.hi == y.hi && x.lo < y.lo);
+}
+
+// A divisor b != 0 prepared for exact divisions: b = 2^s o with o odd, and
+// o's inverse modulo 2^128 (Newton's iteration x <- x (2 - o x) doubles
+// the correct low bits, from 3 since o o = 1 mod 8).
+struct pfa_exdiv {
+    pfa_i128 b;
+    pfa_u128 inv;
+    int s;
+};
+
+PFA_HD static inline pfa_exdiv pfa_exdiv_prep(pfa_i128 b)
+{
+    pfa_exdiv D;
+    D.b = b;
+    D.s = pfa_ctz128((pfa_u128)b);
+    const pfa_u128 o = (pfa_u128)(b >> D.s);
+    pfa_u128 x = o;
+    for (int k = 0; k < 6; ++k) x *= (pfa_u128)2 - o * x;
+    D.inv = x;
+    return D;
+}
+
+// The int128 quotient N / b when b divides N and the quotient fits int128
+// (returns 1), else 0: (N >> s) times o's inverse modulo 2^128 is the
+// quotient modulo 2^128, and q b == N proves it (if the true quotient does
+// not fit int128, no int128 q satisfies q b == N).
+PFA_HD static inline int pfa_exdiv256(pfa_w256 N, const pfa_exdiv &D, pfa_i128 *q)
+{
+    const pfa_w256 t = pfa_sar256(N, D.s);
+    const pfa_i128 r = (pfa_i128)(t.lo * D.inv);
+    const pfa_w256 back = pfa_smul256(r, D.b);
+    *q = r;
+    return back.lo == N.lo && back.hi == N.hi;
+}
+
 PFA_HD static inline pfa_i128 pfa_tmod128(pfa_i128 a, pfa_i128 b)
 {
     if (pfa_div_in64(a, b)) return (pfa_i128)((int64_t)a % (int64_t)b);

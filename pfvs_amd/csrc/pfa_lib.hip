@@ -82,7 +82,8 @@ static RecL rec_layout(int D)
 }
 
 // header (int32 view): [0] status [1] d [20] m0_level [21] leafg [22] nL
-// [23] strict [24] h [25] code [26] geo; (int64 view) [1] qmax [2] Q
+// [23] strict [24] h [25] code [26] geo [27] root proven empty (no factor,
+// rows or Binter written); (int64 view) [1] qmax [2] Q
 // [3] linmin [4..8] Qe QGmul phi slack Kerr (double) [9] hmask
 // [14] max |Binter| [15] max |ZB| [16] max |mat| [17] max |H| of the
 // pruning rows (int64 bounds for the search's exact prefix sums)
@@ -101,6 +102,7 @@ PFA_HD static inline PfaProb rec_view_as(const int64_t *r, const RecL &L)
     P.leafg = ri[21];
     P.nL = ri[22];
     P.strict = ri[23];
+    P.root_empty = ri[27];
     P.qmax = r64[1];
     P.Q = r64[2];
     P.linmin = r64[3];
@@ -196,6 +198,19 @@ PFA_HD static inline void prep_one(const PfaTeam &T, char *ws_hot, char *ws_cold
     }
     PfaWS W;
     pfa_ws_carve(W, ws_hot, ws_cold, h);
+    if (pfa_empty_certificate(T, W, G.kappa + G.kofs[geo], p, G.qmax[geo], G.linmin[geo])) {
+        // the search would enter the root and find no candidate: one node
+        PFA_SYNC();
+        if (T.lane == 0) {
+            ri[20] = d - 1;
+            ri[25] = 1;
+            ri[27] = 1;
+            rec_set_consts(rec, d, rec[1], 0.0);
+            ri[0] = PFA_OK;
+        }
+        PFA_SYNC();
+        return;
+    }
     int st = pfa_build_lattice(T, W, G.kappa + G.kofs[geo], G.Mbasis + G.mofs[geo], p);
     if (st) {
         PFA_SYNC();
@@ -421,10 +436,9 @@ __global__ void k_roots(int64_t n, int64_t *recs, RecL L, SplitArgs S, EntL E, C
     PfaProb P = rec_view(rec, L);
     if (pfa_trivially_empty(P)) return;
     PfaNode nd;
-    pfa_root(P, nd);
     int32_t c[PFA_DMAX];
     for (int j = 0; j < PFA_DMAX; ++j) c[j] = 0;
-    if (pfa_enter(P, c, nd)) { rec_raise_status(rec, PFA_I32); return; }
+    if (pfa_root_enter(P, c, nd)) { rec_raise_status(rec, PFA_I32); return; }
     atomicAdd(&C.c[0], 1ULL);
     if (nd.hi < nd.cur) return;                          // no candidate: a node without children
     unsigned long long t = atomicAdd(&S.st[0], 1ULL);
@@ -630,9 +644,8 @@ __global__ __launch_bounds__(64) void k_search(BatchArgs a)
                         PfaProb P = rec_view(rec, a.L);
                         if (!pfa_trivially_empty(P)) {
                             PfaNode nd;
-                            pfa_root(P, nd);
                             for (int j = 0; j < PFA_DMAX; ++j) c[j] = 0;
-                            if (pfa_enter(P, c, nd)) {
+                            if (pfa_root_enter(P, c, nd)) {
                                 rec_raise_status(rec, PFA_I32);
                             } else {
                                 char *e = ent(0);

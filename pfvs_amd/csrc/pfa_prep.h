@@ -712,6 +712,100 @@ PFA_HD static inline int pfa_hnf(const PfaTeam &T, const PfaWS &W, const int64_t
 }
 
 // --------------------------------------------------- lattice build
+// ---------------------------------------------------------------------
+// A certificate that a direction's tree is empty, before the lattice is
+// built.  Every point has M0 >= linmin and q(M) = -M^T Z M <= qmax with M
+// in the real subspace V = {M : w.M = 0}, w = Z p, where q is positive
+// definite; so M0^2 <= r q(M) <= r qmax with r = max over V of M0^2 / q.
+// With the bordered matrix of -Z and w ordered (j, border, the other
+// coordinates, 0) for a j != 0 with w_j != 0 and Z_jj != 0, its leading
+// principal minors P_k of orders k >= 3 are the bordered minors of q with
+// one constraint, so q is positive definite on V iff every such P_k is
+// negative; then 1 / r = P_{h+1} / P_h (the Schur complement of the
+// coordinate M0, ordered last: the minimum of q over V with M0 = 1).  The
+// minors are the pivots of a fraction-free (Bareiss) elimination, exact in
+// int128; a value beyond it gives no certificate.
+//
+// The tree is declared empty when 5 linmin^2 |P_{h+1}| > 9 (qmax + 1) |P_h|,
+// i.e. linmin^2 > 1.8 r (qmax + 1).  In the cut-aware basis M0 = g c_{d-1},
+// the root's coordinate, and the search's root interval is |U_{d-1,d-1} c|
+// <= R with R^2 = (Qe + slack + Kerr Qe)(1 + Kerr)^2, Qe = qmax / (1 - phi)
+// (1 + 8u), slack <= 1e-4 + 1e-9 (Qe + 1); since q <= (1 + phi) |U c|^2,
+// g^2 / U_{d-1,d-1}^2 <= (1 + phi) r, so the interval reaches at most
+// M0^2 <= r (1 + phi) / (1 - phi) (qmax (1 + 1e-8) + 2e-4) < 1.7 r (qmax + 1)
+// for phi < 0.25 (a larger phi sends the problem to the exact host path),
+// and the interval's 1e-9 rounding allowance stays inside the margin.  The
+// search would enter the root, find no candidate and stop: one node, as
+// the record's root-empty flag counts it.  Lanes own rows; the scratch is
+// the cold workspace (before the lattice is built: Z, then W2 .. mat as
+// (h+1)^2 int128, which fits for h >= 3) and hv.  Returns 1 (empty) or 0.
+PFA_HD static inline int pfa_empty_certificate(const PfaTeam &T, const PfaWS &W, const int64_t *kappa,
+                                               const int64_t *p, int64_t qmax, int64_t linmin)
+{
+    const int h = W.h, n = h + 1;
+    if (linmin <= 0 || qmax < 0 || h < 3) return 0;
+    int64_t *Zt = W.Z;                                   // (h, h)
+    pfa_i128 *w = W.hv;                                  // (h)
+    pfa_i128 *Mx = (pfa_i128 *)W.W2;                     // (n, n) over W2 .. mat
+    int bad = 0;
+    pfa_team_clear(T, W.flag);
+    for (int e = T.lane; e < h * h; e += T.n) {
+        pfa_i128 s = 0;
+        for (int c = 1; c < h; ++c)
+            if (pfa_add_ovf(s, (pfa_i128)kappa[(size_t)e * h + c] * p[c], &s)) bad = 1;
+        if (!pfa_fits64(s)) bad = 1;
+        Zt[e] = (int64_t)s;
+    }
+    if (pfa_team_any(T, W.flag, bad)) return 0;
+    for (int a = T.lane; a < h; a += T.n) {
+        pfa_i128 s = 0;
+        for (int b = 1; b < h; ++b)
+            if (pfa_add_ovf(s, (pfa_i128)Zt[a * h + b] * p[b], &s)) bad = 1;
+        w[a] = s;
+    }
+    if (pfa_team_any(T, W.flag, bad)) return 0;
+    // j: the smallest |w_j| != 0 with j != 0 and Z_jj != 0 (the same on every lane)
+    int j = -1;
+    for (int i = 1; i < h; ++i)
+        if (w[i] != 0 && Zt[i * h + i] != 0 && (j < 0 || pfa_abs128(w[i]) < pfa_abs128(w[j]))) j = i;
+    if (j < 0) return 0;
+    // position r of the ordering -> coordinate (h: the border)
+    auto ord = [&](int r) { return r == 0 ? j : r == 1 ? h : r == h ? 0 : (r - 1 < j ? r - 1 : r); };
+    for (int r = T.lane; r < n; r += T.n) {
+        const int a = ord(r);
+        for (int c = 0; c < n; ++c) {
+            const int b = ord(c);
+            Mx[r * n + c] = a == h ? (b == h ? (pfa_i128)0 : w[b]) : (b == h ? w[a] : -(pfa_i128)Zt[a * h + b]);
+        }
+    }
+    PFA_SYNC();
+    pfa_i128 prev = 1, Ph = 0;
+    for (int k = 0; k < n - 1; ++k) {
+        const pfa_i128 piv = Mx[k * n + k];              // the leading minor of order k + 1
+        if (piv == 0 || (k >= 2 && piv > 0)) return 0;   // not (provably) positive definite on V
+        if (k + 1 == h) Ph = piv;
+        // entries (r, c) > k: (piv M_rc - M_rk M_kc) / prev, exact; the
+        // products need 256 bits, the quotients (minors) must fit int128
+        const pfa_exdiv D = pfa_exdiv_prep(prev);
+        const int m = n - 1 - k;
+        for (int e = T.lane; e < m * m; e += T.n) {
+            const int r = k + 1 + e / m, c = k + 1 + e % m;
+            const pfa_w256 N = pfa_sub256(pfa_smul256(piv, Mx[r * n + c]), pfa_smul256(Mx[r * n + k], Mx[k * n + c]));
+            pfa_i128 q;
+            if (!pfa_exdiv256(N, D, &q)) bad = 1;
+            Mx[r * n + c] = q;
+        }
+        if (pfa_team_any(T, W.flag, bad)) return 0;
+        prev = piv;
+    }
+    const pfa_i128 Pn = Mx[(n - 1) * n + (n - 1)];
+    if (!(Pn < 0 && Ph < 0)) return 0;
+    // 5 linmin^2 |Pn| > 9 (qmax + 1) |Ph|
+    pfa_i128 a, b;
+    if (pfa_mul_ovf((pfa_i128)5 * linmin, (pfa_i128)linmin, &a) || pfa_mul_ovf(9, (pfa_i128)qmax + 1, &b)) return 0;
+    return pfa_ult256(pfa_smul256(b, -Ph), pfa_smul256(a, -Pn));
+}
+
 // Builds Z, Binter (W.Bi), ZB, mat for p (length h, p[0] = 0).  Returns 0
 // or 10 * rc - stage (stage 1 orthogonal, 2 LLL, 4 cut-aware basis, 0 a
 // product overflow), as pfl_build's status (pfv_lattice.h:604-680).

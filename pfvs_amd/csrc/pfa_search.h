@@ -36,6 +36,7 @@ enum {
 // A problem as the search sees it (pointers into the device/host record).
 struct PfaProb {
     int d, m0_level, strict, leafg;   // leafg: 1 = recompute g at leaves
+    int root_empty;                   // 1: proven before the factor that the root has no candidate
     int64_t qmax, Q, linmin;
     double Qe, QGmul, phi, slack, Kerr;
     uint64_t hmask;                   // bit i: a GCD row becomes known at level i
@@ -345,6 +346,16 @@ PFA_HD static inline void pfa_root(const PfaProb &P, PfaNode &nd)
     nd.cmax = 0;
 }
 
+// The root node, entered: its interval (or, for a problem whose root was
+// proven empty at preparation, whose factor and rows are not built, the
+// empty interval).  Returns 0 or PFA_I32; c[] must be all zero.
+PFA_HD static inline int pfa_root_enter(const PfaProb &P, const int32_t *c, PfaNode &nd)
+{
+    pfa_root(P, nd);
+    if (P.root_empty) return 0;
+    return pfa_enter(P, c, nd);
+}
+
 // 1 if the problem has no node at all (M0 identically 0 below linmin, or
 // qmax < 0), matching fp_kernel.h:391 and 435.
 PFA_HD static inline int pfa_trivially_empty(const PfaProb &P)
@@ -373,8 +384,7 @@ PFA_HD static inline int pfa_dfs_t(const PfaProb &P, PfaNode *st, int32_t *c, Pf
         }
     };
     int i = d - 1;
-    pfa_root(P, st[i]);
-    if (pfa_enter(P, c, st[i])) return PFA_I32;
+    if (pfa_root_enter(P, c, st[i])) return PFA_I32;
     entered(i);
     while (i < d) {
         PfaNode &nd = st[i];
