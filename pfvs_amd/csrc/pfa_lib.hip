@@ -408,6 +408,7 @@ struct SplitArgs {
 struct PointArgs {
     int32_t *pidx;
     int64_t *q;
+    uint64_t *g;                  // the leaf's gcd(Kn[1:]) (pfa_leaf), or null
     int32_t *c;                   // cap x D
     unsigned long long cap;
     unsigned long long *count;
@@ -463,11 +464,12 @@ __global__ void k_lane(int64_t n, int64_t *recs, RecL L, PointArgs Pt, int D, Co
     PfaNode st[PFA_DMAX];
     int32_t c[PFA_DMAX];
     PfaCounts cnt{0, 0, 0};
-    auto emit = [&](const int32_t *cc, int64_t q) -> int {
+    auto emit = [&](const int32_t *cc, int64_t q, uint64_t g) -> int {
         unsigned long long k = atomicAdd(Pt.count, 1ULL);
         if (k >= Pt.cap) { *Pt.overflow = 1; return 0; }
         Pt.pidx[k] = (int32_t)ip;
         Pt.q[k] = q;
+        if (Pt.g) Pt.g[k] = g;
         for (int j = 0; j < D; ++j) Pt.c[k * D + j] = j < P.d ? cc[j] : 0;
         return 0;
     };
@@ -776,6 +778,7 @@ __global__ __launch_bounds__(64) void k_search(BatchArgs a)
             ((PfaNode *)ent(top - 1 - F))->cur = part_new_cur;
         int has_child = 0, emit = 0;
         int64_t q = 0;
+        uint64_t lg = 0;
         PfaNode ch;
         // one p-vector for every candidate of the step (the usual case): its
         // header through the scalar cache
@@ -796,7 +799,7 @@ __global__ __launch_bounds__(64) void k_search(BatchArgs a)
                     else has_child = ch.hi >= ch.cur;
                 } else {
                     n_leaf++;
-                    int lr = pfa_leaf(P, ch, c, &q);
+                    int lr = pfa_leaf(P, ch, c, &q, &lg);
                     if (lr == PFA_CAND_OVF) rec_raise_status((int64_t *)rec, PFA_HOST);
                     else emit = lr == 1;
                 }
@@ -821,6 +824,7 @@ __global__ __launch_bounds__(64) void k_search(BatchArgs a)
                 if (k < a.Pt.cap) {
                     a.Pt.pidx[k] = pidx;
                     a.Pt.q[k] = q;
+                    if (a.Pt.g) a.Pt.g[k] = lg;
                     for (int t = 0; t < D; ++t) a.Pt.c[k * D + t] = c[t];
                 } else {
                     *a.Pt.overflow = 1;
@@ -1004,6 +1008,7 @@ __global__ __launch_bounds__(64) void k_expand(int t, int64_t *recs, RecL L, Ent
             }
             int has_child = 0, emit = 0;
             int64_t q = 0;
+            uint64_t lg = 0;
             PfaNode ch;
             const int32_t p0 = __builtin_amdgcn_readfirstlane(j >= 0 ? pidx : -1);
             const bool uni = p0 >= 0 && !__ballot(j >= 0 && pidx != p0);
@@ -1022,7 +1027,7 @@ __global__ __launch_bounds__(64) void k_expand(int t, int64_t *recs, RecL L, Ent
                         else has_child = ch.hi >= ch.cur;
                     } else {
                         n_leaf++;
-                        int lr = pfa_leaf(P, ch, c, &q);
+                        int lr = pfa_leaf(P, ch, c, &q, &lg);
                         if (lr == PFA_CAND_OVF) rec_raise_status((int64_t *)rec, PFA_HOST);
                         else emit = lr == 1;
                     }
@@ -1045,6 +1050,7 @@ __global__ __launch_bounds__(64) void k_expand(int t, int64_t *recs, RecL L, Ent
                     if (k < Pt.cap) {
                         Pt.pidx[k] = pidx;
                         Pt.q[k] = q;
+                        if (Pt.g) Pt.g[k] = lg;
                         for (int tt = 0; tt < D; ++tt) Pt.c[k * D + tt] = c[tt];
                     } else {
                         *Pt.overflow = 1;
@@ -1201,7 +1207,10 @@ PFA_HD static inline int kappa_M(const int64_t *kappa, int h, const int64_t *M, 
 // Kn = ZB c and M = Binter c (only M[0] unless the rows are kept) in
 // int128, checked into int64 (a failure marks the p-vector for the exact
 // host path); K_gcd = gcd(Kn[1:]) and the number n_exp of (g, K0)
-// expansions (0: the K-integrality filter drops the point).  Kept rows are
+// expansions (0: the K-integrality filter drops the point).  Without kept
+// rows, K_gcd is the search leaf's exact gcd of the HNF rows of ZB[1:] at c
+// (the same integer) and only Kn[0] is formed, when a bound proves every
+// entry of Kn within int64.  Kept rows are
 // staged in LDS and stored as one contiguous block per wavefront; points
 // with n_exp > 0 are appended to the list of the det test (one atomic per
 // wavefront).
@@ -1240,6 +1249,8 @@ struct FiltArgs {
     int64_t n;                    // point slots
     const int32_t *pidx;          // < 0: unused slot
     const int64_t *q;
+    const uint64_t *g;            // per slot gcd(Kn[1:]) from the search's leaf (PFA_G_WIDE: not
+                                  // known), or null
     const int32_t *c;             // D int32 per point
     const int64_t *Min;           // instead of c: M rows given (stride MH), all tested
     int D, MH, hl;                // MH = row stride of M, Kn (= hl)
@@ -1389,8 +1400,12 @@ __global__ __launch_bounds__(64) void k_post(FiltArgs a)
         const int64_t zmax = valid ? rec[15] : 0, bmax = valid ? rec[14] : 0;
         const bool fz = zmax < ((int64_t)1 << 31) && (double)zmax * csum < PFA_TWO53;
         const bool fb = bmax < ((int64_t)1 << 31) && (double)bmax * csum < PFA_TWO53;
-        // Kn (staged when kept)
-        for (int i = 0; i < a.MH; ++i) {
+        // Kn (staged when kept).  Unless the rows are kept, gcd(Kn[1:]) is
+        // the search leaf's g when every Kn entry provably fits int64
+        // (max|ZB| sum|c| < 2^62): then only Kn[0] is computed here.
+        const bool gk = !store && a.g && valid && (double)zmax * csum < 4.0e18 && a.g[kp] != PFA_G_WIDE;
+        if (gk) kg = a.g[kp];
+        for (int i = 0; i < (gk ? 1 : a.MH); ++i) {
             int64_t v = 0;
             if (valid && i < h) {
                 v = uni && fz ? post_dot_uni<NMAX>(urec + a.L.ZB + (int64_t)i * du, cd)
@@ -2159,7 +2174,7 @@ enum {
     B_TOT, B_KEEPCNT, B_KEEPKP, B_KEEPNEXP, B_KEEPMETA, B_PREPORDER,
     B_SPR, B_SPKV, B_SPROFS, B_SPKVOFS, B_SPINTS,
     B_SV_PIDX, B_SV_Q, B_SV_M, B_SV_KN, B_SV_NEXP, B_SV_NS, B_SV_CNT, B_STATSAVE, B_STATOUT,
-    B_PCOUNT, B_F0, B_F1, B_SPLITST, B_BOX, B_BOXOUT,
+    B_PCOUNT, B_F0, B_F1, B_SPLITST, B_BOX, B_BOXOUT, B_PT_G,
     B_NBUF
 };
 
@@ -2934,6 +2949,7 @@ struct SearchRun {
     int passes = 0, launches = 0, attempts = 0, nwaves = 0, stack_cap = 0;
     int32_t *d_pidx = nullptr;
     int64_t *d_q = nullptr;
+    uint64_t *d_g = nullptr;
     int32_t *d_c = nullptr;
     unsigned long long *d_cnt = nullptr;     // nodes, cand, leaf, steps, wait, cycles, donated, fetches
 };
@@ -2986,7 +3002,8 @@ static void search_device(DevCtx *C, hipStream_t s, int64_t *d_recs, const RecL 
         R.d_pidx = (int32_t *)dev_get(C, B_PT_PIDX, 4 * pcap, errbuf);
         R.d_q = (int64_t *)dev_get(C, B_PT_Q, 8 * pcap, errbuf);
         R.d_c = (int32_t *)dev_get(C, B_PT_C, 4 * pcap * D, errbuf);
-        PointArgs Pt{R.d_pidx, R.d_q, R.d_c, pcap, d_ctl + 16 * 6, d_ovf};
+        R.d_g = (uint64_t *)dev_get(C, B_PT_G, 8 * pcap, errbuf);
+        PointArgs Pt{R.d_pidx, R.d_q, R.d_g, R.d_c, pcap, d_ctl + 16 * 6, d_ovf};
         CountArgs Cn{R.d_cnt};
         R.passes = 1;
         R.launches = (n_p > 0 || !(plain || lane)) ? 1 : 0;
@@ -3194,6 +3211,7 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
     Fa.n = n;
     Fa.pidx = R.d_pidx;
     Fa.q = R.d_q;
+    Fa.g = R.d_g;
     Fa.c = R.d_c;
     Fa.Min = nullptr;
     Fa.D = D;
@@ -3405,7 +3423,7 @@ static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
             HostPts &hp = tp[t];
             size_t mark = hp.q.size();
             int64_t found = 0;
-            auto emit = [&](const int32_t *cc, int64_t q) -> int {
+            auto emit = [&](const int32_t *cc, int64_t q, uint64_t) -> int {
                 found++;
                 if (!store) return 0;
                 hp.pidx.push_back((int32_t)ip);
@@ -3859,7 +3877,7 @@ static int coni_mu0_one(int h, const int64_t *kappa, const int64_t *p_r, int64_t
     int32_t c[PFA_DMAX], y[PFA_DMAX];
     PfaCounts cnt{0, 0, 0};
     int64_t qmin = INT64_MAX;
-    auto emit = [&](const int32_t *cc, int64_t q) -> int {
+    auto emit = [&](const int32_t *cc, int64_t q, uint64_t) -> int {
         if (q > 0 && q < qmin) {
             qmin = q;
             for (int j = 0; j < n; ++j) y[j] = cc[j];
@@ -3919,6 +3937,7 @@ extern "C" int pfa_reserve(int device, int64_t n_p, int64_t n_points, int h_max,
         C->pts_reserved = std::max<unsigned long long>(C->pts_reserved, pc);
         dev_get(C, B_PT_PIDX, 4 * pc, errbuf);
         dev_get(C, B_PT_Q, 8 * pc, errbuf);
+        dev_get(C, B_PT_G, 8 * pc, errbuf);
         dev_get(C, B_PT_C, 4 * pc * D, errbuf);
         const unsigned long long ring_cap =
             (unsigned long long)std::max<int64_t>(2 * n_p + 64, 1 << 20) + 2 * 64ULL * 4096ULL;
@@ -4407,7 +4426,7 @@ extern "C" int pfa_kernel_batch(const pfa_kernel_input *in, pfa_kernel_output *o
                 PfaProb P = rec_view(r, L);
                 PfaCounts cnt{0, 0, 0};
                 size_t mark = qs.size();
-                auto emit = [&](const int32_t *cc, int64_t q) -> int {
+                auto emit = [&](const int32_t *cc, int64_t q, uint64_t) -> int {
                     pidx.push_back((int32_t)k);
                     qs.push_back(q);
                     for (int j = 0; j < D; ++j) cs.push_back(j < P.d ? cc[j] : 0);

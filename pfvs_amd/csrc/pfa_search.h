@@ -307,10 +307,16 @@ PFA_HD static inline pfa_u128 pfa_leaf_gcd(const PfaProb &P, const int32_t *c, p
     return g;
 }
 
+// g of a kept leaf when it does not fit 64 bits
+#define PFA_G_WIDE UINT64_MAX
+
 // Leaf decision for the vector c whose last step produced ch (ch.i == -1):
-// 1 keep (q in *q_out), 0 drop, PFA_CAND_OVF undecidable here.
+// 1 keep (q in *q_out), 0 drop, PFA_CAND_OVF undecidable here.  *g_out:
+// the exact gcd of the pruning and leaf rows at c -- the rows of the HNF of
+// ZB[1:] (or ZB[1:] itself), so gcd(Kn[1:]) with Kn = ZB c, 0 when every
+// row vanishes -- or PFA_G_WIDE beyond 64 bits.
 PFA_HD static inline int pfa_leaf(const PfaProb &P, const PfaNode &ch, const int32_t *c,
-                                  int64_t *q_out)
+                                  int64_t *q_out, uint64_t *g_out)
 {
     pfa_i128 q = ch.qp;
     if (q > (pfa_i128)P.qmax) return 0;
@@ -327,6 +333,7 @@ PFA_HD static inline int pfa_leaf(const PfaProb &P, const PfaNode &ch, const int
         }
     }
     *q_out = (int64_t)q;
+    *g_out = (g >> 64) == 0 ? (uint64_t)g : PFA_G_WIDE;
     return 1;
 }
 
@@ -364,9 +371,9 @@ PFA_HD static inline int pfa_trivially_empty(const PfaProb &P)
 }
 
 // The whole search of one problem, depth first, in the reference's order:
-// the plain form (one thread per problem) and the host path.  emit(c, q)
-// returns nonzero to stop (-2).  Returns a status.  With CNT the terms of
-// the algorithmic work are added to *wk.
+// the plain form (one thread per problem) and the host path.  emit(c, q, g)
+// (g as pfa_leaf's) returns nonzero to stop (-2).  Returns a status.  With
+// CNT the terms of the algorithmic work are added to *wk.
 template <bool CNT, class Emit>
 PFA_HD static inline int pfa_dfs_t(const PfaProb &P, PfaNode *st, int32_t *c, PfaCounts &cnt,
                                    Emit &emit, PfaWork *wk)
@@ -418,9 +425,10 @@ PFA_HD static inline int pfa_dfs_t(const PfaProb &P, PfaNode *st, int32_t *c, Pf
         cnt.leaf++;
         if (CNT) wk->sleaf += (unsigned long long)d * (d + 1) / 2;
         int64_t q;
-        int lr = pfa_leaf(P, ch, c, &q);
+        uint64_t g;
+        int lr = pfa_leaf(P, ch, c, &q, &g);
         if (lr == PFA_CAND_OVF) return PFA_HOST;
-        if (lr == 1 && emit(c, q)) return PFA_TOO_MANY;
+        if (lr == 1 && emit(c, q, g)) return PFA_TOO_MANY;
     }
     return 0;
 }
