@@ -43,7 +43,10 @@ struct PfaTeam {
 //       j)], j < k); outside the LLL the same region holds the scratch hv,
 //       hw (h int128), hpiv (h int) of the HNF and the unimodular
 //       completion, and lll_form's Gy (h int128) before its LLL starts;
-//   bb, mw, vec (h), flag.
+//   bb, mw, vec (h), flag;
+//   sp  per Gram-Schmidt row kk, the running value of bb[kk] before each
+//       of its kk terms (strict lower triangle, as mu): a row exchange
+//       then moves a row's values instead of recomputing them.
 // The part only O(h^3) products touch ("cold") is per-workgroup global
 // memory on the device: Z, W2, W3 (h*h int64), Binter, ZB (h*(h-1)), mat
 // ((h-1)^2).  On the host both parts are one buffer.
@@ -51,7 +54,7 @@ struct PfaWS {
     int h, d;
     int64_t *Z, *Bi, *ZB, *mat, *W1, *W2, *W3, *vec;
     pfa_i128 *GG, *hv, *hw;
-    double *mu, *bb, *mw;
+    double *mu, *bb, *mw, *sp;
     int *flag, *hpiv;
 };
 
@@ -100,6 +103,7 @@ PFA_HD static inline size_t pfa_ws_lds_bytes(int h)
     w += pfa_ws_mu_bytes(h);                             // mu / hv, hw, hpiv / Gy
     w += PFA_WS_ROUND((size_t)h * 8) * 3;                // bb, mw, vec
     w += 16;                                             // flag
+    w += PFA_WS_ROUND((size_t)h * (h - 1) / 2 * 8);       // sp
     return w;
 }
 
@@ -131,6 +135,7 @@ PFA_HD static inline void pfa_ws_carve(PfaWS &W, char *hot, char *cold, int h)
     W.mw = (double *)take(hot, (size_t)h * 8);
     W.vec = (int64_t *)take(hot, (size_t)h * 8);
     W.flag = (int *)take(hot, 16);
+    W.sp = (double *)take(hot, (size_t)h * (h - 1) / 2 * 8);
     W.Z = (int64_t *)take(cold, (size_t)h * h * 8);
     W.W2 = (int64_t *)take(cold, (size_t)h * h * 8);
     W.W3 = (int64_t *)take(cold, (size_t)h * h * 8);
@@ -307,7 +312,7 @@ PFA_HD static inline int pfa_unimodular_reduced(const PfaTeam &T, const int64_t 
 // bit-identical to the sequential host loop.
 PFA_HD static inline void pfa_gs_row(const PfaTeam &T, const PfaWS &W, int kk, const pfa_i128 *GG)
 {
-    double *mu = W.mu, *bb = W.bb;
+    double *mu = W.mu, *bb = W.bb, *sp = W.sp;
     const int rk = pfa_mi(kk, 0);
 #if defined(__HIP_DEVICE_COMPILE__)
     if (T.n > kk) {
@@ -318,7 +323,10 @@ PFA_HD static inline void pfa_gs_row(const PfaTeam &T, const PfaWS &W, int kk, c
             const double m = pfa_readlane_f64(r, l) / bb[l];
             if (j == l) mu[rk + l] = m;
             else if (j > l && j < kk) r -= mu[rj + l] * m * bb[l];
-            else if (j == kk) r -= m * m * bb[l];
+            else if (j == kk) {
+                sp[rk + l] = r;
+                r -= m * m * bb[l];
+            }
         }
         if (j == kk) bb[kk] = r;
         PFA_SYNC();
@@ -331,7 +339,10 @@ PFA_HD static inline void pfa_gs_row(const PfaTeam &T, const PfaWS &W, int kk, c
         mu[rk + j] = dd / bb[j];
     }
     double dd = (double)GG[pfa_gi(kk, kk)];
-    for (int l = 0; l < kk; ++l) dd -= mu[rk + l] * mu[rk + l] * bb[l];
+    for (int l = 0; l < kk; ++l) {
+        if (T.lane == 0) sp[rk + l] = dd;
+        dd -= mu[rk + l] * mu[rk + l] * bb[l];
+    }
     if (T.lane == 0) bb[kk] = dd;
     PFA_SYNC();
 }
@@ -454,10 +465,25 @@ PFA_HD static inline int pfa_lll_core(const PfaTeam &T, const PfaWS &W, int64_t 
                 GG[pfa_gi(k - 1, l)] = t;
             }
             PFA_SYNC();
-            k = k > 1 ? k - 1 : 1;
-            if (k == 1) pfa_gs_row(T, W, 0, GG);
-            pfa_gs_row(T, W, k - 1, GG);
-            pfa_gs_row(T, W, k, GG);
+            if (k >= 2) {
+                // rows below k-1 are unchanged, and the new row k-1 (the old
+                // row k's vector) has the old row k's first k-1 coefficients
+                // and running values, bit for bit (the same operations on
+                // the same operands); its b* is the old row's running value
+                // before its term k-1.  What the reference recomputes here
+                // is moved.
+                double *sp = W.sp;
+                const int ra = pfa_mi(k - 1, 0), rb = pfa_mi(k, 0);
+                for (int l = T.lane; l < k - 1; l += T.n) {
+                    mu[ra + l] = mu[rb + l];
+                    sp[ra + l] = sp[rb + l];
+                }
+                if (T.lane == 0) bb[k - 1] = sp[rb + k - 1];
+                k--;
+            } else {
+                pfa_gs_row(T, W, 0, GG);
+                pfa_gs_row(T, W, 1, GG);
+            }
             PFA_SYNC();
         } else {
             k++;
