@@ -1280,6 +1280,7 @@ struct FiltArgs {
     int64_t *modkp, *modnexp, *modM;   // list: slot, n_exp, M row (stride MH)
     unsigned long long *modcount;
     int64_t *keepkp, *keepnexp;   // points with n_exp > 0 (k_post -> k_filter)
+    int64_t *rawkp, *rawnexp, *rawmeta;   // k_post's per-block runs of them (capacity n)
     unsigned long long *pcount;   // per p-vector: lattice points (k_post; null: not counted)
     int64_t *keepmeta;            // their p-vector << 32 | geometry << 8 | h
     int n_geo;
@@ -1325,7 +1326,10 @@ __device__ static inline uint64_t wave_sum_u64(uint64_t v)
 // registers as doubles, and a row product runs as unrolled FMAs when
 // max|row| sum|c| < 2^53 and max|row| < 2^31 (every product and partial sum
 // is then an exact integer; the low word of each int64 entry holds it),
-// else as pfa_dot_c (int64 under its bound, int128 beyond).
+// else as pfa_dot_c (int64 under its bound, int128 beyond).  The fast form
+// reads all NMAX entries, unconditionally, so the loads issue together:
+// cd[t] = 0 for t >= d adds nothing, and the records buffer extends past
+// the last row.
 template <int NMAX>
 __device__ __forceinline__ int64_t post_dot(const int64_t *row, const double *cd, const int32_t *cc,
                                             int d, bool fast, int64_t rmax, int64_t cmax, int *ok)
@@ -1334,8 +1338,7 @@ __device__ __forceinline__ int64_t post_dot(const int64_t *row, const double *cd
         const int32_t *r32 = (const int32_t *)row;
         double s = 0.0;
 #pragma unroll
-        for (int t = 0; t < NMAX; ++t)
-            if (t < d) s = fma((double)r32[2 * t], cd[t], s);
+        for (int t = 0; t < NMAX; ++t) s = fma((double)r32[2 * t], cd[t], s);
         return (int64_t)s;
     }
     return pfa_dot_c(row, cc, d, rmax, cmax, ok);
@@ -1357,6 +1360,12 @@ __device__ __forceinline__ int64_t post_dot_uni(const int64_t *row, const double
     return (int64_t)s;
 }
 
+// Each block (one wavefront) takes a contiguous run of 64-slot groups, so
+// the per-p-vector point counts accumulate in a register while the run stays
+// on one p-vector (one atomic per change), and the kept points go to the
+// block's own stretch of the raw list, then to the shared list with one
+// atomic per block: thousands of wavefronts adding to a few counters
+// serialize in the memory-side atomic units.
 template <int NMAX>
 __global__ __launch_bounds__(64) void k_post(FiltArgs a)
 {
@@ -1365,7 +1374,14 @@ __global__ __launch_bounds__(64) void k_post(FiltArgs a)
     const bool store = a.M != nullptr;
     int32_t *cst = (int32_t *)(stage + (store ? 64 * a.MH : 0));
     unsigned long long t_pts = 0, t_kept = 0;
-    for (int64_t base = (int64_t)blockIdx.x * 64; base < a.n; base += (int64_t)gridDim.x * 64) {
+    const int64_t groups = (a.n + 63) / 64;
+    const int64_t g0 = groups * blockIdx.x / gridDim.x, g1 = groups * (blockIdx.x + 1) / gridDim.x;
+    const int64_t r0 = g0 * 64;                    // this block's stretch of the raw list
+    unsigned long long kc = 0;                     // entries in it
+    int32_t cp = -1;                               // p-vector of the running point count
+    unsigned long long cpn = 0;
+    for (int64_t grp = g0; grp < g1; ++grp) {
+        const int64_t base = grp * 64;
         const int64_t kp = base + lane;
         const int32_t pid = kp < a.n ? a.pidx[kp] : -1;
         const bool valid = pid >= 0;
@@ -1442,31 +1458,50 @@ __global__ __launch_bounds__(64) void k_post(FiltArgs a)
         if (a.nexp && valid) a.nexp[kp] = ne;
         if (a.nonsing && valid) a.nonsing[kp] = 0;
         if (a.pcount) {
-            // lattice points per p-vector: one atomic per wavefront when the
-            // wavefront's slots belong to one p-vector
-            const unsigned long long vm0 = __ballot(valid);
-            if (uni) {
-                if (lane == 0) atomicAdd(&a.pcount[p0], (unsigned long long)__popcll(vm0));
-            } else if (valid) {
-                atomicAdd(&a.pcount[pid], 1ULL);
+            // lattice points per p-vector: a running count, flushed with
+            // one atomic when the p-vector changes; a group's slots are
+            // taken one p-vector at a time (the first remaining lane's)
+            unsigned long long rest = __ballot(valid);
+            while (rest) {
+                const int32_t pq = __shfl(pid, (int)__builtin_ctzll(rest));
+                const unsigned long long same = __ballot(valid && pid == pq);
+                if (pq != cp) {
+                    if (lane == 0 && cpn) atomicAdd(&a.pcount[cp], cpn);
+                    cp = pq;
+                    cpn = 0;
+                }
+                cpn += __popcll(same);
+                rest &= ~same;
             }
         }
-        // append n_exp > 0 to the list
+        // n_exp > 0: to the block's stretch of the raw list (entry k <=
+        // slot - r0, so it stays inside the stretch)
         const unsigned long long km = __ballot(ne > 0);
         const int cnt = __popcll(km);
-        unsigned long long b0 = 0;
-        if (lane == 0 && cnt) b0 = atomicAdd(a.keepcount, (unsigned long long)cnt);
-        b0 = shfl_u64(b0, 0);
         if (ne > 0) {
-            const int off = __popcll(km & ((1ull << lane) - 1));
-            a.keepkp[b0 + off] = kp;
-            a.keepnexp[b0 + off] = ne;
-            a.keepmeta[b0 + off] = ((int64_t)a.pidx[kp] << 32) | ((int64_t)ri[26] << 8) | (int64_t)h;
+            const int64_t at = r0 + (int64_t)kc + __popcll(km & ((1ull << lane) - 1));
+            a.rawkp[at] = kp;
+            a.rawnexp[at] = ne;
+            a.rawmeta[at] = ((int64_t)a.pidx[kp] << 32) | ((int64_t)ri[26] << 8) | (int64_t)h;
         }
+        kc += cnt;
         const unsigned long long vm = __ballot(valid);
         if (lane == 0) {
             t_pts += __popcll(vm);
             t_kept += cnt;
+        }
+    }
+    if (a.pcount && lane == 0 && cpn) atomicAdd(&a.pcount[cp], cpn);
+    if (kc) {
+        // the stretch to the shared list: one atomic for the block
+        unsigned long long b0 = 0;
+        if (lane == 0) b0 = atomicAdd(a.keepcount, kc);
+        b0 = shfl_u64(b0, 0);
+        __syncthreads();                           // the stretch's stores, visible to every lane
+        for (unsigned long long k = lane; k < kc; k += 64) {
+            a.keepkp[b0 + k] = a.rawkp[r0 + k];
+            a.keepnexp[b0 + k] = a.rawnexp[r0 + k];
+            a.keepmeta[b0 + k] = a.rawmeta[r0 + k];
         }
     }
     if (a.tot && lane == 0) {
@@ -1727,12 +1762,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     if (row && t < n) acc = fma((double)br[t], cd, acc);
                 }
             } else {
+                // every entry read, unconditionally, so the loads issue
+                // together (cd = 0 adds nothing; a lane without a row reads
+                // its record's first words, and its sum is not used)
+                const int32_t *bq = row ? b32 : (const int32_t *)rec;
 #pragma unroll 4
                 for (int t = 0; t < NMAX; ++t) {
                     const int32_t ct = __shfl(cv, L.src0 + t);
                     const double cd = t < n ? (double)ct : 0.0;
                     csum += fabs(cd);
-                    if (r32 && t < n) acc = fma((double)b32[2 * t], cd, acc);
+                    acc = fma((double)bq[2 * t], cd, acc);
                 }
             }
             const bool fast = r32 && (double)bmax * csum < PFA_TWO53;
@@ -2174,7 +2213,7 @@ enum {
     B_TOT, B_KEEPCNT, B_KEEPKP, B_KEEPNEXP, B_KEEPMETA, B_PREPORDER,
     B_SPR, B_SPKV, B_SPROFS, B_SPKVOFS, B_SPINTS,
     B_SV_PIDX, B_SV_Q, B_SV_M, B_SV_KN, B_SV_NEXP, B_SV_NS, B_SV_CNT, B_STATSAVE, B_STATOUT,
-    B_PCOUNT, B_F0, B_F1, B_SPLITST, B_BOX, B_BOXOUT, B_PT_G,
+    B_PCOUNT, B_F0, B_F1, B_SPLITST, B_BOX, B_BOXOUT, B_PT_G, B_KEEPRAW,
     B_NBUF
 };
 
@@ -2604,8 +2643,12 @@ static FiltLaunch filter_setup(DevCtx *C, hipStream_t s, const pfa_run_input *gi
         a.keepkp = (int64_t *)dev_get(C, B_KEEPKP, 8 * a.n, errbuf);
         a.keepnexp = (int64_t *)dev_get(C, B_KEEPNEXP, 8 * a.n, errbuf);
         a.keepmeta = (int64_t *)dev_get(C, B_KEEPMETA, 8 * a.n, errbuf);
+        a.rawkp = (int64_t *)dev_get(C, B_KEEPRAW, 24 * a.n, errbuf);
+        a.rawnexp = a.rawkp + a.n;
+        a.rawmeta = a.rawnexp + a.n;
         a.keepcount = F.d_kc;
-        F.pgrid = (int)std::min<int64_t>((a.n + 63) / 64, 65536);
+        // contiguous runs of 64-slot groups, 16 wavefronts per CU
+        F.pgrid = (int)std::max<int64_t>(1, std::min<int64_t>((a.n + 63) / 64, (int64_t)std::max(C->cus, 1) * 16));
         F.plds = (a.M ? (size_t)64 * a.MH * 8 : 0) + (size_t)64 * a.D * 4;
     }
     const int R = 64 / a.hl;
@@ -3960,6 +4003,7 @@ extern "C" int pfa_reserve(int device, int64_t n_p, int64_t n_points, int h_max,
         dev_get(C, B_KEEPKP, 8 * n1, errbuf);
         dev_get(C, B_KEEPNEXP, 8 * n1, errbuf);
         dev_get(C, B_KEEPMETA, 8 * n1, errbuf);
+        dev_get(C, B_KEEPRAW, 24 * n1, errbuf);
         // the filter's list of points for the modular determinant (up to 2^20)
         const size_t mc = std::min<size_t>(n1, (size_t)1 << 20);
         dev_get(C, B_MODLIST, 8 * mc, errbuf);
