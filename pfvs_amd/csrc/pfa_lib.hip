@@ -965,6 +965,7 @@ __global__ __launch_bounds__(64) void k_expand(int t, int64_t *recs, RecL L, Ent
     __shared__ int32_t s_c[64 * PFA_CSTRIDE];
     int32_t *c = s_c + lane * PFA_CSTRIDE;
     unsigned long long n_nodes = 0, n_cand = 0, n_leaf = 0;
+    unsigned long long n_cc = 0;                         // the children's candidates (wave total)
     unsigned long long pt_next = 0, pt_end = 0;
     for (unsigned long long base = (unsigned long long)blockIdx.x * 64ULL; base < n_in;
          base += (unsigned long long)gridDim.x * 64ULL) {
@@ -1064,10 +1065,8 @@ __global__ __launch_bounds__(64) void k_expand(int t, int64_t *recs, RecL L, Ent
                 // the children's candidates, summed over the wavefront
                 long long cc = has_child ? (long long)ch.hi - ch.cur + 1 : 0;
                 for (int o = 32; o > 0; o >>= 1) cc += __shfl_xor(cc, o, 64);
-                if (lane == 0) {
-                    s_u64[0] = atomicAdd(&st[t + 1], (unsigned long long)nch);
-                    atomicAdd(&st[64 + t + 1], (unsigned long long)cc);
-                }
+                if (lane == 0) s_u64[0] = atomicAdd(&st[t + 1], (unsigned long long)nch);
+                n_cc += (unsigned long long)cc;
                 __syncthreads();
                 const unsigned long long ob = s_u64[0];
                 __syncthreads();
@@ -1093,6 +1092,7 @@ __global__ __launch_bounds__(64) void k_expand(int t, int64_t *recs, RecL L, Ent
         if (n_nodes) atomicAdd(&C.c[0], n_nodes);
         if (n_cand) atomicAdd(&C.c[1], n_cand);
         if (n_leaf) atomicAdd(&C.c[2], n_leaf);
+        if (n_cc) atomicAdd(&st[64 + t + 1], n_cc);
     }
 }
 
