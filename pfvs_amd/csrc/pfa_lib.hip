@@ -179,7 +179,7 @@ struct GeoArgs {
 // --------------------------------------------------- preparation (team)
 PFA_HD static inline void prep_one(const PfaTeam &T, char *ws_hot, char *ws_cold, const GeoArgs &G,
                                    const int64_t *ps, int MH, const int32_t *pgeo, int64_t ip,
-                                   int64_t *rec, RecL L)
+                                   int64_t *rec, RecL L, bool sp = true)
 {
     const int geo = pgeo[ip];
     const int h = G.h[geo], d = h - 1;
@@ -197,7 +197,7 @@ PFA_HD static inline void prep_one(const PfaTeam &T, char *ws_hot, char *ws_cold
         ri[26] = geo;
     }
     PfaWS W;
-    pfa_ws_carve(W, ws_hot, ws_cold, h);
+    pfa_ws_carve(W, ws_hot, ws_cold, h, sp);
     if (pfa_empty_certificate(T, W, G.kappa + G.kofs[geo], p, G.qmax[geo], G.linmin[geo])) {
         // the search would enter the root and find no candidate: one node
         PFA_SYNC();
@@ -318,16 +318,18 @@ PFA_HD static inline void prep_one(const PfaTeam &T, char *ws_hot, char *ws_cold
 
 // order (optional): the p-vectors this launch prepares, by index (a size
 // class of a mixed batch, its LDS sized for the class's largest h11).
+// sp: the workspace holds the Gram-Schmidt running values (pfa_ws_carve)
 __global__ __launch_bounds__(64) void k_prep(GeoArgs G, const int64_t *ps, int MH,
                                              const int32_t *pgeo, int64_t n, const int32_t *order,
-                                             int64_t *recs, RecL L, char *cold, size_t cold_bytes)
+                                             int64_t *recs, RecL L, char *cold, size_t cold_bytes,
+                                             int sp)
 {
     extern __shared__ __align__(16) char smem[];
     PfaTeam T{(int)threadIdx.x, 64};
     char *mine = cold + (size_t)blockIdx.x * cold_bytes;
     for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
         const int64_t ip = order ? (int64_t)order[i] : i;
-        prep_one(T, smem, mine, G, ps, MH, pgeo, ip, recs + ip * L.words, L);
+        prep_one(T, smem, mine, G, ps, MH, pgeo, ip, recs + ip * L.words, L, sp != 0);
         __syncthreads();
     }
 }
@@ -3214,8 +3216,13 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
             int64_t off = 0;
             for (const PrepClass &c : cls) {
                 const int grid = (int)std::min<int64_t>(c.n, 16384);
-                sg.k(k_prep, dim3(grid), dim3(64), pfa_ws_lds_bytes(c.h), G, d_ps, MH, d_pgeo, c.n,
-                     d_order ? d_order + off : nullptr, d_recs, L, d_cold, pfa_ws_cold_bytes(c.h));
+                // the Gram-Schmidt running values in LDS unless they would
+                // keep a launch of large h11 from holding all its p-vectors
+                // at once (h11 <= 24: at least 9 per CU either way)
+                const int64_t per_cu = (c.n + std::max(C->cus, 1) - 1) / std::max(C->cus, 1);
+                const bool sp = c.h <= 24 || (size_t)per_cu * pfa_ws_lds_bytes(c.h, true) <= C->lds_cu;
+                sg.k(k_prep, dim3(grid), dim3(64), pfa_ws_lds_bytes(c.h, sp), G, d_ps, MH, d_pgeo, c.n,
+                     d_order ? d_order + off : nullptr, d_recs, L, d_cold, pfa_ws_cold_bytes(c.h), (int)sp);
                 off += c.n;
             }
         }, errbuf);

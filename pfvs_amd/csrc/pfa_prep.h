@@ -44,9 +44,11 @@ struct PfaTeam {
 //       hw (h int128), hpiv (h int) of the HNF and the unimodular
 //       completion, and lll_form's Gy (h int128) before its LLL starts;
 //   bb, mw, vec (h), flag;
-//   sp  per Gram-Schmidt row kk, the running value of bb[kk] before each
-//       of its kk terms (strict lower triangle, as mu): a row exchange
-//       then moves a row's values instead of recomputing them.
+//   sp  (optional) per Gram-Schmidt row kk, the running value of bb[kk]
+//       before each of its kk terms (strict lower triangle, as mu): a row
+//       exchange then moves a row's values instead of recomputing them.
+//       Without it (a launch that needs the LDS for occupancy) the row is
+//       recomputed; both give the same bits.
 // The part only O(h^3) products touch ("cold") is per-workgroup global
 // memory on the device: Z, W2, W3 (h*h int64), Binter, ZB (h*(h-1)), mat
 // ((h-1)^2).  On the host both parts are one buffer.
@@ -95,7 +97,7 @@ PFA_HD static inline size_t pfa_ws_mu_bytes(int h)
 
 // Bytes of the hot (LDS) and cold (global) parts for h, 16-byte aligned.
 // (GG and W1 together hold H: (h+1)(h+2) 8 + 8 h^2 >= 16 (h-1)^2.)
-PFA_HD static inline size_t pfa_ws_lds_bytes(int h)
+PFA_HD static inline size_t pfa_ws_lds_bytes(int h, bool sp = true)
 {
     size_t w = 0;
     w += pfa_ws_gg_bytes(h);                             // GG (H with W1)
@@ -103,7 +105,7 @@ PFA_HD static inline size_t pfa_ws_lds_bytes(int h)
     w += pfa_ws_mu_bytes(h);                             // mu / hv, hw, hpiv / Gy
     w += PFA_WS_ROUND((size_t)h * 8) * 3;                // bb, mw, vec
     w += 16;                                             // flag
-    w += PFA_WS_ROUND((size_t)h * (h - 1) / 2 * 8);       // sp
+    if (sp) w += PFA_WS_ROUND((size_t)h * (h - 1) / 2 * 8);   // sp
     return w;
 }
 
@@ -118,7 +120,7 @@ PFA_HD static inline size_t pfa_ws_cold_bytes(int h)
 
 PFA_HD static inline size_t pfa_ws_bytes(int h) { return pfa_ws_lds_bytes(h) + pfa_ws_cold_bytes(h); }
 
-PFA_HD static inline void pfa_ws_carve(PfaWS &W, char *hot, char *cold, int h)
+PFA_HD static inline void pfa_ws_carve(PfaWS &W, char *hot, char *cold, int h, bool sp = true)
 {
     size_t d = h - 1;
     auto take = [](char *&p, size_t bytes) { char *r = p; p += PFA_WS_ROUND(bytes); return r; };
@@ -135,7 +137,7 @@ PFA_HD static inline void pfa_ws_carve(PfaWS &W, char *hot, char *cold, int h)
     W.mw = (double *)take(hot, (size_t)h * 8);
     W.vec = (int64_t *)take(hot, (size_t)h * 8);
     W.flag = (int *)take(hot, 16);
-    W.sp = (double *)take(hot, (size_t)h * (h - 1) / 2 * 8);
+    W.sp = sp ? (double *)take(hot, (size_t)h * (h - 1) / 2 * 8) : nullptr;
     W.Z = (int64_t *)take(cold, (size_t)h * h * 8);
     W.W2 = (int64_t *)take(cold, (size_t)h * h * 8);
     W.W3 = (int64_t *)take(cold, (size_t)h * h * 8);
@@ -324,7 +326,7 @@ PFA_HD static inline void pfa_gs_row(const PfaTeam &T, const PfaWS &W, int kk, c
             if (j == l) mu[rk + l] = m;
             else if (j > l && j < kk) r -= mu[rj + l] * m * bb[l];
             else if (j == kk) {
-                sp[rk + l] = r;
+                if (sp) sp[rk + l] = r;
                 r -= m * m * bb[l];
             }
         }
@@ -340,7 +342,7 @@ PFA_HD static inline void pfa_gs_row(const PfaTeam &T, const PfaWS &W, int kk, c
     }
     double dd = (double)GG[pfa_gi(kk, kk)];
     for (int l = 0; l < kk; ++l) {
-        if (T.lane == 0) sp[rk + l] = dd;
+        if (sp && T.lane == 0) sp[rk + l] = dd;
         dd -= mu[rk + l] * mu[rk + l] * bb[l];
     }
     if (T.lane == 0) bb[kk] = dd;
@@ -465,7 +467,12 @@ PFA_HD static inline int pfa_lll_core(const PfaTeam &T, const PfaWS &W, int64_t 
                 GG[pfa_gi(k - 1, l)] = t;
             }
             PFA_SYNC();
-            if (k >= 2) {
+            if (k >= 2 && !W.sp) {
+                // rows below k-1 are unchanged (what the reference recomputes
+                // of them comes out the same): only the new row k-1
+                k--;
+                pfa_gs_row(T, W, k, GG);
+            } else if (k >= 2) {
                 // rows below k-1 are unchanged, and the new row k-1 (the old
                 // row k's vector) has the old row k's first k-1 coefficients
                 // and running values, bit for bit (the same operations on
