@@ -2177,6 +2177,7 @@ struct DevCtx {
     // oldest is dropped beyond PFA_GRAPHS_MAX)
     std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
     unsigned long long graph_hits = 0, graph_captures = 0;
+    DevBuf up_host[2];                     // pinned staging of packed uploads (inputs, filter forms)
 };
 
 #define PFA_GRAPHS_MAX 48
@@ -2215,7 +2216,7 @@ enum {
     B_TOT, B_KEEPCNT, B_KEEPKP, B_KEEPNEXP, B_KEEPMETA, B_PREPORDER,
     B_SPR, B_SPKV, B_SPROFS, B_SPKVOFS, B_SPINTS,
     B_SV_PIDX, B_SV_Q, B_SV_M, B_SV_KN, B_SV_NEXP, B_SV_NS, B_SV_CNT, B_STATSAVE, B_STATOUT,
-    B_PCOUNT, B_F0, B_F1, B_SPLITST, B_BOX, B_BOXOUT, B_PT_G, B_KEEPRAW,
+    B_PCOUNT, B_F0, B_F1, B_SPLITST, B_BOX, B_BOXOUT, B_PT_G, B_KEEPRAW, B_UPIN, B_UPFILT,
     B_NBUF
 };
 
@@ -2261,6 +2262,51 @@ static void *host_get(DevCtx *C, Stage &S, int slot, size_t bytes, char *errbuf)
 }
 
 #define CTL_BYTES (16 * 8 * 16)
+
+// A call's small inputs sent with one copy: packed (256-byte aligned) into
+// a grow-only pinned host buffer of the device, then one host-to-device
+// copy into one device buffer; each pageable copy would be a staged
+// transfer of its own.  The staging buffer is rewritten by the next call
+// only, after this call's stream has finished.
+struct Packed {
+    std::vector<const void *> src;
+    std::vector<size_t> off, len;
+    size_t total = 0;
+    size_t add(const void *p, size_t bytes)
+    {
+        src.push_back(p);
+        off.push_back(total);
+        len.push_back(bytes);
+        const size_t at = total;
+        total += (std::max<size_t>(bytes, 16) + 255) & ~(size_t)255;
+        return at;
+    }
+};
+
+static void pinned_reserve(DevCtx *C, int which, size_t bytes, char *errbuf)
+{
+    DevBuf &h = C->up_host[which];
+    if (h.n >= bytes) return;
+    auto t0 = std::chrono::steady_clock::now();
+    if (h.p) (void)hipHostFree(h.p);
+    h.p = nullptr;
+    h.n = 0;
+    const size_t want = bytes + bytes / 4 + 4096;
+    HIPCHK(hipHostMalloc(&h.p, want, hipHostMallocDefault));
+    h.n = want;
+    C->ms_alloc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+static char *packed_upload(DevCtx *C, int which, int dslot, const Packed &P, hipStream_t s, char *errbuf)
+{
+    pinned_reserve(C, which, std::max<size_t>(P.total, 16), errbuf);
+    char *h = (char *)C->up_host[which].p;
+    for (size_t k = 0; k < P.src.size(); ++k)
+        if (P.len[k]) memcpy(h + P.off[k], P.src[k], P.len[k]);
+    char *d = (char *)dev_get(C, dslot, std::max<size_t>(P.total, 16), errbuf);
+    if (P.total) HIPCHK(hipMemcpyAsync(d, h, P.total, hipMemcpyHostToDevice, s));
+    return d;
+}
 
 
 // the k largest primes below 2^31 with their Barrett constants
@@ -2623,17 +2669,26 @@ static FiltLaunch filter_setup(DevCtx *C, hipStream_t s, const pfa_run_input *gi
         for (int g = 0; g < gin->n_geo; ++g)
             if (spRofs[g] >= 0 && spInts[g] <= 12 * 1024) lds_floats = std::max(lds_floats, (int)spInts[g]);
     F.d_mc = (unsigned long long *)dev_get(C, B_MODCNT, 8, errbuf);
-    a.kT = (const int32_t *)upload(C, B_KT, kT.data(), 4 * kT.size(), s, errbuf);
-    a.kTofs = (const int64_t *)upload(C, B_KTOFS, kTofs.data(), 8 * kTofs.size(), s, errbuf);
-    a.kTmax = (const double *)upload(C, B_KTMAX, kTmax.data(), 8 * kTmax.size(), s, errbuf);
-    a.spR = (const int32_t *)upload(C, B_SPR, spR.data(), 4 * spR.size(), s, errbuf);
-    a.spKV = (const int32_t *)upload(C, B_SPKV, spKV.data(), 4 * spKV.size(), s, errbuf);
-    a.spRofs = (const int64_t *)upload(C, B_SPROFS, spRofs.data(), 8 * spRofs.size(), s, errbuf);
-    a.spKVofs = (const int64_t *)upload(C, B_SPKVOFS, spKVofs.data(), 8 * spKVofs.size(), s, errbuf);
-    a.spInts = (const int32_t *)upload(C, B_SPINTS, spInts.data(), 4 * spInts.size(), s, errbuf);
+    Packed P;
+    const size_t o_kT = P.add(kT.data(), 4 * kT.size()), o_kTo = P.add(kTofs.data(), 8 * kTofs.size()),
+                 o_kTm = P.add(kTmax.data(), 8 * kTmax.size()), o_spR = P.add(spR.data(), 4 * spR.size()),
+                 o_spKV = P.add(spKV.data(), 4 * spKV.size()),
+                 o_spRo = P.add(spRofs.data(), 8 * spRofs.size()),
+                 o_spKVo = P.add(spKVofs.data(), 8 * spKVofs.size()),
+                 o_spI = P.add(spInts.data(), 4 * spInts.size()),
+                 o_pr = P.add(pr.data(), sizeof(PfaModP) * pr.size());
+    char *d_f = packed_upload(C, 1, B_UPFILT, P, s, errbuf);
+    a.kT = (const int32_t *)(d_f + o_kT);
+    a.kTofs = (const int64_t *)(d_f + o_kTo);
+    a.kTmax = (const double *)(d_f + o_kTm);
+    a.spR = (const int32_t *)(d_f + o_spR);
+    a.spKV = (const int32_t *)(d_f + o_spKV);
+    a.spRofs = (const int64_t *)(d_f + o_spRo);
+    a.spKVofs = (const int64_t *)(d_f + o_spKVo);
+    a.spInts = (const int32_t *)(d_f + o_spI);
     a.n_geo = gin->n_geo;
     a.lds_floats = a.filter ? lds_floats : 0;
-    a.primes = (const PfaModP *)upload(C, B_PRIMES, pr.data(), sizeof(PfaModP) * pr.size(), s, errbuf);
+    a.primes = (const PfaModP *)(d_f + o_pr);
     a.np = (int)pr.size();
     a.modcap = std::max<int64_t>(1, std::min<int64_t>(a.n, 1 << 20));
     a.modkp = (int64_t *)dev_get(C, B_MODLIST, 8 * a.modcap, errbuf);
@@ -3175,17 +3230,24 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
         return d;
     };
     auto t_up = std::chrono::steady_clock::now();
+    Packed P;
+    const size_t o_h = P.add(in->h, 4 * in->n_geo), o_Q = P.add(in->Q, 8 * in->n_geo),
+                 o_qm = P.add(in->qmax, 8 * in->n_geo), o_lm = P.add(in->linmin, 8 * in->n_geo),
+                 o_k = P.add(in->kappa, 8 * ktot), o_mb = P.add(in->Mbasis, 8 * mtot),
+                 o_ko = P.add(kofs.data(), 8 * in->n_geo), o_mo = P.add(mofs.data(), 8 * in->n_geo),
+                 o_ps = P.add(in->ps, 8 * n_p * MH), o_pg = P.add(in->pgeo, 4 * n_p);
+    char *d_in = packed_upload(C, 0, B_UPIN, P, s, errbuf);
     GeoArgs G;
-    G.h = (const int32_t *)up(B_GEO_H, in->h, 4 * in->n_geo);
-    G.Q = (const int64_t *)up(B_GEO_Q, in->Q, 8 * in->n_geo);
-    G.qmax = (const int64_t *)up(B_GEO_QMAX, in->qmax, 8 * in->n_geo);
-    G.linmin = (const int64_t *)up(B_GEO_LIN, in->linmin, 8 * in->n_geo);
-    G.kappa = (const int64_t *)up(B_KAPPA, in->kappa, 8 * ktot);
-    G.Mbasis = (const int64_t *)up(B_MBASIS, in->Mbasis, 8 * mtot);
-    G.kofs = (const int64_t *)up(B_KOFS, kofs.data(), 8 * in->n_geo);
-    G.mofs = (const int64_t *)up(B_MOFS, mofs.data(), 8 * in->n_geo);
-    const int64_t *d_ps = (const int64_t *)up(B_PS, in->ps, 8 * n_p * MH);
-    const int32_t *d_pgeo = (const int32_t *)up(B_PGEO, in->pgeo, 4 * n_p);
+    G.h = (const int32_t *)(d_in + o_h);
+    G.Q = (const int64_t *)(d_in + o_Q);
+    G.qmax = (const int64_t *)(d_in + o_qm);
+    G.linmin = (const int64_t *)(d_in + o_lm);
+    G.kappa = (const int64_t *)(d_in + o_k);
+    G.Mbasis = (const int64_t *)(d_in + o_mb);
+    G.kofs = (const int64_t *)(d_in + o_ko);
+    G.mofs = (const int64_t *)(d_in + o_mo);
+    const int64_t *d_ps = (const int64_t *)(d_in + o_ps);
+    const int32_t *d_pgeo = (const int32_t *)(d_in + o_pg);
     int64_t *d_recs = (int64_t *)dev_get(C, B_RECS, 8 * (size_t)L.words * std::max<int64_t>(n_p, 1), errbuf);
     const double ms_upload =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_up).count();
@@ -3747,6 +3809,11 @@ extern "C" void pfg_release(int device)
             b.n = 0;
         }
     }
+    for (DevBuf &b : C->up_host) {
+        if (b.p) (void)hipHostFree(b.p);
+        b.p = nullptr;
+        b.n = 0;
+    }
 }
 
 // --------------------------------------------------- p-vector enumeration
@@ -4011,6 +4078,12 @@ extern "C" int pfa_reserve(int device, int64_t n_p, int64_t n_points, int h_max,
         dev_get(C, B_KEEPNEXP, 8 * n1, errbuf);
         dev_get(C, B_KEEPMETA, 8 * n1, errbuf);
         dev_get(C, B_KEEPRAW, 24 * n1, errbuf);
+        // packed uploads: the p-vectors and a few MB of geometry tables
+        const size_t up_in = np1 * (8 * (size_t)h_max + 4) + ((size_t)4 << 20);
+        pinned_reserve(C, 0, up_in, errbuf);
+        dev_get(C, B_UPIN, up_in, errbuf);
+        pinned_reserve(C, 1, (size_t)4 << 20, errbuf);
+        dev_get(C, B_UPFILT, (size_t)4 << 20, errbuf);
         // the filter's list of points for the modular determinant (up to 2^20)
         const size_t mc = std::min<size_t>(n1, (size_t)1 << 20);
         dev_get(C, B_MODLIST, 8 * mc, errbuf);
