@@ -1754,14 +1754,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             const bool r32 = row && bmax < ((int64_t)1 << 31);
             double acc = 0.0, csum = 0.0;
             if (!__ballot(row && !lb)) {
-                // every row from LDS
+                // every row from LDS, all NMAX entries read unconditionally
+                // where the slice exists (NMAX <= 24: inside it; cd = 0 adds
+                // nothing, and a lane without a row does not use its sum)
                 const int32_t *br = sB + (i < h ? i : 0) * n;
 #pragma unroll 4
                 for (int t = 0; t < NMAX; ++t) {
                     const int32_t ct = __shfl(cv, L.src0 + t);
                     const double cd = t < n ? (double)ct : 0.0;
                     csum += fabs(cd);
-                    if (row && t < n) acc = fma((double)br[t], cd, acc);
+                    if constexpr (NMAX <= 24) acc = fma((double)br[t], cd, acc);
+                    else if (row && t < n) acc = fma((double)br[t], cd, acc);   // (no LDS slice: lb never holds)
                 }
             } else {
                 // every entry read, unconditionally, so the loads issue
