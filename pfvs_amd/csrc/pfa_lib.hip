@@ -1828,7 +1828,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         if constexpr (NMAX <= 24) {
             if (form == 0) {
 #pragma unroll
-                for (int k = 0; k <= NMAX; ++k) mk[k] = k <= n ? gM[k] : 0;
+                for (int k = 0; k <= NMAX; ++k) {
+                    // (every read issued: the index clamped, the value masked)
+                    const int32_t x = gM[k <= n ? k : 0];
+                    mk[k] = k <= n ? x : 0;
+                }
             }
         }
         const int32_t *rowp = lds_kT + (i < NMAX ? i : NMAX - 1) * (NMAX + 1);
