@@ -1337,10 +1337,15 @@ __device__ __forceinline__ int64_t post_dot(const int64_t *row, const double *cd
                                             int d, bool fast, int64_t rmax, int64_t cmax, int *ok)
 {
     if (fast) {
-        const int32_t *r32 = (const int32_t *)row;
+        // every low word read into registers first (one load each, then
+        // one wait), then the FMAs in order
+        const int2 *r2 = (const int2 *)row;
+        int32_t rv[NMAX];
+#pragma unroll
+        for (int t = 0; t < NMAX; ++t) rv[t] = r2[t].x;
         double s = 0.0;
 #pragma unroll
-        for (int t = 0; t < NMAX; ++t) s = fma((double)r32[2 * t], cd[t], s);
+        for (int t = 0; t < NMAX; ++t) s = fma((double)rv[t], cd[t], s);
         return (int64_t)s;
     }
     return pfa_dot_c(row, cc, d, rmax, cmax, ok);
