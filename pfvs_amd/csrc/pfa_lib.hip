@@ -2607,11 +2607,15 @@ static bool graphs_on()
     return !(e && e[0] == '0');
 }
 
+// graph = false: launched directly.  A segment that starts on an idle
+// device (right after the host's uploads, or after it waited for the
+// search) runs its first kernel sooner that way: a graph launch costs the
+// host about 0.2 ms before the device sees any of it.
 template <class F>
-static void run_seg(DevCtx *C, hipStream_t s, F body, char *errbuf)
+static void run_seg(DevCtx *C, hipStream_t s, F body, char *errbuf, bool graph = true)
 {
     Seg sg{s, errbuf, 0, std::string()};
-    if (!graphs_on()) {
+    if (!graph || !graphs_on()) {
         body(sg);
         return;
     }
@@ -3299,7 +3303,7 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
                      d_order ? d_order + off : nullptr, d_recs, L, d_cold, pfa_ws_cold_bytes(c.h), (int)sp);
                 off += c.n;
             }
-        }, errbuf);
+        }, errbuf, false);
     }
     HIPCHK(hipEventRecord(ev[1], s));
 
@@ -3364,7 +3368,7 @@ static int run_device(const pfa_run_input *in, pfa_run_output *out, char *errbuf
         if (n_p > 0)
             sg.k(k_status_io, dim3((int)std::min<int64_t>((n_p + 255) / 256, 4096)), dim3(256), 0, n_p,
                  d_recs, L.words, d_stat, 2);
-    }, errbuf);
+    }, errbuf, false);
     // the survivors (keep 2): compacted; their count read back, with a
     // second pass if they outgrew the buffer
     int64_t nsv = 0;
