@@ -2438,6 +2438,8 @@ static void canonical_order(std::vector<int64_t> &idx, const int64_t *pidx, cons
         for (int i = 0; i < MH; ++i) if (ka[i] != kb[i]) return ka[i] < kb[i];
         return false;
     };
+    // threads by the points to sort (starting them costs tens of us each)
+    nth = (int)std::max<int64_t>(1, std::min<int64_t>(nth, (int64_t)idx.size() / 4096));
     parallel_chunks(n_p, 64, nth, [&](int64_t lo, int64_t hi) {
         for (int64_t ip = lo; ip < hi; ++ip)
             std::sort(by_p.begin() + start[ip], by_p.begin() + start[ip + 1], less);
@@ -2817,6 +2819,9 @@ static int assemble_core(const AsmIn &A, AsmOut &O, int nth, char *errbuf)
 {
     const int S = A.stride;
     const int64_t nk = A.n_keys, n = A.n;
+    // threads by the points to expand (a key without points costs nothing;
+    // starting 16 threads twice costs about a millisecond)
+    nth = (int)std::max<int64_t>(1, std::min<int64_t>(nth, n / 4096));
     O.done = (int8_t *)calloc((size_t)std::max<int64_t>(nk, 1), 1);
     std::vector<int64_t> pb(nk + 1, 0), rows(nk, -1), base(nk, 0);
     for (int64_t t = 0; t < n; ++t) pb[A.key[t] + 1]++;
