@@ -421,6 +421,35 @@ struct CountArgs {
     unsigned long long *c;        // nodes, cand, leaf
 };
 
+// A node entry's coordinates to the lane's c (those at or below the node's
+// level i are 0 there), and the lane's c to a child's entry (its entries
+// below its level are 0 in c), four at a time with every read of a block
+// issued before its writes: an element at a time, each write waited on its
+// read.  The last block reads at most index D - 1.
+__device__ static inline void coords_in(int32_t *c, const int32_t *ec, int D, int i)
+{
+    for (int k0 = 0; k0 < D; k0 += 4) {
+        int32_t x[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[q] = ec[k0 + q < D ? k0 + q : D - 1];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (k0 + q < D) c[k0 + q] = k0 + q > i ? x[q] : 0;
+    }
+}
+
+__device__ static inline void coords_out(int32_t *ec, const int32_t *c, int D)
+{
+    for (int k0 = 0; k0 < D; k0 += 4) {
+        int32_t x[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[q] = c[k0 + q < D ? k0 + q : D - 1];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (k0 + q < D) ec[k0 + q] = x[q];
+    }
+}
+
 __device__ static inline void entry_copy(char *dst, const char *src, int bytes)
 {
     const int4 *s = (const int4 *)src;
@@ -764,9 +793,8 @@ __global__ __launch_bounds__(64) void k_search(BatchArgs a)
             const char *e = ent(top - 1 - j);
             nd = *(const PfaNode *)e;
             pidx = ((const int32_t *)(e + 128))[0];
-            const int32_t *ec = (const int32_t *)(e + 144);
             // (one loop of the wave-uniform length D: no divergent trip counts)
-            for (int k = 0; k < D; ++k) c[k] = k > nd.i ? ec[k] : 0;
+            coords_in(c, (const int32_t *)(e + 144), D, nd.i);
             v = nd.cur + (lane - (j > 0 ? s_incl[j - 1] : 0));
         }
         int part_new_cur = 0;
@@ -880,8 +908,7 @@ __global__ __launch_bounds__(64) void k_search(BatchArgs a)
             char *e = ent(slot);
             *(PfaNode *)e = ch;
             ((int32_t *)(e + 128))[0] = pidx;
-            int32_t *ec = (int32_t *)(e + 144);
-            for (int k = ch.i + 1; k < D; ++k) ec[k] = c[k];
+            coords_out((int32_t *)(e + 144), c, D);   // (c is 0 at and below ch.i)
         }
         top = newtop + nch;
         __syncthreads();                     // writes visible to the next step
@@ -1005,8 +1032,7 @@ __global__ __launch_bounds__(64) void k_expand(int t, int64_t *recs, RecL L, Ent
                 const char *e = s_nodes + (size_t)j * ES;
                 nd = *(const PfaNode *)e;
                 pidx = ((const int32_t *)(e + 128))[0];
-                const int32_t *ec = (const int32_t *)(e + 144);
-                for (int k = 0; k < D; ++k) c[k] = k > nd.i ? ec[k] : 0;
+                coords_in(c, (const int32_t *)(e + 144), D, nd.i);
                 v = nd.cur + (int32_t)(g - (j > 0 ? s_incl[j - 1] : 0));
             }
             int has_child = 0, emit = 0;
@@ -1077,8 +1103,7 @@ __global__ __launch_bounds__(64) void k_expand(int t, int64_t *recs, RecL L, Ent
                     char *e = Fout + (ob + __popcll(cm & ((1ULL << lane) - 1))) * (unsigned long long)ES;
                     *(PfaNode *)e = ch;
                     ((int32_t *)(e + 128))[0] = pidx;
-                    int32_t *ec = (int32_t *)(e + 144);
-                    for (int k = 0; k < D; ++k) ec[k] = k > ch.i ? c[k] : 0;
+                    coords_out((int32_t *)(e + 144), c, D);   // (c is 0 at and below ch.i)
                 }
             }
         }
