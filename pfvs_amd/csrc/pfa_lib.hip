@@ -1026,6 +1026,7 @@ __global__ __launch_bounds__(64) void k_expand(int t, int64_t *recs, RecL L, Ent
     char *Fout = (t & 1) ? S.F0 : S.F1;
     extern __shared__ __align__(16) char s_nodes[];      // 64 entries
     __shared__ long long s_incl[64];
+    __shared__ int s_mark[64];
     __shared__ unsigned long long s_u64[2];
     __shared__ int32_t s_c[64 * PFA_CSTRIDE];
     int32_t *c = s_c + lane * PFA_CSTRIDE;
@@ -1048,20 +1049,24 @@ __global__ __launch_bounds__(64) void k_expand(int t, int64_t *recs, RecL L, Ent
             cnt = (long long)nd->hi - nd->cur + 1;
             if (cnt < 0) cnt = 0;
         }
-        s_incl[lane] = wave_scan_incl64(cnt);
+        const long long incl = wave_scan_incl64(cnt);
+        s_incl[lane] = incl;
         __syncthreads();
         const long long total = s_incl[63];
         for (long long r0 = 0; r0 < total; r0 += 64) {
             const long long g = r0 + lane;
-            int j = -1;
-            if (g < total) {
-                int lo = 0, hi2 = nb - 1;
-                while (lo < hi2) {
-                    int mid = (lo + hi2) >> 1;
-                    if (s_incl[mid] > g) hi2 = mid; else lo = mid + 1;
-                }
-                j = lo;
-            }
+            // lane -> node, as in the search step: each node whose
+            // candidates meet the round's window marks its first lane in it
+            // (at most one node covers the window's first lane), then a
+            // running max over the lanes
+            s_mark[lane] = -1;
+            __syncthreads();
+            const long long ex = incl - cnt;
+            if (cnt > 0 && incl > r0 && ex < r0 + 64) s_mark[ex > r0 ? (int)(ex - r0) : 0] = lane;
+            __syncthreads();
+            int j = wave_scan_max(s_mark[lane]);
+            if (g >= total) j = -1;
+            __syncthreads();                          // the marks read before the next round's
             PfaNode nd;
             int32_t pidx = 0, v = 0;
             if (j >= 0) {
