@@ -126,11 +126,13 @@ def time_mode(gpu, geoms, pf, pgeo, mode, steps, warmup, **kw):
 
 def search_work(gpu, geoms, pf, pgeo):
     """Algorithmic lane-ops of the search over the batch (host path, count
-    only), with its node count and the threads used."""
+    only), with its node count, the threads used and the host run's stats
+    (prep_work, prep_work_max: the preparation's lane-ops, all p-vectors and
+    the largest one)."""
     threads = max(1, min(len(os.sched_getaffinity(0)), 16))
     r = gpu.run(geoms, pf, pgeo, mode="host", host_threads=threads, keep_points=False,
                 filter=False)
-    return r["stats"]["work"], r["stats"]["nodes"], threads
+    return r["stats"]["work"], r["stats"]["nodes"], threads, r["stats"]
 
 
 TRAFFIC_FILE = os.path.join("profiles", "search_h20_hbm_traffic.md")
@@ -377,7 +379,7 @@ def main():
                    "graph_captures": sb["graph_captures"],
                    "host_fallback_rate": sb["host_p"] / max(len(pf), 1)},
     }
-    work, wnodes, wthreads = search_work(gpu, geoms, pf, pgeo)
+    work, wnodes, wthreads, hst = search_work(gpu, geoms, pf, pgeo)
     assert wnodes == nodes, "host count and device search visited different nodes"
     cus, clk_khz = gpu.device_facts(0)
     peak = cus * 64 * clk_khz * 1e3                 # lane-ops/s
@@ -392,6 +394,17 @@ def main():
                        "work_per_launch": work,
                        "ms_per_launch": ms_search,
                        "peak_basis": f"{cus} CUs x 64 lanes x {clk_khz / 1e6:.2f} GHz"}
+    # k_prep: one wavefront per p-vector; the pass's lane-ops against the same
+    # peak, and the largest p-vector's lane-ops (its wavefront's latency is
+    # the stage's floor: the rest of the GPU is idle once short ones finish)
+    ms_prep = mean("ms_prep", st_b)
+    out["prep_roofline"] = {"bound": "latency of one wavefront", "kernel": "k_prep",
+                            "work_per_launch": hst["prep_work"],
+                            "work_largest_p": hst["prep_work_max"],
+                            "ms_per_launch": ms_prep,
+                            "achieved": hst["prep_work"] / (ms_prep * 1e-3) / 1e12,
+                            "peak": peak / 1e12, "unit": "T lane-op/s",
+                            "frac": hst["prep_work"] / (ms_prep * 1e-3) / peak}
     if not args.no_plain:
         for mode in ("plain", "lane"):
             if mode == "lane" and args.workload != "headline":

@@ -130,6 +130,8 @@ enum {
     PFA_ST_GRAPHS,        /* device: instantiated segment graphs held (0: graphs off) */
     PFA_ST_GRAPH_HITS,    /* device: segment launches replayed from a held graph (cumulative) */
     PFA_ST_GRAPH_CAPTURES,/* device: segments captured and instantiated (cumulative) */
+    PFA_ST_PREP_WORK,     /* host mode: algorithmic lane-ops of the preparation, all p-vectors */
+    PFA_ST_PREP_WORK_MAX, /* host mode: the same for the p-vector with the most */
     PFA_ST_N
 };
 

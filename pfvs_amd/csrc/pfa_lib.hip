@@ -3603,6 +3603,7 @@ static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
     std::vector<PfaCounts> tc(nth, PfaCounts{0, 0, 0});
     std::vector<PfaWork> tw(nth, PfaWork{0, 0, 0, 0, 0, 0});
     std::vector<int64_t> tn(nth, 0);                 // points found per thread
+    std::vector<double> tpw(nth, 0.0), tpw_max(nth, 0.0);   // preparation lane-ops
     std::vector<int64_t> pcount(std::max<int64_t>(n_p, 1), 0);   // points per p-vector
     const bool store = in->keep_points || in->filter; // else counts only
     std::atomic<int64_t> next{0};
@@ -3615,8 +3616,12 @@ static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
             int64_t ip = next.fetch_add(1);
             if (ip >= n_p) break;
             int64_t *rec = recs.data() + ip * L.words;
+            const double ops0 = PFA_OPS_NOW();
             prep_one(T, ws.data(), ws.data() + pfa_ws_lds_bytes(D + 1), G, in->ps, MH, in->pgeo, ip,
                      rec, L);
+            const double ops = PFA_OPS_NOW() - ops0;
+            tpw[t] += ops;
+            tpw_max[t] = std::max(tpw_max[t], ops);
             if (rec_status(rec) != PFA_OK) continue;
             PfaProb P = rec_view(rec, L);
             HostPts &hp = tp[t];
@@ -3713,6 +3718,10 @@ static int run_host(const pfa_run_input *in, pfa_run_output *out, char *errbuf)
     out->stats[PFA_ST_CAND] = (double)tot.cand;
     out->stats[PFA_ST_LEAF] = (double)tot.leaf;
     out->stats[PFA_ST_PASSES] = 1;
+    for (int t = 0; t < nth; ++t) {
+        out->stats[PFA_ST_PREP_WORK] += tpw[t];
+        out->stats[PFA_ST_PREP_WORK_MAX] = std::max(out->stats[PFA_ST_PREP_WORK_MAX], tpw_max[t]);
+    }
     out->stats[PFA_ST_MS_SEARCH] = std::chrono::duration<double, std::milli>(t1 - t0).count();
     int rc = finish_run(in, out, n, HS, all.pidx.data(), all.q.data(), M.data(), Kn.data(),
                         in->filter ? nexp.data() : nullptr, in->filter ? ns.data() : nullptr,

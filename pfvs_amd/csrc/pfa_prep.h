@@ -32,6 +32,20 @@ struct PfaTeam {
     int lane, n;
 };
 
+// Algorithmic work of the preparation, counted on the host path only (the
+// device build compiles it away): one lane-op per arithmetic operation on
+// one element (a product, sum, quotient or rounding of any width; an exact
+// multiply-add counts 2), in the steps' textbook counts.  Moves, swaps and
+// overflow checks are not counted.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define PFA_OPS(x) ((void)0)
+#define PFA_OPS_NOW() 0.0
+#else
+inline thread_local double pfa_prep_ops = 0.0;
+#define PFA_OPS(x) (pfa_prep_ops += (double)(x))
+#define PFA_OPS_NOW() pfa_prep_ops
+#endif
+
 // Working storage of one preparation.  The part the LLL and HNF loops
 // touch lives in LDS on the device ("hot"), sized so that 16 wavefronts
 // of h11 <= 20 share a CU's LDS (10 KB each):
@@ -191,6 +205,7 @@ PFA_HD static inline int pfa_unimodular(const PfaTeam &T, const int64_t *v, int 
         if (wk == 0) continue;
         int64_t s, t, g = pfa_xgcd(w0, wk, &s, &t);
         if (g == 0) return -2;
+        PFA_OPS(6 * n + 4);
         int64_t m10 = -pfa_fdiv64(wk, g), m11 = pfa_fdiv64(w0, g);
         w0 = g;
         for (int r = T.lane; r < n; r += T.n) {      // each lane owns columns r
@@ -316,6 +331,7 @@ PFA_HD static inline void pfa_gs_row(const PfaTeam &T, const PfaWS &W, int kk, c
 {
     double *mu = W.mu, *bb = W.bb, *sp = W.sp;
     const int rk = pfa_mi(kk, 0);
+    if (T.lane == 0) PFA_OPS(1.5 * kk * (kk - 1) + 5 * kk + 1);
 #if defined(__HIP_DEVICE_COMPILE__)
     if (T.n > kk) {
         const int j = T.lane;
@@ -355,6 +371,7 @@ PFA_HD static inline void pfa_gs_row(const PfaTeam &T, const PfaWS &W, int kk, c
 PFA_HD static inline void pfa_gs_last(const PfaTeam &T, const PfaWS &W, int m, const pfa_i128 *GG)
 {
     double *mu = W.mu, *bb = W.bb, *mw = W.mw;
+    if (T.lane == 0) PFA_OPS(1.5 * m * (m - 1) + 2 * m);
 #if defined(__HIP_DEVICE_COMPILE__)
     if (T.n >= m) {
         const int j = T.lane;
@@ -385,6 +402,7 @@ PFA_HD static inline int pfa_lll_sub(const PfaTeam &T, const PfaWS &W, int64_t *
     int64_t *bk = kk == m ? last : &B[kk * n];
     const int64_t *bj = jj == m ? last : &B[jj * n];
     int bad = 0;
+    if (T.lane == 0) PFA_OPS(2 * n + 2 * M + 3);
     for (int c = T.lane; c < n; c += T.n) {
         pfa_i128 x = (pfa_i128)bk[c] - (pfa_i128)q * bj[c];
         if (!pfa_fits64(x)) bad = 1;
@@ -430,7 +448,9 @@ PFA_HD static inline int pfa_lll_core(const PfaTeam &T, const PfaWS &W, int64_t 
             int changed = 0;
             for (int j = k - 1; j >= 0; --j) {
                 double q = nearbyint(mu[pfa_mi(k, j)]);
+                if (T.lane == 0) PFA_OPS(1);
                 if (q == 0.0) continue;
+                if (T.lane == 0) PFA_OPS(2 * j + 1);
                 if (!(fabs(q) <= 9.0e18)) return -1;
                 if (pfa_lll_sub(T, W, B, last, m, n, GG, M, k, j, (int64_t)q)) return -1;
                 for (int l = T.lane; l < j; l += T.n) mu[pfa_mi(k, l)] -= q * mu[pfa_mi(j, l)];
@@ -444,6 +464,7 @@ PFA_HD static inline int pfa_lll_core(const PfaTeam &T, const PfaWS &W, int64_t 
         }
         if (!(bb[k] > 0)) return -2;
         const double mk1 = mu[pfa_mi(k, k - 1)];
+        if (T.lane == 0) PFA_OPS(4);
         if (bb[k] < (0.99 - mk1 * mk1) * bb[k - 1]) {
             for (int c = T.lane; c < n; c += T.n) {
                 int64_t t = B[k * n + c];
@@ -504,7 +525,9 @@ PFA_HD static inline int pfa_lll_core(const PfaTeam &T, const PfaWS &W, int64_t 
             int changed = 0;
             for (int j = m - 1; j >= 0; --j) {
                 double q = nearbyint(mw[j]);
+                if (T.lane == 0) PFA_OPS(1);
                 if (q == 0.0) continue;
+                if (T.lane == 0) PFA_OPS(2 * j + 1);
                 if (!(fabs(q) <= 9.0e18)) return -1;
                 if (pfa_lll_sub(T, W, B, last, m, n, GG, M, m, j, (int64_t)q)) return -1;
                 for (int l = T.lane; l < j; l += T.n) mw[l] -= q * mu[pfa_mi(j, l)];
@@ -524,6 +547,7 @@ PFA_HD static inline int pfa_lll_euclid(const PfaTeam &T, const PfaWS &W, int64_
 {
     if (m < 2) return 0;
     int bad = 0;
+    if (T.lane == 0) PFA_OPS((double)m * (m + 1) * n);
     for (int e = T.lane; e < m * m; e += T.n) {
         int i = e / m, j = e % m;
         if (j > i) continue;
@@ -545,6 +569,7 @@ PFA_HD static inline int pfa_lll_form(const PfaTeam &T, const PfaWS &W, int64_t 
     const int M = m + (last != nullptr);
     pfa_i128 *Gy = (pfa_i128 *)W.mu;        // mu is free until pfa_lll_core
     int bad = 0;
+    if (T.lane == 0) PFA_OPS(2.0 * M * n * n + (double)M * (M + 1) * n);
     pfa_team_clear(T, W.flag);
     for (int j = 0; j < M; ++j) {
         const int64_t *y = j == m ? last : &B[j * n];
@@ -599,6 +624,7 @@ PFA_HD static inline int pfa_matmul(const PfaTeam &T, const PfaWS &W, const int6
                                     const int64_t *B, int64_t *C, int m, int k, int n)
 {
     int bad = 0;
+    if (T.lane == 0) PFA_OPS(2.0 * m * k * n);
     if (pfa_int64_safe(pfa_team_absmax(T, A, m * k), pfa_team_absmax(T, B, k * n), k)) {
         for (int e = T.lane; e < m * n; e += T.n) {
             const int i = e / n, j = e % n;
@@ -650,6 +676,7 @@ PFA_HD static inline int pfa_hnf_rowsub(const PfaTeam &T, pfa_i128 *row, const p
 {
     int bad = 0;
     if (q == 0) return 0;
+    if (T.lane == 0) PFA_OPS(2 * (n - c0));
     for (int c = c0 + T.lane; c < n; c += T.n)
         if (pfa_hnf_muladd(-q, prow[c], row[c], &row[c])) bad = 1;
     return bad;
@@ -663,6 +690,7 @@ PFA_HD static inline int pfa_hnf_reduce(const PfaTeam &T, int *flag, pfa_i128 *r
     for (int l = t0; l < m; ++l) {
         int j = piv[l];
         pfa_i128 q = pfa_fdiv128(row[j], H[l * n + j]);
+        if (T.lane == 0) PFA_OPS(1);
         PFA_SYNC();
         bad |= pfa_hnf_rowsub(T, row, &H[l * n], q, j, n);
         PFA_SYNC();
@@ -705,13 +733,16 @@ PFA_HD static inline int pfa_hnf(const PfaTeam &T, const PfaWS &W, const int64_t
             int bad = 0;
             if (pfa_tmod128(b, a) == 0) {
                 pfa_i128 q = pfa_tdiv128(b, a);
+                if (T.lane == 0) PFA_OPS(2);
                 PFA_SYNC();
                 bad = pfa_hnf_rowsub(T, v, hr, q, j, n);
                 if (pfa_team_any(T, W.flag, bad)) return -1;
             } else {
                 pfa_i128 r0 = a, r1 = b, s0 = 1, s1 = 0, x0 = 0, x1 = 1;
+                if (T.lane == 0) PFA_OPS(8 * (n - j) + 2);
                 while (r1) {
                     pfa_i128 q = pfa_tdiv128(r0, r1), tt;
+                    if (T.lane == 0) PFA_OPS(7);
                     tt = r0 - q * r1; r0 = r1; r1 = tt;
                     tt = s0 - q * s1; s0 = s1; s1 = tt;
                     tt = x0 - q * x1; x0 = x1; x1 = tt;
@@ -781,6 +812,7 @@ PFA_HD static inline int pfa_empty_certificate(const PfaTeam &T, const PfaWS &W,
     pfa_i128 *w = W.hv;                                  // (h)
     pfa_i128 *Mx = (pfa_i128 *)W.W2;                     // (n, n) over W2 .. mat
     int bad = 0;
+    if (T.lane == 0) PFA_OPS(2.0 * h * h * (h - 1) + 2.0 * h * (h - 1));
     pfa_team_clear(T, W.flag);
     for (int e = T.lane; e < h * h; e += T.n) {
         pfa_i128 s = 0;
@@ -821,6 +853,7 @@ PFA_HD static inline int pfa_empty_certificate(const PfaTeam &T, const PfaWS &W,
         // products need 256 bits, the quotients (minors) must fit int128
         const pfa_exdiv D = pfa_exdiv_prep(prev);
         const int m = n - 1 - k;
+        if (T.lane == 0) PFA_OPS(4.0 * m * m);
         for (int e = T.lane; e < m * m; e += T.n) {
             const int r = k + 1 + e / m, c = k + 1 + e % m;
             const pfa_w256 N = pfa_sub256(pfa_smul256(piv, Mx[r * n + c]), pfa_smul256(Mx[r * n + k], Mx[k * n + c]));
@@ -848,6 +881,7 @@ PFA_HD static inline int pfa_build_lattice(const PfaTeam &T, const PfaWS &W, con
     const int h = W.h, d = W.d;
     int bad = 0, rc;
     pfa_team_clear(T, W.flag);
+    if (T.lane == 0) PFA_OPS(2.0 * h * h * h + 4.0 * h * h);
     if (pfa_int64_safe(pfa_team_absmax(T, kappa, h * h * h), pfa_team_absmax(T, p, h), h)) {
         for (int e = T.lane; e < h * h; e += T.n) {
             int64_t s = 0;
@@ -892,6 +926,7 @@ PFA_HD static inline int pfa_build_lattice(const PfaTeam &T, const PfaWS &W, con
     if ((rc = pfa_lll_euclid(T, W, O, d, h))) return rc * 10 - 2;
     // BT[a][i] = sum_l Mbasis[i][l] O[a][l]  -> W2 (d, h)
     int64_t *BT = W.W2;
+    if (T.lane == 0) PFA_OPS(2.0 * d * h * h + 2.0 * d * d * h);
     if (pfa_int64_safe(pfa_team_absmax(T, Mbasis, h * h), pfa_team_absmax(T, O, d * h), h)) {
         for (int e = T.lane; e < d * h; e += T.n) {
             const int a = e / h, i = e % h;
@@ -1008,6 +1043,9 @@ PFA_HD static inline int pfa_cholesky(const PfaTeam &T, const int64_t *mat, int 
                                       double *X, double *phi_out, int *flag)
 {
     int bad = 0;
+    // the factor and U^-1 (d^3 / 6 multiply-adds each), the residual of U X
+    // (d^3 / 6 terms of 5) and the norms
+    if (T.lane == 0) PFA_OPS(1.5 * d * d * d + 2.0 * d * d);
     for (int e = T.lane; e < d * d; e += T.n) {
         int64_t m = mat[e];
         if (m > (INT64_C(1) << 53) || m < -(INT64_C(1) << 53)) bad = 1;

@@ -40,7 +40,7 @@ STAT_NAMES = ("nodes cand leaf points passes ms_prep ms_roots ms_search ms_post 
               "search_launches host_p queue_peak ms_wall kept expansions nonsingular undecided "
               "steps wait_cycles wave_cycles donated fetches sum_k sum_k_m0 sum_k_r sum_r sum_s "
               "sum_leaf work ms_finish ms_alloc ms_d2h attempts ms_upload waves "
-              "survivors graphs graph_hits graph_captures").split()
+              "survivors graphs graph_hits graph_captures prep_work prep_work_max").split()
 
 
 def _struct(name, spec):

@@ -126,3 +126,30 @@ def test_library_certifies_the_restated_set(headline_sample):
     cat = [np.concatenate([p[k] for p in pts]) for k in range(4)]
     assert canon_points(r["M"], r["Kn"], r["q"], r["pidx"], 20) == canon_points(*cat, 20)
     assert int(r["stats"]["nodes"]) == nodes
+
+
+def test_prep_work_count(headline_sample):
+    """The host path's count of the preparation's lane-ops: a certified
+    p-vector costs exactly the certificate (Z p: h^2 (h-1) multiply-adds, w:
+    h (h-1), the elimination: 4 per entry of each trailing block, sizes h
+    down to 1); the batch's total and maximum are the per-p-vector sums and
+    maximum."""
+    inst, pf, qmax = headline_sample
+    from test_gpu_parity import geoms_for_qmax
+    geoms, pgeo = geoms_for_qmax(inst, qmax)
+    r = gpu.run(geoms, pf, pgeo, mode="host", filter=False)
+    cert = np.flatnonzero(r["pcode"] == 1)
+    h = 20
+    closed = 2 * h * h * (h - 1) + 2 * h * (h - 1) + 4 * sum(m * m for m in range(1, h + 1))
+    per = []
+    for i in list(cert[:3]) + list(np.flatnonzero(r["pcode"] != 1)[:3]):
+        s = gpu.run(geoms, pf[i:i + 1], pgeo[i:i + 1], mode="host", filter=False)["stats"]
+        per.append((i, s["prep_work"]))
+        if r["pcode"][i] == 1:
+            assert s["prep_work"] == closed, i
+        else:
+            assert s["prep_work"] > 10 * closed, i      # two LLLs, the HNF and the factor
+    sub = [i for i, _ in per]
+    s = gpu.run(geoms, pf[sub], pgeo[sub], mode="host", filter=False, host_threads=2)["stats"]
+    assert s["prep_work"] == sum(w for _, w in per)
+    assert s["prep_work_max"] == max(w for _, w in per)
