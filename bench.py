@@ -303,6 +303,9 @@ def main():
                     help="fixed donation size instead of the autotuner's (with --waves-per-cu)")
     ap.add_argument("--split", type=int, default=0,
                     help="fixed split target (nodes per wavefront; < 0 none) with --waves-per-cu")
+    ap.add_argument("--select", default="", metavar="I,J,...",
+                    help="keep only these p-vectors of the workload (indices into its batch; "
+                         "for profiling single preparations)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only time a plain device copy of MIB MiB (the reference "
                          "kernel for HBM traffic counters) and exit")
@@ -323,6 +326,10 @@ def main():
         print(json.dumps(serving_bench(gpu, args)))
         return
     desc, geoms, pf, pgeo, items = workload(args.workload, args.n_per)
+    if args.select:
+        sel = [int(x) for x in args.select.split(",")]
+        pf, pgeo, items = pf[sel], pgeo[sel], [items[i] for i in sel]
+        desc += f"; only p-vectors {args.select}"
     print(f"# workload {args.workload}: {len(pf)} p-vectors, {len(geoms)} geometry entries",
           file=sys.stderr, flush=True)
     params = {}
