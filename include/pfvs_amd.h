@@ -201,6 +201,34 @@ void pfa_assemble_output_free(pfa_assemble_output *out);
 int pfa_coni_mu0(int h, const int64_t *kappa, int64_t n_p, const int64_t *ps, int threads,
                  int32_t *status, int64_t *num, int64_t *den, int64_t *K, char *err);
 
+/* coniZpK's lattice points (pfvs/coni.py:554-652, whose C kernel is
+ * pfvs/fp_kernel/coni_zpk.h, bound through fp_kernel.pyx's _coni_zpk_batch):
+ * for each direction (row i of ps, h entries, ps[i][0] = 0) the lattice
+ * points c of the coni PFVs with dilation delta > D0 -- c = j c1 + t k for
+ * every short K_r of Lambda (K_r^T S K_r < Q / D0) with j o > g D0 and
+ * g L sn M0^2 < sd j o (Q L - g j o kn), M0 >= M0min -- as rows M =
+ * Binter c, Kn = ZB c (h entries each) with q = c^T mat c and pidx = i, in
+ * direction order; a point can repeat and some of its PFVs can have
+ * delta <= D0 (the caller filters and deduplicates after the PFV
+ * assembly).  status[i]: 0 done; 1 no short-vector bound (s >= 0, A_rr
+ * singular, S not positive definite on Lambda); < 0 not decided here (a
+ * value beyond int128, a form too ill-conditioned for the FP64 search,
+ * more than max_out points for the direction): the caller searches that
+ * direction to its dilation bound instead, and none of its points are
+ * returned.  Host threads (threads <= 0: all cores).  The library
+ * mallocs the arrays; pfa_zpk_output_free frees them.  Returns 0, or 1
+ * with a message for bad arguments. */
+typedef struct {
+    int64_t n;                /* points */
+    int64_t *M, *Kn;          /* n x h */
+    int64_t *q, *pidx;        /* n */
+} pfa_zpk_output;
+
+int pfa_coni_zpk(int h, const int64_t *kappa, const int64_t *Mbasis, int64_t n_p, const int64_t *ps,
+                 int64_t Q, int64_t M0min, int64_t D0, int64_t max_out, int threads, int32_t *status,
+                 pfa_zpk_output *out, char *err);
+void pfa_zpk_output_free(pfa_zpk_output *out);
+
 /* Single-problem kernel batch (the contract of conipfv_kernel,
  * pfvs/fp_kernel/fp_kernel.pyx:258-318 / fp_kernel.h:25-63): for problem
  * k of dimension dims[k], mat (d*d), H (nh[k] rows of d, echelon form,

@@ -372,6 +372,146 @@ def _canonical(Ks, Ms, keys):
     return A[:, 1 + h:], A[:, 1:1 + h], A[:, 0]
 
 
+def _dilations(kappa, p_full, Ks, Ms, keys):
+    """The dilation delta of each PFV (K, M) of the direction p_full[key], as
+    exact (numerator, denominator) lists: (Z M)_r = delta K_r with Z =
+    kappa . p, so delta = gcd((Z M)_r) / gcd(K_r)."""
+    kap = util.to_pyints(np.asarray(kappa))
+    h = len(kap)
+    num, den = [], []
+    Zs = {}
+    for K, M, k in zip(Ks, Ms, keys):
+        k = int(k)
+        if k not in Zs:
+            p = [int(x) for x in p_full[k]]
+            Zs[k] = [[sum(kap[a][b][c] * p[c] for c in range(h)) for b in range(h)] for a in range(h)]
+        Z = Zs[k]
+        M = [int(x) for x in M]
+        ZM = [sum(Z[a][b] * M[b] for b in range(h)) for a in range(1, h)]
+        num.append(math.gcd(*ZM))
+        den.append(math.gcd(*[int(x) for x in K[1:]]))
+    return num, den
+
+
+def coniZpK(data, ps, D0, Q=None, M0min=13, n_jobs=-1, max_N_pfvs=1_000_000_000,
+            return_formal_pfvs=False, verbosity=0):
+    """The coni PFVs of each direction with dilation delta > D0
+    (pfvs/coni.py:554-652): with coniZpM(..., ellipsoid_dilation=D0) this
+    gives every coni PFV of the direction.  The lattice points come from the
+    library's short-vector enumeration (pfa_coni_zpk, host threads: every
+    short K_r of Lambda with K_r^T S K_r < Q / D0 and the points of each);
+    a direction it cannot handle exactly is searched by coniZpM up to its
+    dilation bound instead, and a direction without a bound raises
+    IncompleteSearchError.  Returns (Ks, Ms) grouped by p-vector in the
+    order of ps, sorted by (M, K) within one, each PFV once."""
+    from .dilation import coni_dilation_bounds
+    _check_inputs(data, ps, 1)
+    if int(D0) != D0 or D0 < 1:
+        raise ValueError(f"D0 must be a positive integer, got {D0}")
+    if return_formal_pfvs:
+        raise NotImplementedError("PFV objects are not provided; use (Ks, Ms)")
+    D0 = int(D0)
+    ps = np.array(ps)
+    kappa, Mbasis, h11 = data.kappa_cob, data.M_lattice(), data.h11
+    Q = h11 + data.h21 + 4 if Q is None else Q
+    p_int = util.as_integral(ps)
+    if p_int.dtype == object:
+        raise IncompleteSearchError("p-vector entries exceed int64")
+    p_full = np.concatenate([np.zeros((len(ps), 1), np.int64), p_int.astype(np.int64)], axis=1)
+    threads = 0 if n_jobs in (-1, None) else max(1, int(n_jobs))
+    linmin = math.ceil(M0min - 1e-9)
+    M, Kn, q, pidx, st = gpu.coni_zpk_points(kappa, Mbasis, p_full, Q, linmin, D0,
+                                             max_out=min(int(max_N_pfvs), 10**8), threads=threads)
+    pieces = [_pfvs_from_points(M.T, Kn.T, q, pidx, kappa, h11, Q, M0min)]
+    bad = np.flatnonzero(st != 0)
+    if verbosity >= 1:
+        print(f"ZpK: {len(q)} lattice points for {len(ps)} p-vectors ({len(bad)} searched by "
+              f"coniZpM to their bound instead)")
+    if len(bad):
+        for i, b in zip(bad, coni_dilation_bounds(p_int[bad], kappa, Q)):
+            if b is None:
+                raise IncompleteSearchError(
+                    f"p={ps[i].tolist()}: its PFVs above D0 cannot be enumerated (s >= 0, or S "
+                    f"not positive definite on the lattice orthogonal to p)")
+            K1, M1 = coniZpM(data, ps[i][None], Q=Q, M0min=M0min,
+                             ellipsoid_dilation=max(math.ceil(b), D0), device="cpu", n_jobs=1,
+                             max_N_pfvs=max_N_pfvs)
+            pieces.append((K1, M1, np.full(len(K1), i, dtype=np.int64)))
+    pieces = [pc for pc in pieces if len(pc[2])]
+    if not pieces:
+        z = np.zeros((0, h11), np.int64)
+        return z, z.copy()
+    Ks = np.vstack([pc[0] for pc in pieces])
+    Ms = np.vstack([pc[1] for pc in pieces])
+    keys = np.concatenate([pc[2] for pc in pieces])
+    num, den = _dilations(kappa, p_full, Ks, Ms, keys)
+    above = np.array([a > D0 * b for a, b in zip(num, den)], dtype=bool)
+    Ks, Ms, _ = _canonical(Ks[above], Ms[above], keys[above])
+    return Ks, Ms
+
+
+def _zpm_batched(data, ps, p_full, idx, dil, Q, M0min, device, threads, max_N_pfvs, verbosity):
+    """coniZpM on the directions idx, each at its own dilation dil[idx], in one
+    batched search (one geometry entry per distinct dilation): (Ks, Ms, keys)
+    with keys indexing ps."""
+    kappa, Mbasis, h11 = data.kappa_cob, data.M_lattice(), data.h11
+    idx = np.asarray(idx, dtype=np.int64)
+    if not len(idx):
+        z = np.zeros((0, h11), np.int64)
+        return z, z.copy(), np.zeros(0, np.int64)
+    Ds, pgeo = np.unique(np.asarray(dil, dtype=np.float64)[idx], return_inverse=True)
+    geoms = [{"kappa": kappa, "Mbasis": Mbasis, "Q": Q, "dilation": float(D), "M0min": M0min}
+             for D in Ds]
+    qmax = np.array([gpu.qmax_of(Q, float(dil[i])) for i in idx], dtype=np.int64)
+    on_gpu = _use_gpu(device, h11, len(idx))
+    call = dict(filter=True, sort=True, keep_points="survivors", assemble=True,
+                host_threads=threads)
+    pgeo = pgeo.astype(np.int32)
+    try:
+        r = gpu.run(geoms, p_full[idx], pgeo, mode="batched" if on_gpu else "host", **call)
+    except RuntimeError as e:
+        if not on_gpu or device == "gpu":
+            raise
+        warnings.warn(f"GPU path failed ({e}); using the CPU", RuntimeWarning, stacklevel=3)
+        r = gpu.run(geoms, p_full[idx], pgeo, mode="host", **call)
+    Ks, Ms, keys = assemble(r, ps[idx], p_full[idx], kappa, Mbasis, Q, qmax, M0min, max_N_pfvs,
+                            verbosity, with_keys=True)
+    return Ks, Ms, idx[np.asarray(keys, dtype=np.int64)]
+
+
+def _calibrate(data, ps, p_full, b, has, Q, M0min, device, threads, n_sample=64):
+    """A CostModel measured on a seeded sample of the bounded directions:
+    coniZpM's time per direction at the split candidates and at the upper
+    bounds, and coniZpK's above each split candidate (the quartiles of the
+    bounds)."""
+    import time
+    from .dilation import CostModel
+    idx = np.flatnonzero(has)
+    s = np.sort(np.random.default_rng(0).choice(idx, min(len(idx), n_sample), replace=False))
+    bs = b[s]
+    D0s = sorted({max(1, int(x)) for x in np.quantile(bs, [0.25, 0.5, 0.75])})
+    Ds = sorted(set(D0s) | {max(1, int(x)) for x in np.quantile(bs, [0.9, 1.0])})
+    dil = np.zeros(len(ps))
+    _zpm_batched(data, ps, p_full, s[:1], np.full(len(ps), float(Ds[0])), Q, M0min, device,
+                 threads, 10**9, 0)                       # (warm-up)
+    G = []
+    for D in Ds:
+        dil[:] = D
+        t = time.perf_counter()
+        _zpm_batched(data, ps, p_full, s, dil, Q, M0min, device, threads, 10**9, 0)
+        G.append((time.perf_counter() - t) / len(s))
+    linmin = math.ceil(M0min - 1e-9)
+    K = []
+    for D0 in D0s:
+        t = time.perf_counter()
+        gpu.coni_zpk_points(data.kappa_cob, data.M_lattice(), p_full[s], Q, linmin, D0,
+                            threads=threads)
+        K.append((time.perf_counter() - t) / len(s))
+    if len(Ds) == 1:
+        Ds, G = Ds[:1], G[:1]
+    return CostModel(Ds, G, D0s, K, shared=not _use_gpu(device, data.h11, len(s)))
+
+
 def coniZp(data, ps, Q=None, M0min=13, fallback_dilation=1, n_jobs=-1, device="auto",
            cost_model=None, max_N_pfvs=1_000_000_000, return_formal_pfvs=False, verbosity=0,
            max_dilation=None):
@@ -383,23 +523,27 @@ def coniZp(data, ps, Q=None, M0min=13, fallback_dilation=1, n_jobs=-1, device="a
     rounded up onto a 2^(1/16) grid; all directions go through one batched
     search with one geometry entry per distinct dilation.  A direction
     without a bound is searched at fallback_dilation and reported
-    incomplete.  The reference can instead split a direction with a large
-    bound into ZpM up to a dilation D0 plus a short-vector search above D0
-    (coniZpK), chosen by measured costs (cost_model); that split is not
-    provided here, so every bounded direction is searched to its bound (the
-    same PFVs).  max_dilation (optional) caps the search: a direction whose
-    bound exceeds it is searched to max_dilation and reported incomplete.
+    incomplete.  cost_model (a dilation.CostModel, or "calibrate" to
+    measure one on a sample of the directions first) routes the directions
+    as the reference's exhaustive search does (pfvs/coni.py:784-855): a
+    direction with b <= max(b*, D0) goes to its bound, the others are split
+    at D0 -- coniZpM up to D0 in the same batched search, coniZpK above D0
+    on host threads while it runs -- and a direction coniZpK leaves
+    undecided is searched to its bound afterwards.  Without a cost model
+    every bounded direction is searched to its bound (the same PFVs).
+    max_dilation (optional) caps the search: a direction whose bound
+    exceeds it is searched to max_dilation and reported incomplete.
     Output as coniZpM's rows, sorted by p-vector, then (M, K), each PFV
     once; complete[i] whether direction i was searched exhaustively."""
+    from concurrent.futures import ThreadPoolExecutor
     from .dilation import coni_dilation_bound_ceils
     if fallback_dilation <= 0:
         raise ValueError(f"fallback_dilation must be > 0, got {fallback_dilation}.")
     _check_inputs(data, ps, 1)
-    if cost_model is not None:
-        raise NotImplementedError("cost_model routes directions to coniZpK, which is not "
-                                  "provided: every bounded direction is searched to its bound")
     if return_formal_pfvs:
         raise NotImplementedError("PFV objects are not provided; use (Ks, Ms)")
+    if isinstance(cost_model, str) and cost_model != "calibrate":
+        raise ValueError(f"cost_model must be a CostModel, 'calibrate' or None, got {cost_model!r}")
     ps = np.array(ps)
     kappa, h11 = data.kappa_cob, data.h11
     Mbasis = data.M_lattice()
@@ -414,29 +558,50 @@ def coniZp(data, ps, Q=None, M0min=13, fallback_dilation=1, n_jobs=-1, device="a
     dil = np.where(has, _grid_up(np.where(has, b, 1)), 0).astype(np.float64)
     dil[~has] = fallback_dilation
     complete = has.copy()
+    capped = np.zeros(len(ps), dtype=bool)
     if max_dilation is not None:
-        over = has & (b > max_dilation)
-        dil[over] = max_dilation
-        complete &= ~over
-    Ds, pgeo = np.unique(dil, return_inverse=True)
-    geoms = [{"kappa": kappa, "Mbasis": Mbasis, "Q": Q, "dilation": float(D), "M0min": M0min}
-             for D in Ds]
-    qmax = np.array([gpu.qmax_of(Q, float(D)) for D in dil], dtype=np.int64)
+        capped = has & (b > max_dilation)
+        dil[capped] = max_dilation
+        complete &= ~capped
+    split = np.zeros(len(ps), dtype=bool)
+    D0 = 0
+    if cost_model is not None and np.any(has & ~capped):
+        if isinstance(cost_model, str):
+            cost_model = _calibrate(data, ps, p_full, b, has & ~capped, Q, M0min, device, threads)
+        b_star, D0, _ = cost_model.route(np.where(has & ~capped, b, np.inf))
+        D0 = int(D0)
+        split = has & ~capped & (b > max(b_star, D0))
+        dil[split] = D0
+    idx_k = np.flatnonzero(split)
     if verbosity >= 1:
         print(f"coniZp: {len(ps)} p-vectors, {int(np.sum(~has))} without a bound (searched at "
-              f"dilation {fallback_dilation}), {len(Ds)} distinct dilations up to {Ds.max():g}")
-    on_gpu = _use_gpu(device, h11, len(ps))
-    call = dict(filter=True, sort=True, keep_points="survivors", assemble=True,
-                host_threads=threads)
-    pgeo = pgeo.astype(np.int32)
-    try:
-        r = gpu.run(geoms, p_full, pgeo, mode="batched" if on_gpu else "host", **call)
-    except RuntimeError as e:
-        if not on_gpu or device == "gpu":
-            raise
-        warnings.warn(f"GPU path failed ({e}); using the CPU", RuntimeWarning, stacklevel=2)
-        r = gpu.run(geoms, p_full, pgeo, mode="host", **call)
-    Ks, Ms, keys = assemble(r, ps, p_full, kappa, Mbasis, Q, qmax, M0min, max_N_pfvs, verbosity,
-                            with_keys=True)
+              f"dilation {fallback_dilation}), {len(idx_k)} split at D0 = {D0}, the rest to their "
+              f"bound ({len(np.unique(dil))} distinct dilations up to {dil.max():g})")
+    linmin = math.ceil(M0min - 1e-9)
+    with ThreadPoolExecutor(1) as ex:
+        fut = ex.submit(gpu.coni_zpk_points, kappa, Mbasis, p_full[idx_k], Q, linmin, D0,
+                        min(int(max_N_pfvs), 10**8), threads) if len(idx_k) else None
+        pieces = [_zpm_batched(data, ps, p_full, np.arange(len(ps)), dil, Q, M0min, device,
+                               threads, max_N_pfvs, verbosity)]
+        zk = fut.result() if fut is not None else None
+    if zk is not None:
+        M, Kn, q, pidx, st = zk
+        pieces.append(_pfvs_from_points(M.T, Kn.T, q, idx_k[pidx], kappa, h11, Q, M0min))
+        redo = idx_k[st != 0]
+        if len(redo):
+            dil2 = np.zeros(len(ps))
+            dil2[redo] = _grid_up(b[redo])
+            pieces.append(_zpm_batched(data, ps, p_full, redo, dil2, Q, M0min, device, threads,
+                                       max_N_pfvs, verbosity))
+        if verbosity >= 1:
+            print(f"coniZp: coniZpK gave {len(q)} lattice points, {len(redo)} split directions "
+                  f"searched to their bound instead")
+    pieces = [pc for pc in pieces if len(pc[2])]
+    if not pieces:
+        z = np.zeros((0, h11), np.int64)
+        return z, z.copy(), complete
+    Ks = np.vstack([pc[0] for pc in pieces])
+    Ms = np.vstack([pc[1] for pc in pieces])
+    keys = np.concatenate([pc[2] for pc in pieces])
     Ks, Ms, _ = _canonical(Ks, Ms, keys)
     return Ks, Ms, complete

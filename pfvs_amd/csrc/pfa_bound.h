@@ -37,6 +37,7 @@ struct PfaBoundForm {
     int64_t V[PFA_BOUND_MAXM * PFA_BOUND_MAXM];    // (n, n): rows = reduced vectors
     int64_t Gr[PFA_BOUND_MAXM * PFA_BOUND_MAXM];   // (n, n): V G V^T (LLL-reduced)
     int64_t L;
+    pfa_i128 sn, sd;                                // -s = sn / sd, sd > 0, in lowest terms
 };
 
 // a * b + c, 1 on overflow
@@ -258,6 +259,12 @@ static inline int pfa_bound_form(int h, const int64_t *kappa, const int64_t *p_r
     for (int i = 0; i < m; ++i)
         if (pfa_bnd_muladd((pfa_i128)A[(i + 1) * h], Y[i * nc + n], sn, &sn)) return -1;
     if (sn != 0 && ((sn < 0) != (d < 0))) return 1;            // s > 0
+    {
+        pfa_i128 a = d < 0 ? -sn : sn, b = d < 0 ? -d : d;
+        const pfa_u128 gs = pfa_gcd128(pfa_abs128(a), (pfa_u128)b);
+        F.sn = a / (pfa_i128)gs;
+        F.sd = b / (pfa_i128)gs;
+    }
     // S on Lambda = Gn / d, Gn = -O Y[:, :n]; reduced by g = gcd(Gn, d)
     pfa_i128 Gn[PFA_BOUND_MAXM * PFA_BOUND_MAXM];
     pfa_u128 g = pfa_abs128(d);

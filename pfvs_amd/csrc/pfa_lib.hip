@@ -4127,6 +4127,391 @@ extern "C" int pfa_coni_mu0(int h, const int64_t *kappa, int64_t n_p, const int6
     return 0;
 }
 
+// ------------------------------------------------------------ coniZpK
+// The lattice points behind the coni PFVs of a direction p_hat with
+// dilation delta > D0 (the reference's coniZpK, pfvs/coni.py:554-652, with
+// its C kernel pfvs/fp_kernel/coni_zpk.h).  Notation as pfa_bound.h:
+// A = kappa . p_hat, S = -A_rr^-1, -s = sn / sd, Lambda = {K : p_r . K = 0}
+// of rank n = m - 1.  A PFV with p = p_hat / delta has K_r in Lambda and
+// tadpole Q = M0 eps + (M0^2 / delta)(-s) + delta K_r^T S K_r with eps > 0,
+// so for -s > 0 the dilation delta > D0 forces K_r^T S K_r < Q / D0: K_r is
+// one of finitely many short vectors of Lambda (the library's search over
+// the reduced form Gr = L S in reduced coordinates x: x^T Gr x D0 < Q L).
+// The lattice points c of the preparation (M = Binter c, Kn = ZB c) satisfy
+// W c = delta K_r with W = ZB[1:].  For K_r = g K' (K' primitive) and o the
+// order of K' in Lambda / Gamma, Gamma = W Z^m, the c with W c parallel to
+// K_r are c = j c1 + t k (W c1 = o K', W k = 0) with delta = j o / g:
+//   delta > D0                <=>  j o > g D0,
+//   the tadpole with eps > 0  <=>  g L sn M0^2 < sd j o (Q L - g j o kn),
+// kn = L K'^T S K', and M0 = j (l . c1) + t (l . k) >= M0min (l = Binter[0]),
+// an arithmetic progression of M0 for each j.  Gamma and k come from a
+// column echelon form of W in Lambda coordinates, Yc E = [Hc | 0] (E
+// unimodular, Hc lower triangular with a positive diagonal): k = E e_{m-1};
+// with Hadj = Hdet Hc^-1 and y' the Lambda coordinates of K',
+// o = Hdet / gcd(Hdet, Hadj y') and c1 = E [Hadj y' / gcd(...); 0].  All
+// exact (int128, checked); a value beyond it leaves the direction to the
+// caller.  Status per direction: 0 done; 1 no short-vector bound (s >= 0,
+// A_rr singular, S not positive definite on Lambda); < 0 undecided here
+// (the caller searches that direction to its dilation bound instead).
+struct ZpkPts {
+    std::vector<int64_t> M, Kn, q, pidx;
+};
+
+static inline int zpk_mul(pfa_i128 a, pfa_i128 b, pfa_i128 *r) { return pfa_mul_ovf(a, b, r); }
+
+// floor(a / b), b > 0
+static inline pfa_i128 zpk_fdiv(pfa_i128 a, pfa_i128 b)
+{
+    pfa_i128 q = a / b;
+    if ((a % b != 0) && (a < 0)) --q;
+    return q;
+}
+
+// s a + t b = g = gcd(a, b) >= 0
+static void zpk_xgcd(pfa_i128 a, pfa_i128 b, pfa_i128 *s, pfa_i128 *t, pfa_i128 *g)
+{
+    pfa_i128 r0 = a, r1 = b, s0 = 1, s1 = 0, t0 = 0, t1 = 1;
+    while (r1 != 0) {
+        const pfa_i128 q = r0 / r1;
+        pfa_i128 x = r0 - q * r1; r0 = r1; r1 = x;
+        x = s0 - q * s1; s0 = s1; s1 = x;
+        x = t0 - q * t1; t0 = t1; t1 = x;
+    }
+    if (r0 < 0) { r0 = -r0; s0 = -s0; t0 = -t0; }
+    *s = s0; *t = t0; *g = r0;
+}
+
+static int zpk_one(int h, const int64_t *kappa, const int64_t *Mbasis, const int64_t *p, int64_t Q,
+                   int64_t M0min, int64_t D0, int64_t pidx, int64_t room, ZpkPts &out)
+{
+    const int m = h - 1, n = m - 1;
+    if (h < 3 || m > PFA_DMAX) return -20;
+    if (D0 < 1 || M0min < 1 || Q < 1) return -21;
+    std::unique_ptr<PfaBoundForm> Fp(new PfaBoundForm);
+    PfaBoundForm &F = *Fp;
+    int st = pfa_bound_form(h, kappa, p + 1, F);
+    if (st == 1) return 1;
+    if (st) return -30 + st;
+    if (F.sn <= 0) return 1;                                    // -s <= 0: no bound
+    // the preparation's lattice: Binter (h x m), ZB (h x m), mat (m x m)
+    std::vector<char> ws(pfa_ws_bytes(h) + 64);
+    PfaWS Wk;
+    pfa_ws_carve(Wk, ws.data(), ws.data() + pfa_ws_lds_bytes(h), h);
+    const PfaTeam T{0, 1};
+    if (pfa_build_lattice(T, Wk, kappa, Mbasis, p)) return -40;
+    const std::vector<int64_t> Bi(Wk.Bi, Wk.Bi + h * m), ZB(Wk.ZB, Wk.ZB + h * m),
+        mat(Wk.mat, Wk.mat + m * m);
+    // Lambda coordinates of W's columns: U unimodular with U p_r = (g, 0, ...)
+    // (the rows 1.. are F.O); z = U^-T v has z_0 = 0 for v in Lambda
+    std::vector<int64_t> U(m * m);
+    {
+        int flag = 0, rc = pfa_unimodular(T, p + 1, m, U.data(), &flag);
+        if (rc == -1) {
+            pfa_i128 x[PFA_DMAX], r[PFA_DMAX];
+            int ord[PFA_DMAX];
+            rc = pfa_unimodular_reduced(T, p + 1, m, U.data(), x, r, ord, &flag);
+        }
+        if (rc) return -41;
+        for (int a = 0; a < n; ++a)
+            for (int i = 0; i < m; ++i)
+                if (F.O[a * m + i] != U[(a + 1) * m + i]) return -42;   // cannot happen
+    }
+    const int w = 2 * m;
+    std::vector<pfa_i128> Mx((size_t)m * w), Y((size_t)m * m);
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) {
+            Mx[i * w + j] = U[j * m + i];
+            Mx[i * w + m + j] = ZB[(i + 1) * m + j];
+        }
+    pfa_i128 det;
+    if (pfa_bnd_solve(Mx.data(), m, w, &det, Y.data()) || (det != 1 && det != -1)) return -43;
+    std::vector<pfa_i128> A((size_t)n * m), E((size_t)m * m, 0);
+    for (int c = 0; c < m; ++c) {
+        if (Y[c] != 0) return -44;                              // (W c lies in Lambda)
+        for (int a = 0; a < n; ++a) A[a * m + c] = det * Y[(a + 1) * m + c];
+    }
+    for (int i = 0; i < m; ++i) E[i * m + i] = 1;
+    // column operations [col c1, col c2] <- [col c1, col c2] [[a11, a12], [a21, a22]]
+    auto colop = [&](int c1, int c2, pfa_i128 a11, pfa_i128 a12, pfa_i128 a21, pfa_i128 a22) -> int {
+        auto one = [&](std::vector<pfa_i128> &X, int rows) -> int {
+            for (int r = 0; r < rows; ++r) {
+                const pfa_i128 x = X[r * m + c1], y = X[r * m + c2];
+                pfa_i128 u, v, s1, s2;
+                if (zpk_mul(a11, x, &u) || zpk_mul(a21, y, &v) || pfa_add_ovf(u, v, &s1)) return 1;
+                if (zpk_mul(a12, x, &u) || zpk_mul(a22, y, &v) || pfa_add_ovf(u, v, &s2)) return 1;
+                X[r * m + c1] = s1;
+                X[r * m + c2] = s2;
+            }
+            return 0;
+        };
+        return one(A, n) || one(E, m);
+    };
+    for (int r = 0; r < n; ++r) {
+        for (int c = r + 1; c < m; ++c) {
+            if (A[r * m + c] == 0) continue;
+            pfa_i128 s, t, g;
+            zpk_xgcd(A[r * m + r], A[r * m + c], &s, &t, &g);
+            if (colop(r, c, s, -A[r * m + c] / g, t, A[r * m + r] / g)) return -45;
+        }
+        if (A[r * m + r] == 0) return -46;                     // W of rank below n
+        if (A[r * m + r] < 0 && colop(r, r == 0 ? 1 : 0, -1, 0, 0, 1)) return -45;
+        for (int c = 0; c < r; ++c) {                           // left of the pivot into [0, pivot)
+            const pfa_i128 q = zpk_fdiv(A[r * m + c], A[r * m + r]);
+            if (q != 0 && colop(c, r, 1, 0, -q, 1)) return -45;
+        }
+    }
+    std::vector<int64_t> k(m);
+    for (int r = 0; r < n; ++r)
+        if (A[r * m + m - 1] != 0) return -47;                  // cannot happen
+    for (int i = 0; i < m; ++i) {
+        if (!pfa_fits64(E[i * m + m - 1])) return -45;
+        k[i] = (int64_t)E[i * m + m - 1];
+    }
+    for (int i = 0; i < m; ++i) {                               // W k = 0, exactly
+        pfa_i128 s = 0;
+        for (int c = 0; c < m; ++c)
+            if (pfa_mac64(&s, ZB[(i + 1) * m + c], k[c])) return -45;
+        if (s != 0) return -47;
+    }
+    // Hdet and Hadj = Hdet Hc^-1 (lower triangular), by forward substitution
+    pfa_i128 Hdet = 1;
+    for (int r = 0; r < n; ++r)
+        if (zpk_mul(Hdet, A[r * m + r], &Hdet)) return -48;
+    std::vector<pfa_i128> Hadj((size_t)n * n, 0);
+    for (int j = 0; j < n; ++j)
+        for (int i = j; i < n; ++i) {
+            pfa_i128 s = i == j ? Hdet : 0;
+            for (int c = j; c < i; ++c) {
+                pfa_i128 t;
+                if (zpk_mul(A[i * m + c], Hadj[c * n + j], &t) || pfa_sub_ovf(s, t, &s)) return -48;
+            }
+            if (s % A[i * m + i] != 0) return -48;              // cannot happen (Hdet Hc^-1 is integral)
+            Hadj[i * n + j] = s / A[i * m + i];
+        }
+    pfa_i128 lk = 0;
+    for (int c = 0; c < m; ++c)
+        if (pfa_mac64(&lk, Bi[c], k[c])) return -49;
+    if (lk == 0) return -49;                                    // M0 would not depend on t
+    const pfa_i128 alk = lk < 0 ? -lk : lk;
+    // the short vectors: x^T Gr x <= (Q L - 1) / D0, through the search
+    pfa_i128 QL, coefL;
+    if (zpk_mul((pfa_i128)Q, (pfa_i128)F.L, &QL)) return -50;
+    const pfa_i128 qs = (QL - 1) / D0;
+    if (qs < 1) return 0;
+    if (!pfa_fits64(qs)) return -50;
+    if (zpk_mul((pfa_i128)F.L, F.sn, &coefL)) return -50;      // L sn (times g below)
+    const RecL L = rec_layout(n);
+    std::vector<int64_t> rec((size_t)L.words, 0);
+    int64_t *r = rec.data();
+    int32_t *ri = (int32_t *)r;
+    ri[0] = PFA_HOST;
+    ri[1] = n;
+    r[1] = (int64_t)qs;
+    r[2] = 1;
+    r[3] = INT64_MIN;
+    ri[20] = -1;
+    ri[23] = 1;
+    ri[24] = n + 1;
+    int64_t mm = 0;
+    for (int e = 0; e < n * n; ++e) {
+        r[L.mat + e] = F.Gr[e];
+        mm = std::max<int64_t>(mm, F.Gr[e] < 0 ? -F.Gr[e] : F.Gr[e]);
+    }
+    r[16] = mm;
+    std::vector<char> fws((size_t)2 * n * n * 8 + 64);
+    factor_one(T, fws.data(), r, L);
+    if (rec_status(r) != PFA_OK) return -51;                   // ill-conditioned for FP64
+    PfaProb P = rec_view(r, L);
+    std::vector<PfaNode> stk(PFA_DMAX);
+    int32_t cc[PFA_DMAX];
+    PfaCounts cnt{0, 0, 0};
+    const size_t n0 = out.q.size();
+    int bad = 0;
+    // the points of one short vector x (q = x^T Gr x)
+    auto per_K = [&](const int32_t *x, int64_t val) -> int {
+        pfa_i128 z[PFA_DMAX], Kp[PFA_DMAX], u[PFA_DMAX], c1[PFA_DMAX];
+        pfa_u128 gu = 0;
+        for (int a = 0; a < n; ++a) {                           // Lambda coordinates V^T x
+            pfa_i128 s = 0;
+            for (int b = 0; b < n; ++b)
+                if (pfa_mac64(&s, F.V[b * n + a], x[b])) return -60;
+            z[a] = s;
+            gu = pfa_gcd128(gu, pfa_abs128(s));
+        }
+        const pfa_i128 g = (pfa_i128)gu;
+        pfa_i128 g2;
+        if (g <= 0 || zpk_mul(g, g, &g2) || val % g2 != 0) return -61;
+        const pfa_i128 kn = val / g2;
+        for (int a = 0; a < n; ++a) {                           // y': K' in Lambda coordinates
+            z[a] /= g;
+            if (!pfa_fits64(z[a])) return -62;
+        }
+        pfa_u128 og = (pfa_u128)Hdet;
+        for (int i = 0; i < n; ++i) {
+            pfa_i128 s = 0;
+            for (int c = 0; c <= i; ++c) {
+                pfa_i128 t;
+                if (zpk_mul(Hadj[i * n + c], z[c], &t) || pfa_add_ovf(s, t, &s)) return -62;
+            }
+            u[i] = s;
+            og = pfa_gcd128(og, pfa_abs128(s));
+        }
+        const pfa_i128 o = Hdet / (pfa_i128)og;
+        for (int i = 0; i < m; ++i) {                           // c1 = E [u / og; 0]
+            pfa_i128 s = 0;
+            for (int a = 0; a < n; ++a) {
+                pfa_i128 t;
+                if (zpk_mul(E[i * m + a], u[a] / (pfa_i128)og, &t) || pfa_add_ovf(s, t, &s)) return -62;
+            }
+            c1[i] = s;
+        }
+        for (int i = 0; i < m; ++i) {                           // K' = O^T y' and W c1 = o K'
+            pfa_i128 s = 0, t = 0;
+            for (int a = 0; a < n; ++a)
+                if (pfa_mac64(&s, F.O[a * m + i], (int64_t)z[a])) return -62;
+            for (int c = 0; c < m; ++c) {
+                pfa_i128 e;
+                if (zpk_mul((pfa_i128)ZB[(i + 1) * m + c], c1[c], &e) || pfa_add_ovf(t, e, &t)) return -62;
+            }
+            pfa_i128 oK;
+            if (zpk_mul(o, s, &oK) || oK != t) return -63;      // cannot happen
+            Kp[i] = s;
+        }
+        pfa_i128 lc1 = 0;
+        for (int c = 0; c < m; ++c) {
+            pfa_i128 t;
+            if (zpk_mul((pfa_i128)Bi[c], c1[c], &t) || pfa_add_ovf(lc1, t, &lc1)) return -62;
+        }
+        // j: j o > g D0 and g j o kn < Q L
+        pfa_i128 go, gokn, gD0, coef, sdo;
+        if (zpk_mul(g, o, &go) || zpk_mul(go, kn, &gokn) || zpk_mul(g, (pfa_i128)D0, &gD0) ||
+            zpk_mul(g, coefL, &coef) || zpk_mul(F.sd, o, &sdo))
+            return -64;
+        const pfa_i128 jlo = zpk_fdiv(gD0, o) + 1, jhi = (QL - 1) / gokn;
+        if (jlo > jhi) return 0;
+        if (jhi - jlo > ((pfa_i128)1 << 26)) return -65;       // too many dilations to walk here
+        const double coef_d = (double)coef, sdo_d = (double)sdo;
+        for (pfa_i128 j = jlo; j <= jhi; ++j) {
+            pfa_i128 diff, R, jl;
+            if (zpk_mul(gokn, j, &diff) || pfa_sub_ovf(QL, diff, &diff)) return -64;
+            if (zpk_mul(j, lc1, &jl)) return -64;
+            // the smallest M0 >= M0min with M0 = j lc1 (mod |lk|)
+            pfa_i128 off = (jl - (pfa_i128)M0min) % alk;
+            if (off < 0) off += alk;
+            pfa_i128 M0 = (pfa_i128)M0min + off;
+            const double M0d = (double)M0;
+            if (M0d * M0d * coef_d > sdo_d * (double)j * (double)diff * (1 + 1e-9) + 1.0) continue;
+            if (zpk_mul(sdo, j, &R) || zpk_mul(R, diff, &R)) return -64;
+            for (;; M0 += alk) {
+                pfa_i128 lhs;
+                if (zpk_mul(M0, M0, &lhs) || zpk_mul(lhs, coef, &lhs) || lhs >= R) break;
+                const pfa_i128 tt = (M0 - jl) / lk;             // exact: M0 = j lc1 (mod lk)
+                if ((int64_t)(out.q.size() - n0) >= room) return -66;
+                int64_t c[PFA_DMAX];
+                for (int i = 0; i < m; ++i) {
+                    pfa_i128 a, b;
+                    if (zpk_mul(j, c1[i], &a) || zpk_mul(tt, (pfa_i128)k[i], &b) || pfa_add_ovf(a, b, &a) ||
+                        !pfa_fits64(a))
+                        return -67;
+                    c[i] = (int64_t)a;
+                }
+                int64_t Mr[PFA_DMAX + 1], Kr[PFA_DMAX + 1];
+                for (int i = 0; i < h; ++i) {
+                    pfa_i128 a = 0, b = 0;
+                    for (int e = 0; e < m; ++e)
+                        if (pfa_mac64(&a, Bi[i * m + e], c[e]) || pfa_mac64(&b, ZB[i * m + e], c[e])) return -67;
+                    if (!pfa_fits64(a) || !pfa_fits64(b)) return -67;
+                    Mr[i] = (int64_t)a;
+                    Kr[i] = (int64_t)b;
+                }
+                if (Mr[0] != (int64_t)M0) return -68;           // cannot happen
+                pfa_i128 q = 0;
+                for (int i = 0; i < m; ++i) {
+                    pfa_i128 s = 0, t;
+                    for (int e = 0; e < m; ++e)
+                        if (pfa_mac64(&s, mat[i * m + e], c[e])) return -67;
+                    if (zpk_mul(s, (pfa_i128)c[i], &t) || pfa_add_ovf(q, t, &q)) return -67;
+                }
+                if (!pfa_fits64(q)) return -67;
+                out.M.insert(out.M.end(), Mr, Mr + h);
+                out.Kn.insert(out.Kn.end(), Kr, Kr + h);
+                out.q.push_back((int64_t)q);
+                out.pidx.push_back(pidx);
+            }
+        }
+        (void)Kp;
+        return 0;
+    };
+    auto emit = [&](const int32_t *x, int64_t q, uint64_t) -> int {
+        if (q <= 0) return 0;                                   // x = 0
+        const int rc = per_K(x, q);
+        if (rc) { bad = rc; return 1; }
+        return 0;
+    };
+    const int rr = pfa_dfs(P, stk.data(), cc, cnt, emit);
+    if (bad) return bad;
+    if (rr) return -52;
+    return 0;
+}
+
+extern "C" int pfa_coni_zpk(int h, const int64_t *kappa, const int64_t *Mbasis, int64_t n_p,
+                            const int64_t *ps, int64_t Q, int64_t M0min, int64_t D0, int64_t max_out,
+                            int threads, int32_t *status, pfa_zpk_output *out, char *err)
+{
+    memset(out, 0, sizeof(*out));
+    if (h < 3 || h > PFA_MAX_H11 || n_p < 0 || D0 < 1 || M0min < 1 || Q < 1 || max_out < 0) {
+        snprintf(err, 256, "pfa_coni_zpk: bad arguments (h11 %d of 3..%d, %lld p-vectors, D0 %lld, "
+                 "M0min %lld, Q %lld)", h, PFA_MAX_H11, (long long)n_p, (long long)D0,
+                 (long long)M0min, (long long)Q);
+        return 1;
+    }
+    const int nth = threads > 0 ? threads
+                                : std::max(1, std::min(64, (int)std::thread::hardware_concurrency()));
+    std::vector<ZpkPts> part((size_t)n_p);
+    parallel_chunks(n_p, 4, nth, [&](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; ++i) {
+            ZpkPts &P = part[i];
+            const int st = zpk_one(h, kappa, Mbasis, ps + i * h, Q, M0min, D0, i, max_out, P);
+            if (st) P = ZpkPts();
+            status[i] = st;
+        }
+    });
+    int64_t n = 0;
+    for (const ZpkPts &P : part) n += (int64_t)P.q.size();
+    out->n = n;
+    const size_t rows = (size_t)std::max<int64_t>(n, 1);
+    out->M = (int64_t *)malloc(8 * rows * h);
+    out->Kn = (int64_t *)malloc(8 * rows * h);
+    out->q = (int64_t *)malloc(8 * rows);
+    out->pidx = (int64_t *)malloc(8 * rows);
+    if (!out->M || !out->Kn || !out->q || !out->pidx) {
+        pfa_zpk_output_free(out);
+        snprintf(err, 256, "pfa_coni_zpk: out of host memory for %lld points", (long long)n);
+        return 1;
+    }
+    int64_t k = 0;
+    for (const ZpkPts &P : part) {
+        const int64_t c = (int64_t)P.q.size();
+        if (!c) continue;
+        memcpy(out->M + k * h, P.M.data(), 8 * (size_t)c * h);
+        memcpy(out->Kn + k * h, P.Kn.data(), 8 * (size_t)c * h);
+        memcpy(out->q + k, P.q.data(), 8 * (size_t)c);
+        memcpy(out->pidx + k, P.pidx.data(), 8 * (size_t)c);
+        k += c;
+    }
+    return 0;
+}
+
+extern "C" void pfa_zpk_output_free(pfa_zpk_output *out)
+{
+    if (!out) return;
+    free(out->M);
+    free(out->Kn);
+    free(out->q);
+    free(out->pidx);
+    memset(out, 0, sizeof(*out));
+}
+
 extern "C" int pfa_reserve(int device, int64_t n_p, int64_t n_points, int h_max, char *err)
 {
     char *errbuf = err;

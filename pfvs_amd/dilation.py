@@ -192,3 +192,79 @@ def coni_dilation_bound_ceils(ps, kappa, Q, n_jobs: int = 1) -> np.ndarray:
     """ceil(coni_dilation_bound) per row of ps as int64 (0: no bound)."""
     return np.array([0 if b is None else math.ceil(b)
                      for b in coni_dilation_bounds(ps, kappa, Q, n_jobs)], dtype=np.int64)
+
+
+# ------------------------------------------------------------- routing
+def bound_routing(bounds, G, K, D0s, c: float = 0.0, shared: bool = False):
+    """Plan coniZp over one geometry's directions (pfvs/dilation.py:266-310):
+    a direction with bound b <= b* is searched by coniZpM to its bound (the
+    GPU); the others are split at D0, coniZpM up to D0 (the GPU) and coniZpK
+    above it (host threads).  G(D): ZpM's time per direction at dilation D
+    (increasing); K(D0): ZpK's time per direction above D0; c: the bound's
+    own time per direction.  The two sides run concurrently (the run time is
+    the larger), or one after the other when shared (no GPU).  Returns
+    (b_star, D0, time per direction); b_star = -inf splits every direction.
+
+    For a fixed D0, moving a direction from the split to its bound costs
+    G(b) - G(D0) of ZpM time and saves K(D0) of ZpK time; the cost grows
+    with b, so the best set to move is the smallest bounds, and every
+    prefix of the sorted bounds is tried."""
+    b = np.sort(np.asarray(bounds, dtype=float))
+    b = b[np.isfinite(b)]
+    n = max(len(b), 1)
+    best = (-math.inf, math.nan, math.inf)
+    for D0 in np.atleast_1d(D0s):
+        g0, k0 = float(G(D0)), float(K(D0))
+        moved = np.concatenate([[0.0], np.cumsum(np.asarray(G(b), dtype=float) - g0)])
+        t_zpm = g0 + moved / n
+        t_zpk = (len(b) - np.arange(len(b) + 1)) / n * k0 + c
+        t = t_zpm + t_zpk if shared else np.maximum(t_zpm, t_zpk)
+        i = int(np.argmin(t))
+        if t[i] < best[2]:
+            best = (float(b[i - 1]) if i else -math.inf, float(D0), float(t[i]))
+    return best
+
+
+class CostModel:
+    """Measured per-direction costs of coniZp's two searches on one
+    geometry (pfvs/dilation.py:313-379): ZpM's time G at increasing
+    dilations D (interpolated linearly in log-log, extended by the first
+    and last slopes), ZpK's time K above each split dilation of D0s, the
+    bound's time c, and whether the searches share the processors."""
+
+    def __init__(self, D, G, D0s, K, c: float = 0.0, shared: bool = False):
+        self.D = np.asarray(D, dtype=float)
+        g = np.maximum(np.asarray(G, dtype=float), 1e-12)
+        self.G = np.maximum.accumulate(g)              # routing assumes G increasing
+        self.D0s = [int(x) for x in D0s]
+        K = [float(x) for x in K]
+        if len(self.D) < 1 or len(self.G) != len(self.D) or np.any(np.diff(self.D) <= 0) \
+                or len(K) != len(self.D0s):
+            raise ValueError("CostModel: D must be increasing with one G each, one K per D0")
+        self.K = dict(zip(self.D0s, K))
+        self.c, self.shared = float(c), bool(shared)
+
+    def __repr__(self):
+        return (f"CostModel(D={self.D.tolist()}, G={self.G.tolist()}, D0s={self.D0s}, "
+                f"K={list(self.K.values())}, c={self.c}, shared={self.shared})")
+
+    def G_at(self, D):
+        """ZpM's time per direction at dilation(s) D."""
+        x = np.log(np.maximum(np.asarray(D, dtype=float), 1.0))
+        lx, ly = np.log(self.D), np.log(self.G)
+        if len(lx) == 1:
+            return np.exp(ly[0] + x - lx[0])            # proportional to D
+        y = np.interp(x, lx, ly)
+        s0 = (ly[1] - ly[0]) / (lx[1] - lx[0])
+        s1 = (ly[-1] - ly[-2]) / (lx[-1] - lx[-2])
+        y = np.where(x < lx[0], ly[0] + s0 * (x - lx[0]), y)
+        y = np.where(x > lx[-1], ly[-1] + s1 * (x - lx[-1]), y)
+        return np.exp(y)
+
+    def K_at(self, D0):
+        """ZpK's time per direction above D0 (one of D0s)."""
+        return self.K[int(D0)]
+
+    def route(self, bounds):
+        """bound_routing with these costs: (b_star, D0, time per direction)."""
+        return bound_routing(bounds, self.G_at, self.K_at, self.D0s, self.c, self.shared)

@@ -68,6 +68,8 @@ _KerIn = _struct("pfa_kernel_input", [
 _KerOut = _struct("pfa_kernel_output", [
     ("n", _c_i64), ("pts", _P32), ("qs", _P64), ("pidx", _P64), ("status", _P32),
     ("nodes", _P64), ("dmax", ctypes.c_int), ("err", ctypes.c_char * 256)])
+_ZpkOut = _struct("pfa_zpk_output", [
+    ("n", _c_i64), ("M", _P64), ("Kn", _P64), ("q", _P64), ("pidx", _P64)])
 _AsmOut = _struct("pfa_assemble_output", [
     ("n_rows", _c_i64), ("Ks", _P64), ("Ms", _P64), ("keys", _P64), ("done", _P8),
     ("err", ctypes.c_char * 256)])
@@ -104,6 +106,9 @@ SIGNATURES = {
                       ctypes.POINTER(_c_f64), _CP], _I),
     "pfa_reserve": ([_I, _c_i64, _c_i64, _I, _CP], _I),
     "pfa_coni_mu0": ([_I, _P64, _c_i64, _P64, _I, _P32, _P64, _P64, _P64, _CP], _I),
+    "pfa_coni_zpk": ([_I, _P64, _P64, _c_i64, _P64, _c_i64, _c_i64, _c_i64, _c_i64, _I, _P32,
+                      ctypes.POINTER(_ZpkOut), _CP], _I),
+    "pfa_zpk_output_free": ([ctypes.POINTER(_ZpkOut)], None),
     "pfa_device_cus": ([_I], _I),
     "pfa_device_clock_khz": ([_I], _I),
     "pfa_arch": ([], _CP),
@@ -418,6 +423,33 @@ def assemble_points(M, Kn, q, key, nonsing, h, Q, n_keys, threads=0):
                 _copy(out.done, int(n_keys), np.int8))
     finally:
         lib.pfa_assemble_output_free(ctypes.byref(out))
+
+
+def coni_zpk_points(kappa, Mbasis, p_full, Q, M0min, D0, max_out=10**7, threads=0):
+    """pfa_coni_zpk: the lattice points behind the coni PFVs with dilation
+    above D0 of each direction (rows of p_full, h entries, p[0] = 0), on
+    host threads.  Returns (M, Kn (n, h) int64, q, pidx, status per row);
+    a row with status != 0 contributes no points (1: no short-vector bound;
+    < 0: left to the caller's search to its dilation bound)."""
+    lib = require()
+    kappa = np.ascontiguousarray(kappa, dtype=np.int64)
+    Mbasis = np.ascontiguousarray(Mbasis, dtype=np.int64)
+    h = kappa.shape[0]
+    p_full = np.ascontiguousarray(np.atleast_2d(np.asarray(p_full, dtype=np.int64)).reshape(-1, h))
+    n_p = len(p_full)
+    st = np.zeros(max(n_p, 1), np.int32)
+    out = _ZpkOut()
+    err = ctypes.create_string_buffer(256)
+    rc = lib.pfa_coni_zpk(h, _ptr(kappa), _ptr(Mbasis), n_p, _ptr(p_full), int(Q), int(M0min),
+                          int(D0), int(max_out), int(threads), _ptr(st, _P32), ctypes.byref(out), err)
+    try:
+        if rc != 0:
+            raise ValueError(err.value.decode(errors="replace"))
+        n = int(out.n)
+        return (_copy(out.M, n * h, np.int64).reshape(n, h), _copy(out.Kn, n * h, np.int64).reshape(n, h),
+                _copy(out.q, n, np.int64), _copy(out.pidx, n, np.int64), st[:n_p])
+    finally:
+        lib.pfa_zpk_output_free(ctypes.byref(out))
 
 
 def box_enum(H, rhs, B, primitive=True, cap=10**8, device=0):
