@@ -292,9 +292,18 @@ static inline int pfa_bound_form(int h, const int64_t *kappa, const int64_t *p_r
             if (F.G[a * n + b] != F.G[b * n + a]) return -4;    // cannot happen (S symmetric)
     const int pd = pfa_bnd_pd(F.G, n);
     if (pd == 0) return 1;
-    if (pd < 0) return -1;
     rc = pfa_bnd_lll(F.G, n, F.V, F.Gr);
-    return rc ? (rc == -2 ? -2 : -1) : 0;
+    if (pd > 0) return rc ? (rc == -2 ? -2 : -1) : 0;
+    // the minors of the unreduced form left int128: decide on the reduced
+    // form Gr = V G V^T instead (V unimodular: the same definiteness); a
+    // reduction that met a nonpositive Gram-Schmidt value decides nothing.
+    // When the reduced form's minors leave int128 too (L large), the
+    // callers' FP64 factor of Gr decides: its proven margin phi < 1 gives
+    // c^T Gr c >= (1 - phi) |U c|^2 > 0 for c != 0, and a factor that fails
+    // leaves the direction undecided (never "no bound").
+    if (rc) return -1;
+    const int pd2 = pfa_bnd_pd(F.Gr, n);
+    return pd2 == 0 ? 1 : 0;
 }
 
 // K = O^T V^T y for a vector y in the reduced coordinates; 0 or -1 overflow

@@ -116,6 +116,24 @@ def test_library_and_exact_path_agree():
     assert decided > 100 and bounded > 30
 
 
+def test_library_decides_large_forms():
+    """Directions whose form L S has minors beyond int128 (h11 = 7 dataset
+    geometries, where the exact positive-definiteness test of the reduced
+    form overflows and the proven FP64 factor decides): the library decides
+    every one, and its mu0 equals the exact rational path's."""
+    gpu.require()
+    n = 0
+    for data, Q, ps in _cases(10, 7, 7, step=2):
+        kap = data.kappa_cob
+        st, num, den, K = dilation._mu0_lib(ps, kap)
+        assert np.all(st == 0)
+        for i in range(0, len(ps), 2):
+            ref = dilation._mu0_exact(ps[i], kap)
+            assert ref is not None and ref[0] == Fraction(int(num[i]), int(den[i])), ps[i]
+            n += 1
+    assert n >= 10
+
+
 def test_basis_change_leaves_mu0():
     """A signed permutation of the non-coni directions applied to kappa and
     p_hat leaves mu0 unchanged."""
