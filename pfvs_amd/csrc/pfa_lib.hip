@@ -4157,7 +4157,6 @@ struct ZpkPts {
     std::vector<int64_t> M, Kn, q, pidx;
 };
 
-static inline int zpk_mul(pfa_i128 a, pfa_i128 b, pfa_i128 *r) { return pfa_mul_ovf(a, b, r); }
 
 // floor(a / b), b > 0
 static inline pfa_i128 zpk_fdiv(pfa_i128 a, pfa_i128 b)
@@ -4237,8 +4236,8 @@ static int zpk_one(int h, const int64_t *kappa, const int64_t *Mbasis, const int
             for (int r = 0; r < rows; ++r) {
                 const pfa_i128 x = X[r * m + c1], y = X[r * m + c2];
                 pfa_i128 u, v, s1, s2;
-                if (zpk_mul(a11, x, &u) || zpk_mul(a21, y, &v) || pfa_add_ovf(u, v, &s1)) return 1;
-                if (zpk_mul(a12, x, &u) || zpk_mul(a22, y, &v) || pfa_add_ovf(u, v, &s2)) return 1;
+                if (pfa_mul_ovf(a11, x, &u) || pfa_mul_ovf(a21, y, &v) || pfa_add_ovf(u, v, &s1)) return 1;
+                if (pfa_mul_ovf(a12, x, &u) || pfa_mul_ovf(a22, y, &v) || pfa_add_ovf(u, v, &s2)) return 1;
                 X[r * m + c1] = s1;
                 X[r * m + c2] = s2;
             }
@@ -4276,14 +4275,14 @@ static int zpk_one(int h, const int64_t *kappa, const int64_t *Mbasis, const int
     // Hdet and Hadj = Hdet Hc^-1 (lower triangular), by forward substitution
     pfa_i128 Hdet = 1;
     for (int r = 0; r < n; ++r)
-        if (zpk_mul(Hdet, A[r * m + r], &Hdet)) return -48;
+        if (pfa_mul_ovf(Hdet, A[r * m + r], &Hdet)) return -48;
     std::vector<pfa_i128> Hadj((size_t)n * n, 0);
     for (int j = 0; j < n; ++j)
         for (int i = j; i < n; ++i) {
             pfa_i128 s = i == j ? Hdet : 0;
             for (int c = j; c < i; ++c) {
                 pfa_i128 t;
-                if (zpk_mul(A[i * m + c], Hadj[c * n + j], &t) || pfa_sub_ovf(s, t, &s)) return -48;
+                if (pfa_mul_ovf(A[i * m + c], Hadj[c * n + j], &t) || pfa_sub_ovf(s, t, &s)) return -48;
             }
             if (s % A[i * m + i] != 0) return -48;              // cannot happen (Hdet Hc^-1 is integral)
             Hadj[i * n + j] = s / A[i * m + i];
@@ -4295,11 +4294,11 @@ static int zpk_one(int h, const int64_t *kappa, const int64_t *Mbasis, const int
     const pfa_i128 alk = lk < 0 ? -lk : lk;
     // the short vectors: x^T Gr x <= (Q L - 1) / D0, through the search
     pfa_i128 QL, coefL;
-    if (zpk_mul((pfa_i128)Q, (pfa_i128)F.L, &QL)) return -50;
+    if (pfa_mul_ovf((pfa_i128)Q, (pfa_i128)F.L, &QL)) return -50;
     const pfa_i128 qs = (QL - 1) / D0;
     if (qs < 1) return 0;
     if (!pfa_fits64(qs)) return -50;
-    if (zpk_mul((pfa_i128)F.L, F.sn, &coefL)) return -50;      // L sn (times g below)
+    if (pfa_mul_ovf((pfa_i128)F.L, F.sn, &coefL)) return -50;      // L sn (times g below)
     const RecL L = rec_layout(n);
     std::vector<int64_t> rec((size_t)L.words, 0);
     int64_t *r = rec.data();
@@ -4329,7 +4328,7 @@ static int zpk_one(int h, const int64_t *kappa, const int64_t *Mbasis, const int
     int bad = 0;
     // the points of one short vector x (q = x^T Gr x)
     auto per_K = [&](const int32_t *x, int64_t val) -> int {
-        pfa_i128 z[PFA_DMAX], Kp[PFA_DMAX], u[PFA_DMAX], c1[PFA_DMAX];
+        pfa_i128 z[PFA_DMAX], u[PFA_DMAX], c1[PFA_DMAX];
         pfa_u128 gu = 0;
         for (int a = 0; a < n; ++a) {                           // Lambda coordinates V^T x
             pfa_i128 s = 0;
@@ -4340,7 +4339,7 @@ static int zpk_one(int h, const int64_t *kappa, const int64_t *Mbasis, const int
         }
         const pfa_i128 g = (pfa_i128)gu;
         pfa_i128 g2;
-        if (g <= 0 || zpk_mul(g, g, &g2) || val % g2 != 0) return -61;
+        if (g <= 0 || pfa_mul_ovf(g, g, &g2) || val % g2 != 0) return -61;
         const pfa_i128 kn = val / g2;
         for (int a = 0; a < n; ++a) {                           // y': K' in Lambda coordinates
             z[a] /= g;
@@ -4351,7 +4350,7 @@ static int zpk_one(int h, const int64_t *kappa, const int64_t *Mbasis, const int
             pfa_i128 s = 0;
             for (int c = 0; c <= i; ++c) {
                 pfa_i128 t;
-                if (zpk_mul(Hadj[i * n + c], z[c], &t) || pfa_add_ovf(s, t, &s)) return -62;
+                if (pfa_mul_ovf(Hadj[i * n + c], z[c], &t) || pfa_add_ovf(s, t, &s)) return -62;
             }
             u[i] = s;
             og = pfa_gcd128(og, pfa_abs128(s));
@@ -4361,7 +4360,7 @@ static int zpk_one(int h, const int64_t *kappa, const int64_t *Mbasis, const int
             pfa_i128 s = 0;
             for (int a = 0; a < n; ++a) {
                 pfa_i128 t;
-                if (zpk_mul(E[i * m + a], u[a] / (pfa_i128)og, &t) || pfa_add_ovf(s, t, &s)) return -62;
+                if (pfa_mul_ovf(E[i * m + a], u[a] / (pfa_i128)og, &t) || pfa_add_ovf(s, t, &s)) return -62;
             }
             c1[i] = s;
         }
@@ -4371,21 +4370,20 @@ static int zpk_one(int h, const int64_t *kappa, const int64_t *Mbasis, const int
                 if (pfa_mac64(&s, F.O[a * m + i], (int64_t)z[a])) return -62;
             for (int c = 0; c < m; ++c) {
                 pfa_i128 e;
-                if (zpk_mul((pfa_i128)ZB[(i + 1) * m + c], c1[c], &e) || pfa_add_ovf(t, e, &t)) return -62;
+                if (pfa_mul_ovf((pfa_i128)ZB[(i + 1) * m + c], c1[c], &e) || pfa_add_ovf(t, e, &t)) return -62;
             }
             pfa_i128 oK;
-            if (zpk_mul(o, s, &oK) || oK != t) return -63;      // cannot happen
-            Kp[i] = s;
+            if (pfa_mul_ovf(o, s, &oK) || oK != t) return -63;      // cannot happen
         }
         pfa_i128 lc1 = 0;
         for (int c = 0; c < m; ++c) {
             pfa_i128 t;
-            if (zpk_mul((pfa_i128)Bi[c], c1[c], &t) || pfa_add_ovf(lc1, t, &lc1)) return -62;
+            if (pfa_mul_ovf((pfa_i128)Bi[c], c1[c], &t) || pfa_add_ovf(lc1, t, &lc1)) return -62;
         }
         // j: j o > g D0 and g j o kn < Q L
         pfa_i128 go, gokn, gD0, coef, sdo;
-        if (zpk_mul(g, o, &go) || zpk_mul(go, kn, &gokn) || zpk_mul(g, (pfa_i128)D0, &gD0) ||
-            zpk_mul(g, coefL, &coef) || zpk_mul(F.sd, o, &sdo))
+        if (pfa_mul_ovf(g, o, &go) || pfa_mul_ovf(go, kn, &gokn) || pfa_mul_ovf(g, (pfa_i128)D0, &gD0) ||
+            pfa_mul_ovf(g, coefL, &coef) || pfa_mul_ovf(F.sd, o, &sdo))
             return -64;
         const pfa_i128 jlo = zpk_fdiv(gD0, o) + 1, jhi = (QL - 1) / gokn;
         if (jlo > jhi) return 0;
@@ -4393,24 +4391,24 @@ static int zpk_one(int h, const int64_t *kappa, const int64_t *Mbasis, const int
         const double coef_d = (double)coef, sdo_d = (double)sdo;
         for (pfa_i128 j = jlo; j <= jhi; ++j) {
             pfa_i128 diff, R, jl;
-            if (zpk_mul(gokn, j, &diff) || pfa_sub_ovf(QL, diff, &diff)) return -64;
-            if (zpk_mul(j, lc1, &jl)) return -64;
+            if (pfa_mul_ovf(gokn, j, &diff) || pfa_sub_ovf(QL, diff, &diff)) return -64;
+            if (pfa_mul_ovf(j, lc1, &jl)) return -64;
             // the smallest M0 >= M0min with M0 = j lc1 (mod |lk|)
             pfa_i128 off = (jl - (pfa_i128)M0min) % alk;
             if (off < 0) off += alk;
             pfa_i128 M0 = (pfa_i128)M0min + off;
             const double M0d = (double)M0;
             if (M0d * M0d * coef_d > sdo_d * (double)j * (double)diff * (1 + 1e-9) + 1.0) continue;
-            if (zpk_mul(sdo, j, &R) || zpk_mul(R, diff, &R)) return -64;
+            if (pfa_mul_ovf(sdo, j, &R) || pfa_mul_ovf(R, diff, &R)) return -64;
             for (;; M0 += alk) {
                 pfa_i128 lhs;
-                if (zpk_mul(M0, M0, &lhs) || zpk_mul(lhs, coef, &lhs) || lhs >= R) break;
+                if (pfa_mul_ovf(M0, M0, &lhs) || pfa_mul_ovf(lhs, coef, &lhs) || lhs >= R) break;
                 const pfa_i128 tt = (M0 - jl) / lk;             // exact: M0 = j lc1 (mod lk)
                 if ((int64_t)(out.q.size() - n0) >= room) return -66;
                 int64_t c[PFA_DMAX];
                 for (int i = 0; i < m; ++i) {
                     pfa_i128 a, b;
-                    if (zpk_mul(j, c1[i], &a) || zpk_mul(tt, (pfa_i128)k[i], &b) || pfa_add_ovf(a, b, &a) ||
+                    if (pfa_mul_ovf(j, c1[i], &a) || pfa_mul_ovf(tt, (pfa_i128)k[i], &b) || pfa_add_ovf(a, b, &a) ||
                         !pfa_fits64(a))
                         return -67;
                     c[i] = (int64_t)a;
@@ -4430,7 +4428,7 @@ static int zpk_one(int h, const int64_t *kappa, const int64_t *Mbasis, const int
                     pfa_i128 s = 0, t;
                     for (int e = 0; e < m; ++e)
                         if (pfa_mac64(&s, mat[i * m + e], c[e])) return -67;
-                    if (zpk_mul(s, (pfa_i128)c[i], &t) || pfa_add_ovf(q, t, &q)) return -67;
+                    if (pfa_mul_ovf(s, (pfa_i128)c[i], &t) || pfa_add_ovf(q, t, &q)) return -67;
                 }
                 if (!pfa_fits64(q)) return -67;
                 out.M.insert(out.M.end(), Mr, Mr + h);
@@ -4439,7 +4437,6 @@ static int zpk_one(int h, const int64_t *kappa, const int64_t *Mbasis, const int
                 out.pidx.push_back(pidx);
             }
         }
-        (void)Kp;
         return 0;
     };
     auto emit = [&](const int32_t *x, int64_t q, uint64_t) -> int {
