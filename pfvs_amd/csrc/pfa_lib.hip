@@ -41,6 +41,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <type_traits>
 #include <numeric>
 #include <thread>
 #include <string>
@@ -1639,20 +1640,27 @@ __device__ __forceinline__ double col_f64(const int32_t *kt, const double *gMd, 
     return w;
 }
 
-// SP: the launch has geometries with a sparse form of kappa (a separate
-// instantiation, so the dense-only kernel keeps its registers)
-// f(0), f(1), ..., f(k - 1) with constant arguments, as nested tests (a
-// step k evaluates k + 1 of them)
-template <int T, int NMAX, class F>
-__device__ __forceinline__ void det_chain(int k, F &f)
+// f(T), f(T + 1), ..., f(K - 1) with constant arguments
+template <int T, int K, class F>
+__device__ __forceinline__ void static_for(F &f)
 {
-    if constexpr (T < NMAX) {
-        if (T < k) {
-            f(T);
-            det_chain<T + 1, NMAX>(k, f);
-        }
+    if constexpr (T < K) {
+        f(T);
+        static_for<T + 1, K>(f);
     }
 }
+
+// f(integral_constant<K>), f(<K + 1>), ... while f returns true, below N
+template <int K, int N, class F>
+__device__ __forceinline__ void static_steps(F &f)
+{
+    if constexpr (K < N) {
+        if (f(std::integral_constant<int, K>{})) static_steps<K + 1, N>(f);
+    }
+}
+
+// SP: the launch has geometries with a sparse form of kappa (a separate
+// instantiation, so the dense-only kernel keeps its registers)
 
 template <int NMAX, bool SP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_filter(FiltArgs a)
@@ -1936,54 +1944,54 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         int ustep = 255;                          // the step this row was the pivot of
         double prev = 1.0;
         double pc[NMAX];
-        // (the step loop stays rolled; the loops over t run to NMAX with
-        // wave-uniform predicates so every index of pc is a constant)
-        for (int k = 0; k < nlmax; ++k) {
+        // one step: column k built, the steps t < k applied, the pivot
+        // chosen; false once no group of the wavefront runs.  Up to NMAX 20
+        // the steps are unrolled with k a constant (pc[k] a plain register,
+        // the updates a fixed sequence: the first steps, where rank-r points
+        // stop after r + 1, stay short); beyond, the loop stays rolled and
+        // the loops over t run to NMAX with wave-uniform predicates so every
+        // index of pc is a constant (a chain of tests would cost occupancy).
+        auto step = [&](auto KC) -> bool {
+            constexpr bool CK = !std::is_same<decltype(KC), int>::value;
+            int k;
+            if constexpr (CK) k = decltype(KC)::value;
+            else k = KC;
+            if (k >= nlmax) return false;
             if (run && k >= n) { run = false; res = 1; }
-            if (!__ballot(run)) break;
+            if (!__ballot(run)) return false;
             double v = build(k);
             int okb = 1;
-            // the steps t < k: a chain of k + 1 wave-uniform tests up to NMAX
-            // 20; beyond, one test per t (the chain's registers would cost
-            // occupancy there)
-            if constexpr (NMAX <= 20) {
-                auto upd = [&](int t) {
-                    const int pt = s_p[t];
-                    const double2 e = s_e[t];
-                    const double cp = __shfl(v, L.inlane ? L.src0 + pt : L.lane);
-                    if (run && i < n && ustep > t) {
-                        const double x1 = v * e.x, x2 = pc[t] * cp, x = x1 - x2, y = x * e.y;
-                        // |x1|, |x2|, |x| < 2^53 and |y| < 2^50 (8|y| is exact)
-                        const double m = fmax(fmax(fabs(x1), fabs(x2)), fmax(fabs(x), 8.0 * fabs(y)));
-                        okb &= (int)(m < PFA_TWO53);
-                        v = rint(y);
-                    }
-                };
-                det_chain<0, NMAX>(k, upd);
+            auto upd = [&](int t) {
+                const int pt = s_p[t];
+                const double2 e = s_e[t];
+                const double cp = __shfl(v, L.inlane ? L.src0 + pt : L.lane);
+                if (run && i < n && ustep > t) {
+                    const double x1 = v * e.x, x2 = pc[t] * cp, x = x1 - x2, y = x * e.y;
+                    // |x1|, |x2|, |x| < 2^53 and |y| < 2^50 (8|y| is exact)
+                    const double m = fmax(fmax(fabs(x1), fabs(x2)), fmax(fabs(x), 8.0 * fabs(y)));
+                    okb &= (int)(m < PFA_TWO53);
+                    v = rint(y);
+                }
+            };
+            if constexpr (CK) {
+                static_for<0, decltype(KC)::value>(upd);
             } else {
 #pragma unroll
-                for (int t = 0; t < NMAX; ++t) {
-                    if (t < k) {                  // wave-uniform
-                        const int pt = s_p[t];
-                        const double2 e = s_e[t];
-                        const double cp = __shfl(v, L.inlane ? L.src0 + pt : L.lane);
-                        if (run && i < n && ustep > t) {
-                            const double x1 = v * e.x, x2 = pc[t] * cp, x = x1 - x2, y = x * e.y;
-                            const double m = fmax(fmax(fabs(x1), fabs(x2)), fmax(fabs(x), 8.0 * fabs(y)));
-                            okb &= (int)(m < PFA_TWO53);
-                            v = rint(y);
-                        }
-                    }
-                }
+                for (int t = 0; t < NMAX; ++t)
+                    if (t < k) upd(t);            // wave-uniform
             }
             if (run && (__ballot(!okb) & L.gm)) { run = false; trip = true; }
             const uint64_t cm = __ballot(run && i < n && ustep > k && v != 0.0) & L.gm;
             const int pl = cm ? (int)__builtin_ctzll(cm) : L.lane;
             const double pk = __shfl(v, pl);
             if (run && !cm) { run = false; res = 0; }
+            if constexpr (CK) {
+                pc[decltype(KC)::value] = v;
+            } else {
 #pragma unroll
-            for (int t = 0; t < NMAX; ++t)
-                if (t == k) pc[t] = v;
+                for (int t = 0; t < NMAX; ++t)
+                    if (t == k) pc[t] = v;
+            }
             if (run) {
                 if (L.lane == pl) ustep = k;
                 if (i == 0) {
@@ -1993,6 +2001,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 prev = pk;
             }
             wave_lds_sync();
+            return true;
+        };
+        if constexpr (NMAX <= 20) {
+            static_steps<0, NMAX>(step);
+        } else {
+            for (int k = 0; k < nlmax; ++k)
+                if (!step(k)) break;
         }
         if (run) res = 1;
 
