@@ -1650,12 +1650,16 @@ __device__ __forceinline__ void static_for(F &f)
     }
 }
 
-// f(integral_constant<K>), f(<K + 1>), ... while f returns true, below N
+// f(integral_constant<K>), f(<K + 1>), ... while f returns true, below N;
+// true if every call returned true
 template <int K, int N, class F>
-__device__ __forceinline__ void static_steps(F &f)
+__device__ __forceinline__ bool static_steps(F &f)
 {
     if constexpr (K < N) {
-        if (f(std::integral_constant<int, K>{})) static_steps<K + 1, N>(f);
+        if (!f(std::integral_constant<int, K>{})) return false;
+        return static_steps<K + 1, N>(f);
+    } else {
+        return true;
     }
 }
 
@@ -1945,12 +1949,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         double prev = 1.0;
         double pc[NMAX];
         // one step: column k built, the steps t < k applied, the pivot
-        // chosen; false once no group of the wavefront runs.  Up to NMAX 20
-        // the steps are unrolled with k a constant (pc[k] a plain register,
-        // the updates a fixed sequence: the first steps, where rank-r points
-        // stop after r + 1, stay short); beyond, the loop stays rolled and
-        // the loops over t run to NMAX with wave-uniform predicates so every
-        // index of pc is a constant (a chain of tests would cost occupancy).
+        // chosen; false once no group of the wavefront runs.  The first steps
+        // (all of them up to NMAX 20, the first 6 beyond) are unrolled with k
+        // a constant: pc[k] a plain register, the updates a fixed sequence
+        // (rank-r points stop after r + 1 steps).  The rest of the loop stays
+        // rolled, its loops over t running to NMAX with wave-uniform
+        // predicates so every index of pc is a constant.
         auto step = [&](auto KC) -> bool {
             constexpr bool CK = !std::is_same<decltype(KC), int>::value;
             int k;
@@ -2003,11 +2007,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             wave_lds_sync();
             return true;
         };
-        if constexpr (NMAX <= 20) {
-            static_steps<0, NMAX>(step);
-        } else {
-            for (int k = 0; k < nlmax; ++k)
-                if (!step(k)) break;
+        constexpr int NU = NMAX <= 20 ? NMAX : 6;
+        if (static_steps<0, NU>(step)) {
+            if constexpr (NU < NMAX) {
+                for (int k = NU; k < nlmax; ++k)
+                    if (!step(k)) break;
+            }
         }
         if (run) res = 1;
 
