@@ -1683,7 +1683,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     // per wavefront: Binter of one p-vector as int32 (NMAX <= 24), the
     // p-vector staged (bp, wave-uniform) and its max|Binter|
     constexpr int BSZ = NMAX <= 24 ? (NMAX + 1) * NMAX : 1;
-    __shared__ int32_t s_B[4 * BSZ];
+    __shared__ __align__(16) int32_t s_B[4 * BSZ];   // rows at stride NMAX (a multiple of 4)
+    // per wavefront: each group's c at a stride of hl rounded up to 4, read
+    // four entries at a time (at most 64 / hl groups: 128 + NMAX entries)
+    constexpr int CSZ = 128 + NMAX;
+    __shared__ __align__(16) int32_t s_cg[4 * CSZ];
     const DetLanes L = det_lanes(a.hl);
     const int R = L.R, i = L.j, nlmax = a.hl - 1;
     const int wv = threadIdx.x >> 6;
@@ -1823,7 +1827,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     const int64_t b0 = r0[14];
                     wave_lds_sync();                       // the old copy's reads
                     if (b0 < ((int64_t)1 << 31) && h0 >= 2 && h0 - 1 <= NMAX) {
-                        for (int e = L.lane; e < h0 * (h0 - 1); e += 64) sB[e] = (int32_t)r0[a.L.Bi + e];
+                        for (int e = L.lane; e < h0 * (h0 - 1); e += 64) {
+                            const int rw = e / (h0 - 1);
+                            sB[rw * NMAX + (e - rw * (h0 - 1))] = (int32_t)r0[a.L.Bi + e];
+                        }
                         bp = p0;
                         bB = b0;
                     } else {
@@ -1835,31 +1842,50 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             }
             const int64_t bmax = lb ? bB : rec[14];
             const bool r32 = row && bmax < ((int64_t)1 << 31);
+            // the group's c through LDS, four entries a read (entries at or
+            // beyond n are multiplied by 0)
+            const int hlp = (a.hl + 3) & ~3;
+            int32_t *cg = s_cg + wv * CSZ + (L.inlane ? L.r : 0) * hlp;
+            wave_lds_sync();
+            if (L.inlane && i < hlp) cg[i] = cv;
+            wave_lds_sync();
             double acc = 0.0, csum = 0.0;
             if (!__ballot(row && !lb)) {
                 // every row from LDS, all NMAX entries read unconditionally
                 // where the slice exists (NMAX <= 24: inside it; cd = 0 adds
                 // nothing, and a lane without a row does not use its sum)
-                const int32_t *br = sB + (i < h ? i : 0) * n;
-#pragma unroll 4
-                for (int t = 0; t < NMAX; ++t) {
-                    const int32_t ct = __shfl(cv, L.src0 + t);
-                    const double cd = t < n ? (double)ct : 0.0;
-                    csum += fabs(cd);
-                    if constexpr (NMAX <= 24) acc = fma((double)br[t], cd, acc);
-                    else if (row && t < n) acc = fma((double)br[t], cd, acc);   // (no LDS slice: lb never holds)
+                const int32_t *br = sB + (i < h ? i : 0) * NMAX;
+#pragma unroll 1
+                for (int t0 = 0; t0 < NMAX; t0 += 4) {
+                    const int4 c4 = *(const int4 *)(cg + t0);
+                    int4 b4 = make_int4(0, 0, 0, 0);
+                    if constexpr (NMAX <= 24) b4 = *(const int4 *)(br + t0);
+                    const int32_t cc[4] = {c4.x, c4.y, c4.z, c4.w}, bb[4] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int t = t0 + u;
+                        const double cd = t < n ? (double)cc[u] : 0.0;
+                        csum += fabs(cd);
+                        if constexpr (NMAX <= 24) acc = fma((double)bb[u], cd, acc);
+                        else if (row && t < n) acc = fma((double)br[t], cd, acc);   // (no LDS slice: lb never holds)
+                    }
                 }
             } else {
                 // every entry read, unconditionally, so the loads issue
                 // together (cd = 0 adds nothing; a lane without a row reads
                 // its record's first words, and its sum is not used)
                 const int32_t *bq = row ? b32 : (const int32_t *)rec;
-#pragma unroll 4
-                for (int t = 0; t < NMAX; ++t) {
-                    const int32_t ct = __shfl(cv, L.src0 + t);
-                    const double cd = t < n ? (double)ct : 0.0;
-                    csum += fabs(cd);
-                    acc = fma((double)bq[2 * t], cd, acc);
+#pragma unroll 1
+                for (int t0 = 0; t0 < NMAX; t0 += 4) {
+                    const int4 c4 = *(const int4 *)(cg + t0);
+                    const int32_t cc[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int t = t0 + u;
+                        const double cd = t < n ? (double)cc[u] : 0.0;
+                        csum += fabs(cd);
+                        acc = fma((double)bq[2 * t], cd, acc);
+                    }
                 }
             }
             const bool fast = r32 && (double)bmax * csum < PFA_TWO53;
