@@ -1685,8 +1685,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     constexpr int BSZ = NMAX <= 24 ? (NMAX + 1) * NMAX : 1;
     __shared__ __align__(16) int32_t s_B[4 * BSZ];   // rows at stride NMAX (a multiple of 4)
     // per wavefront: each group's c at a stride of hl rounded up to 4, read
-    // four entries at a time (at most 64 / hl groups: 128 + NMAX entries)
-    constexpr int CSZ = 128 + NMAX;
+    // four entries at a time (at most 64 / hl groups: 128 + NMAX entries);
+    // later each group's M, zero-padded at a stride of NMAX + 4 (a launch of
+    // class NMAX has hl > NMAX - 3 unless NMAX = 8: at most 3 x 24, 4 x 20,
+    // 6 x 16, 2 x 28 entries, and 32 x 12 for NMAX = 8)
+    constexpr int CSZ = NMAX == 8 ? 384 : 128 + NMAX;
     __shared__ __align__(16) int32_t s_cg[4 * CSZ];
     const DetLanes L = det_lanes(a.hl);
     const int R = L.R, i = L.j, nlmax = a.hl - 1;
@@ -1935,12 +1938,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
         for (int k = 0; k < MKN; ++k) mk[k] = 0;
         if constexpr (NMAX <= 24) {
+            // the group's M zero-padded to NMAX + 4 entries (lanes at or
+            // beyond h hold M = 0), read four entries at a time
             if (form == 0) {
+                constexpr int MS = NMAX + 4;
+                int32_t *gz = s_cg + wv * CSZ + (L.inlane ? L.r : 0) * MS;
+                if (L.inlane) {
+                    gz[i] = (int32_t)Mi;
+                    for (int e = i; e < MS - a.hl; e += a.hl) gz[a.hl + e] = 0;
+                }
+                wave_lds_sync();
 #pragma unroll
-                for (int k = 0; k <= NMAX; ++k) {
-                    // (every read issued: the index clamped, the value masked)
-                    const int32_t x = gM[k <= n ? k : 0];
-                    mk[k] = k <= n ? x : 0;
+                for (int k0 = 0; k0 <= NMAX; k0 += 4) {
+                    const int4 x4 = *(const int4 *)(gz + k0);
+                    const int32_t xs[4] = {x4.x, x4.y, x4.z, x4.w};
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (k0 + u <= NMAX) mk[k0 + u] = xs[u];
                 }
             }
         }
