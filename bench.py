@@ -25,7 +25,13 @@ same nodes (asserted):
 (k = d - 1 - i coordinates set above a node at level i, r = 1 where an H
 row leads at column i, s = binary-gcd iterations), and divided by the
 stage's duration from HIP events on the stream it runs on; the peak is
-CUs x 64 lanes x peak clock.
+CUs x 64 lanes x peak clock.  `prep_roofline` puts the preparation (k_prep)
+against the same peak: its lane-ops, counted on the same host run (one per
+element operation of the certificate, the two LLLs, the products, the HNF
+and the factor), over the stage's duration, with the largest p-vector's
+count beside them (one wavefront's latency bounds that stage).
+--select I,J keeps only those p-vectors of the workload (for profiling
+single preparations).
 """
 
 import argparse
