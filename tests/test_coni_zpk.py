@@ -14,7 +14,9 @@ checks them:
   bound_routing's choice is the brute-force optimum over every threshold;
 - coniZp with a cost model that splits directions returns the same PFVs
   as coniZp searching every direction to its bound (host; on the GPU in
-  the gpu-marked test)."""
+  the gpu-marked test);
+- the library's points do not depend on its thread count, and a direction
+  alone gives its slice of the batch."""
 
 import math
 
@@ -148,3 +150,23 @@ def test_gpu_coniZp_split_matches_the_host(need_gpu):
             assert np.array_equal(Ks, Kc) and np.array_equal(Ms, Mc)
         n += len(Kc)
     assert n > 20
+
+
+def test_zpk_points_do_not_depend_on_threads():
+    """pfa_coni_zpk gathers each direction's points and concatenates them in
+    direction order: the same arrays on 1 and 4 host threads, and a
+    direction alone gives its own slice of the batch."""
+    gpu.require()
+    data, Q, ps = CASES[2]
+    p_full = np.hstack([np.zeros((len(ps), 1), np.int64), ps.astype(np.int64)])
+    one = gpu.coni_zpk_points(data.kappa_cob, data.M_lattice(), p_full, Q, 13, 2, threads=1)
+    four = gpu.coni_zpk_points(data.kappa_cob, data.M_lattice(), p_full, Q, 13, 2, threads=4)
+    for a, b in zip(one, four):
+        assert np.array_equal(a, b)
+    M, Kn, q, pidx, st = one
+    assert np.all(np.diff(pidx) >= 0) and len(q) > 0
+    k = int(pidx[len(pidx) // 2])
+    Ma, Kna, qa, pa, sa = gpu.coni_zpk_points(data.kappa_cob, data.M_lattice(), p_full[k:k + 1], Q, 13, 2)
+    sel = pidx == k
+    assert sa[0] == st[k] and np.array_equal(Ma, M[sel]) and np.array_equal(Kna, Kn[sel])
+    assert np.array_equal(qa, q[sel]) and np.all(pa == 0)
